@@ -1,0 +1,274 @@
+"""Python handle on libdpa_hip.so (the C ABI in include/dpa_hip.h).
+
+This module is plumbing for tests and the benchmark: it binds the C ABI with
+ctypes and moves numpy arrays across it.  All computation happens in the HIP
+kernels behind the ABI.  There is no fallback: if the shared library is missing
+or no GPU is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libdpa_hip.so"
+HEADER = HERE.parent / "include" / "dpa_hip.h"
+
+OK = 0
+ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_ADMISSION, ERR_CANCELLED, ERR_UNSUPPORTED, ERR_TIMEOUT, ERR_PENDING = range(1, 9)
+INTEGRATOR_DPA_RK4 = 0
+INTEGRATOR_RK4 = 1
+STATUS_ACTIVE, STATUS_COMPLETED, STATUS_EXITED, STATUS_ZERO = range(4)
+
+_u64p = C.POINTER(C.c_uint64)
+_f32p = C.POINTER(C.c_float)
+
+
+class DpahError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dpah error {code}: {msg}")
+        self.code = code
+
+
+class _Options(C.Structure):
+    _fields_ = [("memory_budget_bytes", C.c_uint64), ("slot_budget_bytes", C.c_uint64),
+                ("kernel_timing", C.c_int), ("autotune_cache_path", C.c_char_p)]
+
+
+class _AdvectParams(C.Structure):
+    _fields_ = [("particles_per_round", C.c_uint64), ("step_size", C.c_float),
+                ("start_time", C.c_float), ("integrator", C.c_int), ("record", C.c_int),
+                ("use_64_bit_indices", C.c_int)]
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Loads libdpa_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m dpa_amd.build` "
+                           "(or __graft_entry__.build())")
+    lib = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    sig = {
+        "dpah_last_error": (C.c_char_p, []),
+        "dpah_version": (C.c_int, []),
+        "dpah_create": (C.c_int, [C.c_int, C.POINTER(_Options), C.POINTER(vp)]),
+        "dpah_destroy": (C.c_int, [vp]),
+        "dpah_synchronize": (C.c_int, [vp]),
+        "dpah_device_alloc": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
+        "dpah_device_free": (C.c_int, [vp, vp]),
+        "dpah_copy_to_device": (C.c_int, [vp, vp, vp, C.c_uint64]),
+        "dpah_copy_to_host": (C.c_int, [vp, vp, vp, C.c_uint64]),
+        "dpah_field_create": (C.c_int, [vp, vp, _u64p, _f32p, _f32p, C.POINTER(vp)]),
+        "dpah_field_create_unsteady": (C.c_int, [vp, vp, _u64p, C.c_uint64, _f32p, _f32p,
+                                                 C.c_float, C.c_float, C.POINTER(vp)]),
+        "dpah_field_release": (C.c_int, [vp]),
+        "dpah_advect": (C.c_int, [vp, vp, vp, vp, C.c_uint64, C.c_uint64,
+                                  C.POINTER(_AdvectParams), C.POINTER(vp)]),
+        "dpah_result_round_count": (C.c_int, [vp, _u64p]),
+        "dpah_result_round_sizes": (C.c_int, [vp, C.c_uint64, _u64p, _u64p, C.POINTER(C.c_int)]),
+        "dpah_result_copy_round": (C.c_int, [vp, C.c_uint64, vp, vp, vp]),
+        "dpah_result_final_state": (C.c_int, [vp, vp, vp, vp]),
+        "dpah_result_total_steps": (C.c_int, [vp, _u64p]),
+        "dpah_result_free": (C.c_int, [vp]),
+        "dpah_advect_endpoints_device": (C.c_int, [vp, vp, vp, vp, C.c_uint64, C.c_uint64,
+                                                   C.c_float, C.c_float, C.c_int, vp, vp, vp,
+                                                   _u64p]),
+        "dpah_kernel_timing": (C.c_int, [vp, C.c_char_p, _u64p, C.POINTER(C.c_double)]),
+        "dpah_kernel_timing_reset": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def declared_symbols() -> list[str]:
+    """Function names declared in include/dpa_hip.h."""
+    import re
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dpah_\w+)\s*\(", text, re.M)))
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        msg = load_library().dpah_last_error().decode(errors="replace")
+        raise DpahError(rc, msg)
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f3(v) -> C.Array:
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def _u3(v) -> C.Array:
+    return (C.c_uint64 * 3)(*[int(x) for x in v])
+
+
+class Context:
+    """A dpah_context on one GPU."""
+
+    def __init__(self, device: int = 0, memory_budget: int = 0, slot_budget: int = 0,
+                 kernel_timing: bool = False):
+        self.lib = load_library()
+        opt = _Options(memory_budget, slot_budget, 1 if kernel_timing else 0, None)
+        h = C.c_void_p()
+        _check(self.lib.dpah_create(device, C.byref(opt), C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            _check(self.lib.dpah_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self) -> None:
+        _check(self.lib.dpah_synchronize(self.h))
+
+    # device memory
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        _check(self.lib.dpah_device_alloc(self.h, nbytes, C.byref(p)))
+        return p.value
+
+    def free(self, ptr: int) -> None:
+        _check(self.lib.dpah_device_free(self.h, C.c_void_p(ptr)))
+
+    def to_device(self, arr: np.ndarray) -> int:
+        arr = np.ascontiguousarray(arr)
+        p = self.alloc(max(arr.nbytes, 1))
+        _check(self.lib.dpah_copy_to_device(self.h, C.c_void_p(p), _ptr(arr), arr.nbytes))
+        return p
+
+    def to_host(self, ptr: int, out: np.ndarray) -> np.ndarray:
+        _check(self.lib.dpah_copy_to_host(self.h, _ptr(out), C.c_void_p(ptr), out.nbytes))
+        return out
+
+    def field(self, data: np.ndarray, offset=(0, 0, 0), spacing=(1, 1, 1)) -> "Field":
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        assert data.ndim == 4 and data.shape[3] == 3, "field must be [n0][n1][n2][3]"
+        h = C.c_void_p()
+        _check(self.lib.dpah_field_create(self.h, _ptr(data), _u3(data.shape[:3]), _f3(offset),
+                                          _f3(spacing), C.byref(h)))
+        return Field(self, h, data.shape[:3], 1)
+
+    def field_unsteady(self, data: np.ndarray, offset=(0, 0, 0), spacing=(1, 1, 1),
+                       time_offset=0.0, time_spacing=1.0) -> "Field":
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        assert data.ndim == 5 and data.shape[4] == 3, "field must be [nt][n0][n1][n2][3]"
+        h = C.c_void_p()
+        _check(self.lib.dpah_field_create_unsteady(
+            self.h, _ptr(data), _u3(data.shape[1:4]), data.shape[0], _f3(offset), _f3(spacing),
+            float(time_offset), float(time_spacing), C.byref(h)))
+        return Field(self, h, data.shape[1:4], data.shape[0])
+
+    def advect(self, field: "Field", seeds: np.ndarray, remaining=None, uniform_remaining=0,
+               particles_per_round=None, step_size=0.1, start_time=0.0,
+               integrator=INTEGRATOR_DPA_RK4, record=True, use_64_bit=False) -> "Result":
+        seeds = np.ascontiguousarray(seeds, dtype=np.float32).reshape(-1, 3)
+        n = seeds.shape[0]
+        rem = None if remaining is None else np.ascontiguousarray(remaining, dtype=np.uint64)
+        ppr = n if particles_per_round is None else particles_per_round
+        p = _AdvectParams(int(ppr), float(step_size), float(start_time), int(integrator),
+                          1 if record else 0, 1 if use_64_bit else 0)
+        h = C.c_void_p()
+        _check(self.lib.dpah_advect(self.h, field.h, _ptr(seeds), _ptr(rem),
+                                    int(uniform_remaining), n, C.byref(p), C.byref(h)))
+        return Result(self, h, n)
+
+    def advect_endpoints_device(self, field: "Field", d_seeds: int, count: int, d_pos_out: int,
+                                d_remaining: int | None = None, uniform_remaining: int = 0,
+                                step_size: float = 0.1, start_time: float = 0.0,
+                                integrator: int = INTEGRATOR_DPA_RK4,
+                                d_rem_out: int | None = None, d_status_out: int | None = None,
+                                wait: bool = True) -> int | None:
+        steps = C.c_uint64()
+        _check(self.lib.dpah_advect_endpoints_device(
+            self.h, field.h, C.c_void_p(d_seeds), C.c_void_p(d_remaining) if d_remaining else None,
+            int(uniform_remaining), int(count), float(step_size), float(start_time), int(integrator),
+            C.c_void_p(d_pos_out), C.c_void_p(d_rem_out) if d_rem_out else None,
+            C.c_void_p(d_status_out) if d_status_out else None,
+            C.byref(steps) if wait else None))
+        return steps.value if wait else None
+
+    def kernel_timing(self, name: str) -> tuple[int, float]:
+        n = C.c_uint64()
+        ms = C.c_double()
+        _check(self.lib.dpah_kernel_timing(self.h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def kernel_timing_reset(self) -> None:
+        _check(self.lib.dpah_kernel_timing_reset(self.h))
+
+
+class Field:
+    def __init__(self, ctx: Context, h, shape, nt):
+        self.ctx, self.h, self.shape, self.nt = ctx, h, tuple(int(s) for s in shape), int(nt)
+
+    def release(self) -> None:
+        if self.h:
+            _check(self.ctx.lib.dpah_field_release(self.h))
+            self.h = None
+
+
+class Result:
+    def __init__(self, ctx: Context, h, n: int):
+        self.ctx, self.h, self.n = ctx, h, n
+
+    def free(self) -> None:
+        if self.h:
+            _check(self.ctx.lib.dpah_result_free(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def round_count(self) -> int:
+        v = C.c_uint64()
+        _check(self.ctx.lib.dpah_result_round_count(self.h, C.byref(v)))
+        return v.value
+
+    def round(self, r: int):
+        """(vertices [V,3] f32, colors [V] f32, indices [I] u32|u64) of round r."""
+        V, I, ib = C.c_uint64(), C.c_uint64(), C.c_int()
+        _check(self.ctx.lib.dpah_result_round_sizes(self.h, r, C.byref(V), C.byref(I), C.byref(ib)))
+        verts = np.empty((V.value, 3), np.float32)
+        cols = np.empty(V.value, np.float32)
+        idx = np.empty(I.value, np.uint64 if ib.value == 8 else np.uint32)
+        _check(self.ctx.lib.dpah_result_copy_round(self.h, r, _ptr(verts), _ptr(cols), _ptr(idx)))
+        return verts, cols, idx
+
+    def final_state(self):
+        pos = np.empty((self.n, 3), np.float32)
+        rem = np.empty(self.n, np.uint64)
+        st = np.empty(self.n, np.uint8)
+        _check(self.ctx.lib.dpah_result_final_state(self.h, _ptr(pos), _ptr(rem), _ptr(st)))
+        return pos, rem, st
+
+    @property
+    def total_steps(self) -> int:
+        v = C.c_uint64()
+        _check(self.ctx.lib.dpah_result_total_steps(self.h, C.byref(v)))
+        return v.value

@@ -1,0 +1,173 @@
+// advect_kernels.hpp -- K_advect: the per-particle loop of
+// particle_advector.cpp:325-396 as HIP kernels for gfx950.
+//
+// One lane integrates one particle at a time.  In the persistent form a lane
+// whose particle retires (completed, left the grid, zero vector) takes the next
+// particle from an on-device queue (one atomic per wave, ranks by mbcnt), so
+// divergent retirement never idles a wave while work is left.  Particle state
+// lives in registers; positions are AoS float3 in HBM, which 64 lanes read and
+// write as one contiguous 768-byte run.
+#pragma once
+
+#include "device_math.hpp"
+
+namespace dpah {
+
+struct AdvectArgs {
+  FieldView f;
+  const float*    seeds;        // [n][3]
+  const uint64_t* rem_in;       // [n], or null -> uniform_rem for every particle
+  uint64_t        uniform_rem;
+  float*          pos_out;      // [n][3]
+  uint64_t*       rem_out;      // [n] or null
+  uint8_t*        status_out;   // [n] or null
+  uint32_t*       steps_out;    // [n] or null
+  uint8_t*        special_out;  // [n] or null (record): a vertex equals a sentinel
+  float*          slots;        // record: [n][stride][3], slot 0 = seed
+  uint64_t        stride;
+  uint64_t        n;
+  float           t0;
+  StepCoef        coef;
+  unsigned long long* queue;        // [0]: next particle to hand out (persistent form)
+  unsigned long long* total_steps;  // one atomic add per wave at exit
+};
+
+__device__ __forceinline__ bool is_sentinel(float a, float b, float c) {
+  return (a == -1.0f && b == -1.0f && c == -1.0f) || (a == -2.0f && b == -2.0f && c == -2.0f);
+}
+
+__device__ __forceinline__ void put3(float* p, float a, float b, float c) {
+  p[0] = a; p[1] = b; p[2] = c;
+}
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ void add_wave_steps(unsigned long long* total, uint64_t mine) {
+  // wave64 butterfly sum, then one atomic from lane 0
+  for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, 64);
+  if ((threadIdx.x & 63) == 0 && mine) atomicAdd(total, (unsigned long long)mine);
+}
+
+// Particle registers carried across the step loop.
+struct Lane {
+  uint64_t pid;
+  uint64_t rem;
+  uint32_t it;
+  float p0, p1, p2;
+  bool special;
+};
+
+template <bool kRecord>
+__device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uint64_t pid) {
+  L.pid = pid;
+  const float* s = a.seeds + 3 * pid;
+  L.p0 = s[0]; L.p1 = s[1]; L.p2 = s[2];
+  L.rem = a.rem_in ? a.rem_in[pid] : a.uniform_rem;
+  L.it = 0;
+  if constexpr (kRecord) {
+    L.special = is_sentinel(L.p0, L.p1, L.p2);
+    put3(a.slots + 3 * (pid * a.stride), L.p0, L.p1, L.p2);
+  }
+}
+
+template <bool kRecord>
+__device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const Lane& L, uint8_t st) {
+  const uint64_t pid = L.pid;
+  if constexpr (kRecord) {
+    put3(a.slots + 3 * (pid * a.stride + (uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
+    if (a.special_out) a.special_out[pid] = L.special ? 1 : 0;
+  }
+  put3(a.pos_out + 3 * pid, L.p0, L.p1, L.p2);
+  if (a.rem_out) a.rem_out[pid] = L.rem;
+  if (a.status_out) a.status_out[pid] = st;
+  if (a.steps_out) a.steps_out[pid] = L.it;
+  return L.it;
+}
+
+// One iteration of the loop at particle_advector.cpp:336-389.  Returns the
+// retirement status, or kActive while the particle keeps going.
+template <bool kRecord, int kIntegrator, bool kUnsteady>
+__device__ __forceinline__ uint8_t step_once(const AdvectArgs& a, Lane& L) {
+  if (L.rem == 0) return kCompleted;
+  float t = 0.0f;
+  if constexpr (kUnsteady || kIntegrator == kRk4) {
+    const float ft = (float)L.it * a.coef.dt;   // iteration_index * step_size_
+    t = a.t0 + ft;
+  }
+  V3 v;
+  if (!sample<kUnsteady>(a.f, L.p0, L.p1, L.p2, t, v)) return kExited;
+  if (is_zero(v)) return kZero;
+  if constexpr (kIntegrator == kRk4) {
+    if (!step_rk4<kUnsteady>(a.f, L.p0, L.p1, L.p2, v, t, a.coef)) return kExited;
+  } else {
+    step_dpa(L.p0, L.p1, L.p2, v, a.coef);
+  }
+  L.rem -= 1;
+  L.it += 1;
+  if constexpr (kRecord) {
+    put3(a.slots + 3 * (L.pid * a.stride + (uint64_t)L.it), L.p0, L.p1, L.p2);
+    L.special |= is_sentinel(L.p0, L.p1, L.p2);
+  }
+  return L.rem == 0 ? kCompleted : kActive;
+}
+
+// Per-request form: one thread per particle, runs it to retirement.
+template <bool kRecord, int kIntegrator, bool kUnsteady>
+__global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
+  const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t mine = 0;
+  if (pid < a.n) {
+    Lane L;
+    begin_particle<kRecord>(a, L, pid);
+    uint8_t st;
+    do {
+      st = step_once<kRecord, kIntegrator, kUnsteady>(a, L);
+    } while (st == kActive);
+    mine = finish_particle<kRecord>(a, L, st);
+  }
+  add_wave_steps(a.total_steps, mine);
+}
+
+// Persistent form with the on-device work queue.
+template <bool kRecord, int kIntegrator, bool kUnsteady>
+__global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
+  const uint32_t lane = threadIdx.x & 63;
+  Lane L;
+  L.pid = ~0ull;
+  L.rem = 0; L.it = 0; L.p0 = L.p1 = L.p2 = 0.0f; L.special = false;
+  bool open = true;       // wave-uniform: the queue still has particles
+  uint64_t mine = 0;
+  while (true) {
+    const bool idle = (L.pid == ~0ull);
+    if (open) {
+      const uint64_t need = __ballot(idle);
+      if (need) {
+        const uint32_t cnt = (uint32_t)__popcll(need);
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)cnt);
+        base = __shfl(base, (int)leader, 64);
+        if (idle) {
+          const uint64_t cand = base + lane_rank(need);
+          if (cand < a.n) begin_particle<kRecord>(a, L, cand);
+        }
+        if (base + cnt >= a.n) open = false;
+      }
+    }
+    const bool busy = (L.pid != ~0ull);
+    if (!__any(busy)) break;
+    if (busy) {
+      const uint8_t st = step_once<kRecord, kIntegrator, kUnsteady>(a, L);
+      if (st != kActive) {
+        mine += finish_particle<kRecord>(a, L, st);
+        L.pid = ~0ull;
+      }
+    }
+  }
+  add_wave_steps(a.total_steps, mine);
+}
+
+}  // namespace dpah

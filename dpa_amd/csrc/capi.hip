@@ -1,0 +1,544 @@
+// capi.hip -- the C ABI declared in include/dpa_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "runtime.hpp"
+
+#define DPAH_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace dpah {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+template <class F>
+int guarded(F&& body) {
+  try {
+    g_error.clear();
+    body();
+    return DPAH_OK;
+  } catch (const Error& e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_error = "host allocation failed";
+    return DPAH_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return DPAH_ERR_INVALID;
+  }
+}
+
+}  // namespace dpah
+
+using namespace dpah;
+
+// ------------------------------------------------------------ context ----
+
+void* dpah_context::alloc(uint64_t bytes) {
+  if (bytes == 0) bytes = 1;
+  if (bytes_in_use.load() + bytes > memory_budget)
+    throw Error(DPAH_ERR_NOMEM, "device memory budget exceeded: " + std::to_string(bytes) +
+                                    " bytes requested, " + std::to_string(bytes_in_use.load()) +
+                                    " of " + std::to_string(memory_budget) + " in use");
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess)
+    throw Error(DPAH_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " +
+                                    hipGetErrorString(e));
+  bytes_in_use += bytes;
+  return p;
+}
+
+void dpah_context::release(void* p, uint64_t bytes) {
+  if (!p) return;
+  if (bytes == 0) bytes = 1;
+  (void)hipFree(p);
+  bytes_in_use -= bytes;
+}
+
+hipEvent_t dpah_context::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  DPAH_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void dpah_context::resolve_timings() {
+  for (auto& t : pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+      auto& k = timings[t.name];
+      k.launches += 1;
+      k.total_ms += ms;
+    }
+    event_pool.push_back(t.start);
+    event_pool.push_back(t.stop);
+  }
+  pending.clear();
+}
+
+dpah_result::~dpah_result() {
+  for (auto& r : rounds)
+    for (auto& c : r.chunks)
+      if (c.vertices) ctx->release(c.vertices, c.bytes);
+}
+
+DPAH_EXPORT const char* dpah_last_error(void) { return dpah::g_error.c_str(); }
+
+DPAH_EXPORT int dpah_version(void) { return 1; }
+
+DPAH_EXPORT int dpah_create(int device, const dpah_options* options, dpah_context** out) {
+  return guarded([&] {
+    DPAH_REQUIRE(out, "out is null");
+    int count = 0;
+    DPAH_HIP(hipGetDeviceCount(&count));
+    DPAH_REQUIRE(device >= 0 && device < count, "device index out of range");
+    DPAH_HIP(hipSetDevice(device));
+    auto* ctx = new dpah_context();
+    try {
+      ctx->device = device;
+      hipDeviceProp_t prop;
+      DPAH_HIP(hipGetDeviceProperties(&prop, device));
+      ctx->cu_count = prop.multiProcessorCount;
+      size_t free_b = 0, total_b = 0;
+      DPAH_HIP(hipMemGetInfo(&free_b, &total_b));
+      ctx->memory_budget = (options && options->memory_budget_bytes)
+                               ? options->memory_budget_bytes
+                               : (uint64_t)(0.9 * (double)free_b);
+      ctx->slot_budget = (options && options->slot_budget_bytes) ? options->slot_budget_bytes
+                                                                 : (4ull << 30);
+      ctx->timing = options && options->kernel_timing;
+      DPAH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+      ctx->d_counters = static_cast<unsigned long long*>(ctx->alloc(8 * sizeof(unsigned long long)));
+      DPAH_HIP(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+      ctx->sync();
+    } catch (...) {
+      delete ctx;
+      throw;
+    }
+    *out = ctx;
+  });
+}
+
+DPAH_EXPORT int dpah_destroy(dpah_context* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->resolve_timings();
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto& a : ctx->user_allocs) ctx->release(a.first, a.second);
+    if (ctx->d_counters) ctx->release(ctx->d_counters, 8 * sizeof(unsigned long long));
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+DPAH_EXPORT int dpah_synchronize(dpah_context* ctx) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx, "ctx is null");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    ctx->sync();
+  });
+}
+
+DPAH_EXPORT int dpah_device_alloc(dpah_context* ctx, uint64_t bytes, void** out) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && out, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    *out = ctx->alloc(bytes);
+    ctx->user_allocs[*out] = bytes;
+  });
+}
+
+DPAH_EXPORT int dpah_device_free(dpah_context* ctx, void* ptr) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx, "ctx is null");
+    if (!ptr) return;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    DPAH_HIP(hipStreamSynchronize(ctx->stream));
+    auto it = ctx->user_allocs.find(ptr);
+    DPAH_REQUIRE(it != ctx->user_allocs.end(), "pointer was not returned by dpah_device_alloc");
+    ctx->release(ptr, it->second);
+    ctx->user_allocs.erase(it);
+  });
+}
+
+DPAH_EXPORT int dpah_copy_to_device(dpah_context* ctx, void* dst, const void* src, uint64_t bytes) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && (bytes == 0 || (dst && src)), "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    if (bytes) DPAH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    ctx->sync();
+  });
+}
+
+DPAH_EXPORT int dpah_copy_to_host(dpah_context* ctx, void* dst, const void* src, uint64_t bytes) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && (bytes == 0 || (dst && src)), "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    if (bytes) DPAH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
+}
+
+// -------------------------------------------------------------- fields ----
+
+static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shape[3], uint64_t nt,
+                       const float offset[3], const float spacing[3], float t_off, float t_sp,
+                       dpah_field** out) {
+  DPAH_REQUIRE(ctx && xyzv && shape && offset && spacing && out, "null argument");
+  for (int i = 0; i < 3; ++i) {
+    DPAH_REQUIRE(shape[i] >= 2 && shape[i] < (1ull << 24), "field shape must be in [2, 2^24) per axis");
+    DPAH_REQUIRE(spacing[i] > 0.0f, "field spacing must be positive");
+  }
+  DPAH_REQUIRE(nt >= 1 && nt < (1ull << 24), "time_steps must be in [1, 2^24)");
+  if (nt > 1) DPAH_REQUIRE(nt >= 2 && t_sp > 0.0f, "time-variant field needs time_spacing > 0");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DPAH_HIP(hipSetDevice(ctx->device));
+  const uint64_t nodes = shape[0] * shape[1] * shape[2];
+  const uint64_t bytes = nodes * nt * 3 * sizeof(float);
+  auto* f = new dpah_field();
+  try {
+    f->ctx = ctx;
+    f->bytes = bytes;
+    f->d_data = static_cast<float*>(ctx->alloc(bytes));
+    DPAH_HIP(hipMemcpyAsync(f->d_data, xyzv, bytes, hipMemcpyHostToDevice, ctx->stream));
+    ctx->sync();
+  } catch (...) {
+    if (f->d_data) ctx->release(f->d_data, bytes);
+    delete f;
+    throw;
+  }
+  for (int i = 0; i < 3; ++i) f->shape[i] = shape[i];
+  f->time_steps = nt;
+  FieldView& v = f->view;
+  v.data = f->d_data;
+  v.slice = (int64_t)nodes;
+  v.n0 = (int32_t)shape[0]; v.n1 = (int32_t)shape[1]; v.n2 = (int32_t)shape[2];
+  v.nt = (int32_t)nt;
+  v.o0 = offset[0]; v.o1 = offset[1]; v.o2 = offset[2]; v.to = t_off;
+  v.h0 = spacing[0]; v.h1 = spacing[1]; v.h2 = spacing[2]; v.th = nt > 1 ? t_sp : 1.0f;
+  *out = f;
+}
+
+DPAH_EXPORT int dpah_field_create(dpah_context* ctx, const float* xyzv, const uint64_t shape[3],
+                                  const float offset[3], const float spacing[3], dpah_field** out) {
+  return guarded([&] { make_field(ctx, xyzv, shape, 1, offset, spacing, 0.0f, 1.0f, out); });
+}
+
+DPAH_EXPORT int dpah_field_create_unsteady(dpah_context* ctx, const float* xyzv,
+                                           const uint64_t shape[3], uint64_t time_steps,
+                                           const float offset[3], const float spacing[3],
+                                           float time_offset, float time_spacing,
+                                           dpah_field** out) {
+  return guarded([&] {
+    DPAH_REQUIRE(time_steps >= 2, "a time-variant field needs at least two time steps");
+    make_field(ctx, xyzv, shape, time_steps, offset, spacing, time_offset, time_spacing, out);
+  });
+}
+
+DPAH_EXPORT int dpah_field_release(dpah_field* field) {
+  return guarded([&] {
+    if (!field) return;
+    dpah_context* ctx = field->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    DPAH_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->release(field->d_data, field->bytes);
+    delete field;
+  });
+}
+
+// ------------------------------------------------------------- advect ----
+
+namespace {
+
+constexpr uint64_t kMaxRemaining = (1ull << 32) - 3;
+
+void check_integrator(int integrator) {
+  DPAH_REQUIRE(integrator == DPAH_INTEGRATOR_DPA_RK4 || integrator == DPAH_INTEGRATOR_RK4,
+               "unknown integrator");
+}
+
+}  // namespace
+
+DPAH_EXPORT int dpah_advect(dpah_context* ctx, const dpah_field* field, const float* seeds,
+                            const uint64_t* remaining, uint64_t uniform_remaining, uint64_t count,
+                            const dpah_advect_params* params, dpah_result** out) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && field && params && out, "null argument");
+    DPAH_REQUIRE(field->ctx == ctx, "field belongs to another context");
+    DPAH_REQUIRE(count == 0 || seeds, "seeds is null");
+    DPAH_REQUIRE(count == 0 || params->particles_per_round > 0,
+                 "particles_per_round must be positive (dpa would never finish)");
+    check_integrator(params->integrator);
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+
+    std::vector<uint64_t> rem(count);
+    for (uint64_t i = 0; i < count; ++i) {
+      rem[i] = remaining ? remaining[i] : uniform_remaining;
+      DPAH_REQUIRE(rem[i] <= kMaxRemaining, "remaining iterations must be below 2^32 - 2");
+    }
+
+    auto res = std::make_unique<dpah_result>();
+    res->ctx = ctx;
+    res->use64 = params->use_64_bit_indices ? 1 : 0;
+    res->final_pos.resize(3 * count);
+    res->final_rem.resize(count);
+    res->final_status.resize(count);
+
+    const uint64_t n1 = count ? count : 1;
+    DeviceBuffer d_seeds(ctx, 3 * n1 * sizeof(float));
+    DeviceBuffer d_rem(ctx, n1 * sizeof(uint64_t));
+    DeviceBuffer d_pos(ctx, 3 * n1 * sizeof(float));
+    DeviceBuffer d_rem_out(ctx, n1 * sizeof(uint64_t));
+    DeviceBuffer d_status(ctx, n1);
+    if (count) {
+      DPAH_HIP(hipMemcpyAsync(d_seeds.ptr, seeds, 3 * count * sizeof(float), hipMemcpyHostToDevice,
+                              ctx->stream));
+      DPAH_HIP(hipMemcpyAsync(d_rem.ptr, rem.data(), count * sizeof(uint64_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+    }
+
+    AdvectLaunch L{};
+    L.f = &field->view;
+    L.uniform_rem = 0;
+    L.dt = params->step_size;
+    L.t0 = params->start_time;
+    L.integrator = params->integrator;
+    L.persistent = true;
+
+    uint64_t total_steps = 0;
+    auto read_steps = [&] {
+      unsigned long long s = 0;
+      DPAH_HIP(hipMemcpyAsync(&s, ctx->d_counters + 1, sizeof(s), hipMemcpyDeviceToHost,
+                              ctx->stream));
+      ctx->sync();
+      total_steps += s;
+    };
+
+    if (!params->record) {
+      // Rounds only decide the order in which independent particles run, and
+      // with record off they produce no output: one launch covers them all.
+      L.seeds = d_seeds.as<float>();
+      L.rem_in = d_rem.as<uint64_t>();
+      L.pos_out = d_pos.as<float>();
+      L.rem_out = d_rem_out.as<uint64_t>();
+      L.status_out = d_status.as<uint8_t>();
+      L.n = count;
+      launch_advect(ctx, L);
+      read_steps();
+    } else {
+      const uint64_t ppr = params->particles_per_round;
+      DeviceBuffer d_steps(ctx, n1 * sizeof(uint32_t));
+      DeviceBuffer d_special(ctx, n1);
+      uint64_t active = count;
+      do {  // pipeline.cpp:100-146: at least one round, tail of the seeds first
+        const uint64_t rc = std::min(ppr, active);
+        const uint64_t first = active - rc;
+        uint64_t max_rem = 0;
+        for (uint64_t i = 0; i < rc; ++i) max_rem = std::max(max_rem, rem[first + i]);
+        const uint64_t stride = max_rem + 2;   // compute_round_state (305)
+        Round round;
+        if (rc) {
+          const uint64_t row_bytes = stride * 3 * sizeof(float);
+          uint64_t chunk = std::max<uint64_t>(1, ctx->slot_budget / row_bytes);
+          chunk = std::min(chunk, rc);
+          DeviceBuffer slots(ctx, chunk * row_bytes);
+          for (uint64_t c0 = 0; c0 < rc; c0 += chunk) {
+            const uint64_t cn = std::min(chunk, rc - c0);
+            const uint64_t pid0 = first + c0;
+            L.seeds = d_seeds.as<float>() + 3 * pid0;
+            L.rem_in = d_rem.as<uint64_t>() + pid0;
+            L.pos_out = d_pos.as<float>() + 3 * pid0;
+            L.rem_out = d_rem_out.as<uint64_t>() + pid0;
+            L.status_out = d_status.as<uint8_t>() + pid0;
+            L.steps_out = d_steps.as<uint32_t>() + pid0;
+            L.special_out = d_special.as<uint8_t>() + pid0;
+            L.slots = slots.as<float>();
+            L.stride = stride;
+            L.n = cn;
+            launch_advect(ctx, L);
+            read_steps();
+            Chunk ch = compact_rows(ctx, slots.as<float>(), stride, L.steps_out, L.special_out, cn,
+                                    round.V, res->use64);
+            round.V += ch.V;
+            round.I += ch.I;
+            round.chunks.push_back(ch);
+          }
+          if (!res->use64)
+            DPAH_REQUIRE(round.V <= 0xffffffffull,
+                         "round has more than 2^32-1 vertices; 32-bit indices cannot address it");
+        }
+        res->rounds.push_back(std::move(round));
+        active = first;
+      } while (active > 0);
+    }
+    if (count) {
+      DPAH_HIP(hipMemcpyAsync(res->final_pos.data(), d_pos.ptr, 3 * count * sizeof(float),
+                              hipMemcpyDeviceToHost, ctx->stream));
+      DPAH_HIP(hipMemcpyAsync(res->final_rem.data(), d_rem_out.ptr, count * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+      DPAH_HIP(hipMemcpyAsync(res->final_status.data(), d_status.ptr, count, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+    ctx->sync();
+    res->total_steps = total_steps;
+    *out = res.release();
+  });
+}
+
+DPAH_EXPORT int dpah_result_round_count(const dpah_result* r, uint64_t* rounds) {
+  return guarded([&] {
+    DPAH_REQUIRE(r && rounds, "null argument");
+    *rounds = r->rounds.size();
+  });
+}
+
+DPAH_EXPORT int dpah_result_round_sizes(const dpah_result* r, uint64_t round, uint64_t* vertex_count,
+                                        uint64_t* index_count, int* index_bytes) {
+  return guarded([&] {
+    DPAH_REQUIRE(r, "null argument");
+    DPAH_REQUIRE(round < r->rounds.size(), "round out of range");
+    if (vertex_count) *vertex_count = r->rounds[round].V;
+    if (index_count) *index_count = r->rounds[round].I;
+    if (index_bytes) *index_bytes = r->use64 ? 8 : 4;
+  });
+}
+
+DPAH_EXPORT int dpah_result_copy_round(const dpah_result* r, uint64_t round, float* vertices,
+                                       float* colors, void* indices) {
+  return guarded([&] {
+    DPAH_REQUIRE(r, "null argument");
+    DPAH_REQUIRE(round < r->rounds.size(), "round out of range");
+    dpah_context* ctx = r->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    const uint64_t ib = r->use64 ? 8 : 4;
+    uint64_t v0 = 0, i0 = 0;
+    for (const Chunk& c : r->rounds[round].chunks) {
+      if (vertices && c.V)
+        DPAH_HIP(hipMemcpyAsync(vertices + 3 * v0, c.vertices, 3 * c.V * sizeof(float),
+                                hipMemcpyDeviceToHost, ctx->stream));
+      if (colors && c.V)
+        DPAH_HIP(hipMemcpyAsync(colors + v0, c.colors, c.V * sizeof(float), hipMemcpyDeviceToHost,
+                                ctx->stream));
+      if (indices && c.I)
+        DPAH_HIP(hipMemcpyAsync(static_cast<char*>(indices) + i0 * ib, c.indices, c.I * ib,
+                                hipMemcpyDeviceToHost, ctx->stream));
+      v0 += c.V;
+      i0 += c.I;
+    }
+    ctx->sync();
+  });
+}
+
+DPAH_EXPORT int dpah_result_final_state(const dpah_result* r, float* positions, uint64_t* remaining,
+                                        uint8_t* status) {
+  return guarded([&] {
+    DPAH_REQUIRE(r, "null argument");
+    const uint64_t n = r->final_rem.size();
+    if (positions && n) std::memcpy(positions, r->final_pos.data(), 3 * n * sizeof(float));
+    if (remaining && n) std::memcpy(remaining, r->final_rem.data(), n * sizeof(uint64_t));
+    if (status && n) std::memcpy(status, r->final_status.data(), n);
+  });
+}
+
+DPAH_EXPORT int dpah_result_total_steps(const dpah_result* r, uint64_t* steps) {
+  return guarded([&] {
+    DPAH_REQUIRE(r && steps, "null argument");
+    *steps = r->total_steps;
+  });
+}
+
+DPAH_EXPORT int dpah_result_free(dpah_result* r) {
+  return guarded([&] {
+    if (!r) return;
+    dpah_context* ctx = r->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    delete r;
+  });
+}
+
+DPAH_EXPORT int dpah_advect_endpoints_device(dpah_context* ctx, const dpah_field* field,
+                                             const float* seeds, const uint64_t* remaining,
+                                             uint64_t uniform_remaining, uint64_t count,
+                                             float step_size, float start_time, int integrator,
+                                             float* positions_out, uint64_t* remaining_out,
+                                             uint8_t* status_out, uint64_t* total_steps) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && field, "null argument");
+    DPAH_REQUIRE(field->ctx == ctx, "field belongs to another context");
+    DPAH_REQUIRE(count == 0 || (seeds && positions_out), "seeds / positions_out is null");
+    DPAH_REQUIRE(remaining || uniform_remaining <= kMaxRemaining,
+                 "remaining iterations must be below 2^32 - 2");
+    check_integrator(integrator);
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    AdvectLaunch L{};
+    L.f = &field->view;
+    L.seeds = seeds;
+    L.rem_in = remaining;
+    L.uniform_rem = uniform_remaining;
+    L.pos_out = positions_out;
+    L.rem_out = remaining_out;
+    L.status_out = status_out;
+    L.n = count;
+    L.dt = step_size;
+    L.t0 = start_time;
+    L.integrator = integrator;
+    L.persistent = true;
+    launch_advect(ctx, L);
+    if (total_steps) {
+      unsigned long long s = 0;
+      DPAH_HIP(hipMemcpyAsync(&s, ctx->d_counters + 1, sizeof(s), hipMemcpyDeviceToHost,
+                              ctx->stream));
+      ctx->sync();
+      *total_steps = s;
+    }
+  });
+}
+
+DPAH_EXPORT int dpah_kernel_timing(dpah_context* ctx, const char* kernel, uint64_t* launches,
+                                   double* total_ms) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && kernel, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    ctx->sync();
+    auto it = ctx->timings.find(kernel);
+    if (launches) *launches = it == ctx->timings.end() ? 0 : it->second.launches;
+    if (total_ms) *total_ms = it == ctx->timings.end() ? 0.0 : it->second.total_ms;
+  });
+}
+
+DPAH_EXPORT int dpah_kernel_timing_reset(dpah_context* ctx) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    ctx->sync();
+    ctx->timings.clear();
+  });
+}

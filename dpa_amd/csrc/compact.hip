@@ -1,0 +1,65 @@
+// compact.hip -- launchers of the K_compact kernels (compact_kernels.hpp).
+#include "compact_kernels.hpp"
+#include "runtime.hpp"
+
+namespace dpah {
+
+Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const uint32_t* steps,
+                   const uint8_t* special, uint64_t count, uint64_t vertex_base, int use64) {
+  Chunk ch;
+  if (count == 0) return ch;
+  DeviceBuffer vcount(ctx, count * sizeof(uint64_t));
+  DeviceBuffer icount(ctx, count * sizeof(uint64_t));
+  const uint64_t blocks = (count + kScanTile - 1) / kScanTile;
+  DeviceBuffer sums(ctx, (2 * blocks + 2) * sizeof(uint64_t));
+
+  CompactArgs a{};
+  a.slots = slots;
+  a.stride = stride;
+  a.steps = steps;
+  a.special = special;
+  a.count = count;
+  a.vcount = vcount.as<uint64_t>();
+  a.icount = icount.as<uint64_t>();
+  a.use64 = use64;
+  a.vertex_base = vertex_base;
+
+  const unsigned g1 = (unsigned)((count + 255) / 256);
+  hipLaunchKernelGGL(k_compact_count, dim3(g1), dim3(256), 0, ctx->stream, a);
+  DPAH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)blocks), dim3(kScanThreads), 0, ctx->stream,
+                     a.vcount, a.icount, count, sums.as<uint64_t>());
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, ctx->stream, sums.as<uint64_t>(),
+                     blocks);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)blocks), dim3(kScanThreads), 0, ctx->stream,
+                     a.vcount, a.icount, count, sums.as<uint64_t>());
+  DPAH_HIP(hipGetLastError());
+  uint64_t totals[2] = {0, 0};
+  DPAH_HIP(hipMemcpyAsync(totals, sums.as<uint64_t>() + 2 * blocks, sizeof(totals),
+                          hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  ch.V = totals[0];
+  ch.I = 2 * totals[1];
+  const uint64_t ib = use64 ? 8 : 4;
+  ch.bytes = ch.V * 12 + ch.V * 4 + ch.I * ib;
+  // one allocation per chunk, carved into vertices | colors | indices
+  if (ch.bytes) {
+    void* base = ctx->alloc(ch.bytes);
+    ch.vertices = static_cast<float*>(base);
+    ch.colors = ch.vertices + 3 * ch.V;
+    ch.indices = ch.colors + ch.V;
+  }
+  a.vertices = ch.vertices;
+  a.colors = ch.colors;
+  a.indices = ch.indices;
+  const uint64_t threads = count * 64;
+  ctx->timed("compact_write", [&] {
+    hipLaunchKernelGGL(k_compact_write, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       ctx->stream, a);
+  });
+  DPAH_HIP(hipGetLastError());
+  ctx->sync();
+  return ch;
+}
+
+}  // namespace dpah

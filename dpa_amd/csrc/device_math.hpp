@@ -1,0 +1,223 @@
+// device_math.hpp -- per-step arithmetic of dpa's advection stage for gfx950.
+//
+// Every operation is one IEEE binary32 rounding, in the order dpa's CPU build
+// performs it; the library is compiled with -ffp-contract=off and correctly
+// rounded division/sqrt, so no multiply-add is fused behind our back.  The
+// only fused operation is the explicit fmaf in exact_fmod(), whose result is
+// exact (see there).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dpah {
+
+// A regular float3 grid resident in HBM (regular_grid.hpp:148-151):
+// node (i0,i1,i2) of time slice it lives at data + 3*(it*slice + (i0*n1+i1)*n2+i2).
+// Axis 0 is the slowest; nt == 1 for steady fields.
+struct FieldView {
+  const float* data;
+  int64_t slice;     // n0*n1*n2
+  int32_t n0, n1, n2, nt;
+  float o0, o1, o2, to;
+  float h0, h1, h2, th;
+};
+
+enum Status : uint8_t { kActive = 0, kCompleted = 1, kExited = 2, kZero = 3 };
+enum Integrator : int { kDpaRk4 = 0, kRk4 = 1 };
+
+struct Axis {
+  float w;   // fmod(d, h) / h
+  int32_t c; // floor(d / h)
+};
+
+// fmodf(d, h) for d >= 0 (or -0) and h > 0, given s = floorf(fl(d / h)).
+// fl(d/h) is monotone in the exact quotient, so s is the exact floor k or k+1.
+// When s == k the exact remainder d - k*h is representable (it is fmod's
+// result), so one fmaf returns it without rounding; when s == k+1 the fmaf
+// result is negative (rounding keeps the sign of a non-zero value) and the
+// second fmaf with k is exact.  s == 0 returns d itself, keeping -0.
+__device__ __forceinline__ float exact_fmod(float d, float h, float s) {
+  if (s == 0.0f) return d;
+  float r = __builtin_fmaf(-s, h, d);
+  if (r < 0.0f) {
+    const float k = s - 1.0f;
+    r = (k == 0.0f) ? d : __builtin_fmaf(-k, h, d);
+  }
+  return r;
+}
+
+// regular_grid::contains for one axis (regular_grid.hpp:41-50) and the weight /
+// start index of regular_grid::interpolate (52-75).  Returns false outside.
+__device__ __forceinline__ bool locate(float x, float o, float h, int32_t n, Axis& a) {
+  const float d = x - o;
+  const float s = floorf(d / h);
+  if (!(s >= 0.0f) || s >= (float)(n - 1)) return false;
+  const float r = exact_fmod(d, h, s);
+  a.w = r / h;
+  a.c = (int32_t)s;
+  return true;
+}
+
+struct V3 { float a, b, c; };
+
+// (1 - w) * lo + w * hi per component, two products then one sum.
+__device__ __forceinline__ V3 lerp3(const V3& lo, const V3& hi, float w, float om) {
+  V3 r;
+  { const float x = om * lo.a; const float y = w * hi.a; r.a = x + y; }
+  { const float x = om * lo.b; const float y = w * hi.b; r.b = x + y; }
+  { const float x = om * lo.c; const float y = w * hi.c; r.c = x + y; }
+  return r;
+}
+
+__device__ __forceinline__ V3 node(const FieldView& f, const float* base, int64_t off) {
+  const float* p = base + 3 * off;
+  return V3{p[0], p[1], p[2]};
+}
+
+// Trilinear sample of one time slice, corner order and lerp cascade of
+// regular_grid.hpp:59-74: list index 4a+2b+c, axis 2 first, then 1, then 0.
+__device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
+                                        const Axis& a0, const Axis& a1, const Axis& a2) {
+  const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
+  const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
+  const V3 L0 = node(f, slice, b);
+  const V3 L1 = node(f, slice, b + 1);
+  const V3 L2 = node(f, slice, b + n2);
+  const V3 L3 = node(f, slice, b + n2 + 1);
+  const V3 L4 = node(f, slice, b + n12);
+  const V3 L5 = node(f, slice, b + n12 + 1);
+  const V3 L6 = node(f, slice, b + n12 + n2);
+  const V3 L7 = node(f, slice, b + n12 + n2 + 1);
+  const float w2 = a2.w, m2 = 1.0f - w2;
+  const V3 A0 = lerp3(L0, L1, w2, m2), A1 = lerp3(L2, L3, w2, m2);
+  const V3 A2 = lerp3(L4, L5, w2, m2), A3 = lerp3(L6, L7, w2, m2);
+  const float w1 = a1.w, m1 = 1.0f - w1;
+  const V3 B0 = lerp3(A0, A1, w1, m1), B1 = lerp3(A2, A3, w1, m1);
+  const float w0 = a0.w, m0 = 1.0f - w0;
+  return lerp3(B0, B1, w0, m0);
+}
+
+// Four-dimensional sample of a time-variant field (regular_grid<vector3,4>,
+// regular_fields.hpp:27): the 16 corners in permute_for order have time as the
+// innermost axis, so the cascade lerps time first, then axes 2, 1, 0.
+__device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, const Axis& a1,
+                                         const Axis& a2, const Axis& at) {
+  const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
+  const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
+  const float* s0 = f.data + 3 * (int64_t)at.c * f.slice;
+  const float* s1 = s0 + 3 * f.slice;
+  const float wt = at.w, mt = 1.0f - wt;
+  const int64_t off[8] = {b, b + 1, b + n2, b + n2 + 1, b + n12, b + n12 + 1, b + n12 + n2,
+                          b + n12 + n2 + 1};
+  V3 L[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) L[k] = lerp3(node(f, s0, off[k]), node(f, s1, off[k]), wt, mt);
+  const float w2 = a2.w, m2 = 1.0f - w2;
+  const V3 A0 = lerp3(L[0], L[1], w2, m2), A1 = lerp3(L[2], L[3], w2, m2);
+  const V3 A2 = lerp3(L[4], L[5], w2, m2), A3 = lerp3(L[6], L[7], w2, m2);
+  const float w1 = a1.w, m1 = 1.0f - w1;
+  const V3 B0 = lerp3(A0, A1, w1, m1), B1 = lerp3(A2, A3, w1, m1);
+  const float w0 = a0.w, m0 = 1.0f - w0;
+  return lerp3(B0, B1, w0, m0);
+}
+
+// contains + interpolate at (p, t).  Returns false when p (or t, for a
+// time-variant field) lies outside; v is the stored (unswizzled) vector.
+template <bool kUnsteady>
+__device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, float p2, float t,
+                                       V3& v) {
+  Axis a0, a1, a2;
+  if (!locate(p0, f.o0, f.h0, f.n0, a0)) return false;
+  if (!locate(p1, f.o1, f.h1, f.n1, a1)) return false;
+  if (!locate(p2, f.o2, f.h2, f.n2, a2)) return false;
+  if constexpr (kUnsteady) {
+    Axis at;
+    if (!locate(t, f.to, f.th, f.nt, at)) return false;
+    v = quadlinear(f, a0, a1, a2, at);
+  } else {
+    (void)t;
+    v = trilinear(f, f.data, a0, a1, a2);
+  }
+  return true;
+}
+
+// Eigen isZero() at float dummy_precision (particle_advector.cpp:363).
+__device__ __forceinline__ bool is_zero(const V3& v) {
+  return fabsf(v.a) <= 1e-5f && fabsf(v.b) <= 1e-5f && fabsf(v.c) <= 1e-5f;
+}
+
+// Step coefficients a_j = fl(b_j * dt) with b = (fl(1/6), fl(1/3), fl(1/3), fl(1/6)).
+struct StepCoef {
+  float a16, a13, half_dt, zero_dt, dt;
+};
+__host__ __device__ inline StepCoef make_coef(float dt) {
+  const float b16 = 1.0f / 6.0f, b13 = 1.0f / 3.0f, half = 0.5f, zero = 0.0f;
+  StepCoef c;
+  c.a16 = b16 * dt;
+  c.a13 = b13 * dt;
+  c.half_dt = half * dt;
+  c.zero_dt = zero * dt;
+  c.dt = dt;
+  return c;
+}
+
+// x + a16 k + a13 k + a13 k + a16 k, left to right (odeint runge_kutta4 final
+// stage over vector_space_algebra, with dpa's single sample in every stage).
+__device__ __forceinline__ float rk_sum_dpa(float x, float k, const StepCoef& c) {
+  const float t1 = c.a16 * k; x = x + t1;
+  const float t2 = c.a13 * k; x = x + t2;
+  const float t3 = c.a13 * k; x = x + t3;
+  const float t4 = c.a16 * k; x = x + t4;
+  return x;
+}
+
+__device__ __forceinline__ void step_dpa(float& p0, float& p1, float& p2, const V3& v,
+                                         const StepCoef& c) {
+  // dxdt = (v2, v1, v0): axis k moves by stored component 2-k.
+  p0 = rk_sum_dpa(p0, v.c, c);
+  p1 = rk_sum_dpa(p1, v.b, c);
+  p2 = rk_sum_dpa(p2, v.a, c);
+}
+
+// Classic four-sample RK4 in odeint's explicit_generic_rk order (not dpa's
+// mode; an extension).  A stage point outside the grid returns false and leaves
+// the position unchanged.
+template <bool kUnsteady>
+__device__ __forceinline__ bool step_rk4(const FieldView& f, float& p0, float& p1, float& p2,
+                                         const V3& v, float t, const StepCoef& c) {
+  const float k10 = v.c, k11 = v.b, k12 = v.a;
+  float x0, x1, x2;
+  { const float s = c.half_dt * k10; x0 = p0 + s; }
+  { const float s = c.half_dt * k11; x1 = p1 + s; }
+  { const float s = c.half_dt * k12; x2 = p2 + s; }
+  V3 s2;
+  if (!sample<kUnsteady>(f, x0, x1, x2, t + c.half_dt, s2)) return false;
+  const float k20 = s2.c, k21 = s2.b, k22 = s2.a;
+  { float y = p0; const float a = c.zero_dt * k10; y = y + a; const float b = c.half_dt * k20; x0 = y + b; }
+  { float y = p1; const float a = c.zero_dt * k11; y = y + a; const float b = c.half_dt * k21; x1 = y + b; }
+  { float y = p2; const float a = c.zero_dt * k12; y = y + a; const float b = c.half_dt * k22; x2 = y + b; }
+  V3 s3;
+  if (!sample<kUnsteady>(f, x0, x1, x2, t + c.half_dt, s3)) return false;
+  const float k30 = s3.c, k31 = s3.b, k32 = s3.a;
+  const float one_dt = 1.0f * c.dt;
+  { float y = p0; const float a = c.zero_dt * k10; y = y + a; const float b = c.zero_dt * k20; y = y + b; const float d = one_dt * k30; x0 = y + d; }
+  { float y = p1; const float a = c.zero_dt * k11; y = y + a; const float b = c.zero_dt * k21; y = y + b; const float d = one_dt * k31; x1 = y + d; }
+  { float y = p2; const float a = c.zero_dt * k12; y = y + a; const float b = c.zero_dt * k22; y = y + b; const float d = one_dt * k32; x2 = y + d; }
+  V3 s4;
+  if (!sample<kUnsteady>(f, x0, x1, x2, t + one_dt, s4)) return false;
+  const float k40 = s4.c, k41 = s4.b, k42 = s4.a;
+  auto fin = [&](float x, float q1, float q2, float q3, float q4) {
+    const float a = c.a16 * q1; x = x + a;
+    const float b = c.a13 * q2; x = x + b;
+    const float d = c.a13 * q3; x = x + d;
+    const float e = c.a16 * q4; x = x + e;
+    return x;
+  };
+  p0 = fin(p0, k10, k20, k30, k40);
+  p1 = fin(p1, k11, k21, k31, k41);
+  p2 = fin(p2, k12, k22, k32, k42);
+  return true;
+}
+
+}  // namespace dpah

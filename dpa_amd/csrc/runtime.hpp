@@ -1,0 +1,178 @@
+// runtime.hpp -- internal state behind the C ABI: contexts, resident fields,
+// results, error reporting and timed launches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dpa_hip.h"
+#include "device_math.hpp"
+
+namespace dpah {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define DPAH_HIP(expr)                                                                        \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw ::dpah::Error(DPAH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+#define DPAH_REQUIRE(cond, msg)                                     \
+  do {                                                              \
+    if (!(cond)) throw ::dpah::Error(DPAH_ERR_INVALID, (msg));      \
+  } while (0)
+
+void set_error(const std::string& msg);
+
+struct TimedKernel {
+  uint64_t launches = 0;
+  double total_ms = 0.0;
+};
+
+struct PendingTiming {
+  std::string name;
+  hipEvent_t start, stop;
+};
+
+// Launch-shape parameters picked by the autotuner (or defaults).
+struct Tuning {
+  int block = 256;          // threads per block of the persistent advect kernel
+  int blocks_per_cu = 8;    // persistent grid = CUs * blocks_per_cu
+};
+
+}  // namespace dpah
+
+struct dpah_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  uint64_t memory_budget = 0;
+  uint64_t slot_budget = 0;
+  std::atomic<uint64_t> bytes_in_use{0};
+  std::map<void*, uint64_t> user_allocs;      // dpah_device_alloc blocks and their sizes
+  unsigned long long* d_counters = nullptr;   // [0] queue head, [1] steps, [2..7] spare
+  bool timing = false;
+  std::map<std::string, dpah::TimedKernel> timings;
+  std::vector<dpah::PendingTiming> pending;
+  std::vector<hipEvent_t> event_pool;
+  dpah::Tuning tune;
+  std::recursive_mutex mu;
+
+  void* alloc(uint64_t bytes);
+  void release(void* p, uint64_t bytes);
+  hipEvent_t take_event();
+  void resolve_timings();  // call after the stream is synchronised
+  template <class F>
+  void timed(const char* name, F&& launch) {
+    if (!timing) { launch(); return; }
+    dpah::PendingTiming t{name, take_event(), take_event()};
+    DPAH_HIP(hipEventRecord(t.start, stream));
+    launch();
+    DPAH_HIP(hipEventRecord(t.stop, stream));
+    pending.push_back(t);
+  }
+  void sync() {
+    DPAH_HIP(hipStreamSynchronize(stream));
+    resolve_timings();
+  }
+};
+
+struct dpah_field {
+  dpah_context* ctx = nullptr;
+  float* d_data = nullptr;
+  uint64_t bytes = 0;
+  dpah::FieldView view{};
+  uint64_t shape[3] = {0, 0, 0};
+  uint64_t time_steps = 1;
+};
+
+namespace dpah {
+
+struct Chunk {
+  float* vertices = nullptr;
+  float* colors = nullptr;
+  void* indices = nullptr;
+  uint64_t V = 0, I = 0;   // vertices, index entries
+  uint64_t bytes = 0;
+};
+
+struct Round {
+  std::vector<Chunk> chunks;
+  uint64_t V = 0, I = 0;
+};
+
+}  // namespace dpah
+
+struct dpah_result {
+  dpah_context* ctx = nullptr;
+  std::vector<dpah::Round> rounds;
+  int use64 = 0;
+  std::vector<float> final_pos;
+  std::vector<uint64_t> final_rem;
+  std::vector<uint8_t> final_status;
+  uint64_t total_steps = 0;
+  ~dpah_result();
+};
+
+namespace dpah {
+
+// RAII device buffer owned by a context.
+struct DeviceBuffer {
+  dpah_context* ctx = nullptr;
+  void* ptr = nullptr;
+  uint64_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(dpah_context* c, uint64_t b) : ctx(c), ptr(c->alloc(b)), bytes(b) {}
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ctx(o.ctx), ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+    reset();
+    ctx = o.ctx; ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr;
+    return *this;
+  }
+  ~DeviceBuffer() { reset(); }
+  void reset() {
+    if (ptr && ctx) ctx->release(ptr, bytes);
+    ptr = nullptr;
+  }
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Launchers (advect.hip / compact.hip).
+struct AdvectLaunch {
+  const FieldView* f;
+  const float* seeds;
+  const uint64_t* rem_in;
+  uint64_t uniform_rem;
+  float* pos_out;
+  uint64_t* rem_out;
+  uint8_t* status_out;
+  uint32_t* steps_out;
+  uint8_t* special_out;
+  float* slots;
+  uint64_t stride;
+  uint64_t n;
+  float dt;
+  float t0;
+  int integrator;
+  bool persistent;
+};
+void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
+
+// Prunes, indexes and colours `count` slot rows; allocates the chunk's outputs.
+Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const uint32_t* steps,
+                   const uint8_t* special, uint64_t count, uint64_t vertex_base, int use64);
+
+}  // namespace dpah

@@ -1,0 +1,155 @@
+/*
+ * dpa_hip.h -- C ABI of the MI355X particle-advection library (libdpa_hip.so).
+ *
+ * Drop-in boundary for one dpa rank's advection stage.  dpa has no C API: the
+ * entry points below replace the C++ stage objects a single rank of dpa's
+ * pipeline calls (source/pipeline.cpp:91-171), with plain pointers and sizes.
+ * Each declaration cites the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns DPAH_OK (0) or a DPAH_ERR_* code; no C++
+ *     exception crosses the ABI.  dpah_last_error() gives the calling
+ *     thread's last message.
+ *   - Positions are float32 (x0, x1, x2) triples in dpa's component order:
+ *     component i indexes grid axis i (regular_grid.hpp:32, 45, 61-62).
+ *   - Fields are AoS float32 [n0][n1][n2][3], axis 0 slowest, exactly the
+ *     layout dpa's loader reads (regular_grid_loader.cpp:65-88).  The vector
+ *     applied to axis i is stored component 2-i (particle_advector.cpp:369).
+ *   - Functions named *_device take device pointers (from dpah_device_alloc);
+ *     all others take host pointers.
+ *   - A context serialises its own calls; use one context per thread for
+ *     concurrency.
+ */
+#ifndef DPA_HIP_H
+#define DPA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPAH_OK              0
+#define DPAH_ERR_INVALID     1  /* bad argument                               */
+#define DPAH_ERR_HIP         2  /* HIP runtime error                          */
+#define DPAH_ERR_NOMEM       3  /* allocation failed or budget exceeded       */
+#define DPAH_ERR_ADMISSION   4  /* request does not fit the memory budget     */
+#define DPAH_ERR_CANCELLED   5  /* request was cancelled                      */
+#define DPAH_ERR_UNSUPPORTED 6  /* configuration not supported                */
+#define DPAH_ERR_TIMEOUT     7  /* wait timed out                             */
+#define DPAH_ERR_PENDING     8  /* result not ready yet                       */
+
+/* particle_advector_integrator values (particle_advector.cpp:29-36).
+ * DPA_RK4 is dpa's "runge_kutta_4": one field sample per step, odeint's
+ * four-term sum.  RK4 samples four times (classic RK4; not a dpa mode). */
+#define DPAH_INTEGRATOR_DPA_RK4 0
+#define DPAH_INTEGRATOR_RK4     1
+
+/* Particle status after advection (where dpa moves the particle to
+ * output.inactive_particles, particle_advector.cpp:353, 365, 395). */
+#define DPAH_STATUS_ACTIVE    0
+#define DPAH_STATUS_COMPLETED 1  /* remaining_iterations reached 0        */
+#define DPAH_STATUS_EXITED    2  /* failed regular_grid::contains         */
+#define DPAH_STATUS_ZERO      3  /* interpolated vector isZero()          */
+
+typedef struct dpah_context dpah_context;
+typedef struct dpah_field   dpah_field;
+typedef struct dpah_result  dpah_result;
+
+typedef struct dpah_options {
+  uint64_t    memory_budget_bytes;  /* 0: 90% of free device memory at create */
+  uint64_t    slot_budget_bytes;    /* record-mode staging per chunk; 0: 4 GiB */
+  int         kernel_timing;        /* 1: time advect kernels with HIP events */
+  const char* autotune_cache_path;  /* nullable */
+} dpah_options;
+
+/* Last error message of the calling thread ("" if none). */
+const char* dpah_last_error(void);
+int dpah_version(void);
+
+/* Context on one GPU (replaces the per-rank process set up at pipeline.cpp:24-47). */
+int dpah_create(int device, const dpah_options* options, dpah_context** out);
+int dpah_destroy(dpah_context* ctx);
+int dpah_synchronize(dpah_context* ctx);
+
+/* Device memory helpers for the *_device entry points. */
+int dpah_device_alloc(dpah_context* ctx, uint64_t bytes, void** out);
+int dpah_device_free(dpah_context* ctx, void* ptr);
+int dpah_copy_to_device(dpah_context* ctx, void* dst, const void* src, uint64_t bytes);
+int dpah_copy_to_host(dpah_context* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* regular_grid_loader::load_vector_field (regular_grid_loader.cpp:65-88) and the
+ * regular_vector_field_3d it returns (regular_fields.hpp:14): copies the field
+ * to HBM, where it stays resident until released.  offset/spacing are the
+ * grid's physical origin and node spacing (offset = ghosted_offset * spacing,
+ * 86; on one rank 0). */
+int dpah_field_create(dpah_context* ctx, const float* xyzv, const uint64_t shape[3],
+                      const float offset[3], const float spacing[3], dpah_field** out);
+/* Time-variant field regular_time_variant_vector_field_3d (regular_fields.hpp:27):
+ * time_steps slices of [n0][n1][n2][3], slice t at time_offset + t*time_spacing. */
+int dpah_field_create_unsteady(dpah_context* ctx, const float* xyzv, const uint64_t shape[3],
+                               uint64_t time_steps, const float offset[3],
+                               const float spacing[3], float time_offset, float time_spacing,
+                               dpah_field** out);
+int dpah_field_release(dpah_field* field);
+
+/* Constructor arguments of particle_advector (particle_advector.hpp:124-131)
+ * that the single-GPU path uses, plus index_generator's width flag
+ * (index_generator.hpp:11; dpa's rule is pipeline.cpp:155). */
+typedef struct dpah_advect_params {
+  uint64_t particles_per_round;  /* particle_advector_particles_per_round */
+  float    step_size;            /* particle_advector_step_size           */
+  float    start_time;           /* time-variant fields: t at iteration 0 */
+  int      integrator;           /* DPAH_INTEGRATOR_*                     */
+  int      record;               /* particle_advector_record              */
+  int      use_64_bit_indices;   /* index_generator::generate use_64_bit  */
+} dpah_advect_params;
+
+/* One rank's advection stage with load_balancer "none": the round loop of
+ * pipeline.cpp:98-147 (compute_round_state, allocate_integral_curves, advect,
+ * prune_integral_curves; particle_advector.cpp:261-408) followed by
+ * index_generator::generate and color_generator::generate_from_angular_velocities
+ * (pipeline.cpp:153-159).  seeds: count x 3 floats; remaining: count values or
+ * NULL to give every particle uniform_remaining (seed_generation_iterations).
+ * Rounds take the tail of the seed array first, as dpa does (327). */
+int dpah_advect(dpah_context* ctx, const dpah_field* field, const float* seeds,
+                const uint64_t* remaining, uint64_t uniform_remaining, uint64_t count,
+                const dpah_advect_params* params, dpah_result** out);
+
+/* The per-round integral_curve (integral_curves.hpp:11-18) and the inactive
+ * particle set (particle_advector.hpp:85-89) of a finished advection.
+ * index_count counts index entries (two per segment), like indices.size(). */
+int dpah_result_round_count(const dpah_result* r, uint64_t* rounds);
+int dpah_result_round_sizes(const dpah_result* r, uint64_t round, uint64_t* vertex_count,
+                            uint64_t* index_count, int* index_bytes);
+int dpah_result_copy_round(const dpah_result* r, uint64_t round, float* vertices, float* colors,
+                           void* indices);
+int dpah_result_final_state(const dpah_result* r, float* positions, uint64_t* remaining,
+                            uint8_t* status);
+int dpah_result_total_steps(const dpah_result* r, uint64_t* steps);
+int dpah_result_free(dpah_result* r);
+
+/* Endpoint advection (record off) on device-resident particles, the form the
+ * FTLE and batched paths use: the particle loop of particle_advector.cpp:325-396
+ * over count particles.  seeds/positions_out are device count x 3 floats;
+ * remaining may be NULL (uniform_remaining for all).  remaining_out / status_out
+ * may be NULL.  If total_steps is non-NULL the call waits for the kernel and
+ * stores the number of particle-steps taken; otherwise it returns after the
+ * launch (stream-ordered on the context's stream). */
+int dpah_advect_endpoints_device(dpah_context* ctx, const dpah_field* field, const float* seeds,
+                                 const uint64_t* remaining, uint64_t uniform_remaining,
+                                 uint64_t count, float step_size, float start_time,
+                                 int integrator, float* positions_out, uint64_t* remaining_out,
+                                 uint8_t* status_out, uint64_t* total_steps);
+
+/* Kernel timing recorded with HIP events on the context's stream (enabled by
+ * dpah_options.kernel_timing): launches and summed milliseconds of the named
+ * kernel ("advect_persistent", "advect_simple", "compact_write", ...). */
+int dpah_kernel_timing(dpah_context* ctx, const char* kernel, uint64_t* launches, double* total_ms);
+int dpah_kernel_timing_reset(dpah_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
