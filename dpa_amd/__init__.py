@@ -82,6 +82,12 @@ def load_library() -> C.CDLL:
         "dpah_advect_endpoints_device": (C.c_int, [vp, vp, vp, vp, C.c_uint64, C.c_uint64,
                                                    C.c_float, C.c_float, C.c_int, vp, vp, vp,
                                                    _u64p]),
+        "dpah_ftle_estimate_device": (C.c_int, [vp, _u64p, _f32p, _f32p, _f32p, C.c_uint64,
+                                                C.c_float, vp, vp, C.c_uint64, vp, _u64p]),
+        "dpah_ftle_lattice": (C.c_int, [vp, vp, _f32p, C.c_uint64, C.c_float, C.c_int, vp,
+                                        _u64p, _u64p]),
+        "dpah_ftle_aux_device": (C.c_int, [vp, vp, _f32p, _f32p, _u64p, _f32p, C.c_uint64,
+                                           C.c_float, C.c_float, C.c_int, vp, _u64p]),
         "dpah_kernel_timing": (C.c_int, [vp, C.c_char_p, _u64p, C.POINTER(C.c_double)]),
         "dpah_kernel_timing_reset": (C.c_int, [vp]),
     }
@@ -208,6 +214,39 @@ class Context:
             C.c_void_p(d_status_out) if d_status_out else None,
             C.byref(steps) if wait else None))
         return steps.value if wait else None
+
+    def ftle_lattice(self, field: "Field", stride, iterations: int, step_size: float,
+                     integrator: int = INTEGRATOR_DPA_RK4):
+        """dpa's FTLE stage (lattice seeds, advection, estimate); host volume."""
+        shape = (C.c_uint64 * 3)()
+        _check(self.lib.dpah_ftle_lattice(self.h, field.h, _f3(stride), int(iterations),
+                                          float(step_size), int(integrator), None, shape, None))
+        out = np.empty(tuple(shape), np.float32)
+        steps = C.c_uint64()
+        _check(self.lib.dpah_ftle_lattice(self.h, field.h, _f3(stride), int(iterations),
+                                          float(step_size), int(integrator), _ptr(out), shape,
+                                          C.byref(steps)))
+        return out, steps.value
+
+    def ftle_estimate_device(self, field_shape, offset, spacing, stride, iterations, step_size,
+                             d_original: int, d_final: int, count: int, d_out: int | None):
+        shape = (C.c_uint64 * 3)()
+        _check(self.lib.dpah_ftle_estimate_device(
+            self.h, _u3(field_shape), _f3(offset), _f3(spacing), _f3(stride), int(iterations),
+            float(step_size), C.c_void_p(d_original) if d_original else None,
+            C.c_void_p(d_final) if d_final else None, int(count),
+            C.c_void_p(d_out) if d_out else None, shape))
+        return tuple(int(x) for x in shape)
+
+    def ftle_aux_device(self, field: "Field", origin, spacing, shape, delta, iterations: int,
+                        step_size: float, d_out: int, start_time: float = 0.0,
+                        integrator: int = INTEGRATOR_DPA_RK4) -> int:
+        steps = C.c_uint64()
+        _check(self.lib.dpah_ftle_aux_device(self.h, field.h, _f3(origin), _f3(spacing),
+                                             _u3(shape), _f3(delta), int(iterations),
+                                             float(step_size), float(start_time), int(integrator),
+                                             C.c_void_p(d_out), C.byref(steps)))
+        return steps.value
 
     def kernel_timing(self, name: str) -> tuple[int, float]:
         n = C.c_uint64()

@@ -49,6 +49,8 @@ __global__ void k_reset_counters(unsigned long long* c) {
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   AdvectArgs a{};
   a.f = *L.f;
+  a.seed_kind = L.seed_kind;
+  a.lat = L.lat;
   a.seeds = L.seeds;
   a.rem_in = L.rem_in;
   a.uniform_rem = L.uniform_rem;

@@ -13,9 +13,26 @@
 
 namespace dpah {
 
+// Where a particle's start position comes from.
+enum SeedKind : int {
+  kSeedArray = 0,    // seeds[pid]
+  kSeedLattice = 1,  // lattice point pid in C order (uniform_seed_generator.cpp:33-36)
+  kSeedAux = 2,      // pid = 6*point + a: lattice point +- delta along axis a/2
+};
+
+struct SeedLattice {
+  float lo0, lo1, lo2;        // offset
+  float st0, st1, st2;        // stride (physical)
+  uint64_t d0, d1, d2;        // points per axis
+  float dl0, dl1, dl2;        // auxiliary offsets (kSeedAux)
+  uint64_t first;             // lattice index of particle / point 0 (chunked launches)
+};
+
 struct AdvectArgs {
   FieldView f;
-  const float*    seeds;        // [n][3]
+  int             seed_kind;
+  SeedLattice     lat;
+  const float*    seeds;        // [n][3] (kSeedArray)
   const uint64_t* rem_in;       // [n], or null -> uniform_rem for every particle
   uint64_t        uniform_rem;
   float*          pos_out;      // [n][3]
@@ -60,11 +77,44 @@ struct Lane {
   bool special;
 };
 
+// position = offset + stride * multi_index, each product and sum rounded
+// (uniform_seed_generator.cpp:35-36 with indexing.hpp:10-20's C-order unravel).
+__device__ __forceinline__ void lattice_point(const SeedLattice& g, uint64_t i, float& p0,
+                                              float& p1, float& p2) {
+  i += g.first;
+  const uint64_t i2 = i % g.d2;
+  const uint64_t r = i / g.d2;
+  const uint64_t i1 = r % g.d1;
+  const uint64_t i0 = r / g.d1;
+  { const float m = g.st0 * (float)i0; p0 = g.lo0 + m; }
+  { const float m = g.st1 * (float)i1; p1 = g.lo1 + m; }
+  { const float m = g.st2 * (float)i2; p2 = g.lo2 + m; }
+}
+
+__device__ __forceinline__ void seed_position(const AdvectArgs& a, uint64_t pid, float& p0,
+                                              float& p1, float& p2) {
+  if (a.seed_kind == kSeedArray) {
+    const float* s = a.seeds + 3 * pid;
+    p0 = s[0]; p1 = s[1]; p2 = s[2];
+  } else if (a.seed_kind == kSeedLattice) {
+    lattice_point(a.lat, pid, p0, p1, p2);
+  } else {
+    const uint64_t point = pid / 6;
+    const uint32_t aux = (uint32_t)(pid - 6 * point);
+    lattice_point(a.lat, point, p0, p1, p2);
+    const bool plus = aux & 1;
+    switch (aux >> 1) {
+      case 0: p0 = plus ? p0 + a.lat.dl0 : p0 - a.lat.dl0; break;
+      case 1: p1 = plus ? p1 + a.lat.dl1 : p1 - a.lat.dl1; break;
+      default: p2 = plus ? p2 + a.lat.dl2 : p2 - a.lat.dl2; break;
+    }
+  }
+}
+
 template <bool kRecord>
 __device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uint64_t pid) {
   L.pid = pid;
-  const float* s = a.seeds + 3 * pid;
-  L.p0 = s[0]; L.p1 = s[1]; L.p2 = s[2];
+  seed_position(a, pid, L.p0, L.p1, L.p2);
   L.rem = a.rem_in ? a.rem_in[pid] : a.uniform_rem;
   L.it = 0;
   if constexpr (kRecord) {

@@ -9,33 +9,15 @@
 
 #include "runtime.hpp"
 
-#define DPAH_EXPORT extern "C" __attribute__((visibility("default")))
 
 namespace dpah {
 
-namespace {
-thread_local std::string g_error;
+std::string& error_slot() {
+  thread_local std::string msg;
+  return msg;
 }
 
-void set_error(const std::string& msg) { g_error = msg; }
-
-template <class F>
-int guarded(F&& body) {
-  try {
-    g_error.clear();
-    body();
-    return DPAH_OK;
-  } catch (const Error& e) {
-    g_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_error = "host allocation failed";
-    return DPAH_ERR_NOMEM;
-  } catch (const std::exception& e) {
-    g_error = e.what();
-    return DPAH_ERR_INVALID;
-  }
-}
+void set_error(const std::string& msg) { error_slot() = msg; }
 
 }  // namespace dpah
 
@@ -96,7 +78,7 @@ dpah_result::~dpah_result() {
       if (c.vertices) ctx->release(c.vertices, c.bytes);
 }
 
-DPAH_EXPORT const char* dpah_last_error(void) { return dpah::g_error.c_str(); }
+DPAH_EXPORT const char* dpah_last_error(void) { return dpah::error_slot().c_str(); }
 
 DPAH_EXPORT int dpah_version(void) { return 1; }
 

@@ -6,13 +6,14 @@
 
 #include <atomic>
 #include <map>
+#include <new>
 #include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/dpa_hip.h"
-#include "device_math.hpp"
+#include "advect_kernels.hpp"
 
 namespace dpah {
 
@@ -34,6 +35,28 @@ struct Error : std::runtime_error {
   } while (0)
 
 void set_error(const std::string& msg);
+std::string& error_slot();   // the calling thread's last error message
+
+#define DPAH_EXPORT extern "C" __attribute__((visibility("default")))
+
+// Runs an entry point's body, turning exceptions into status codes.
+template <class F>
+int guarded(F&& body) {
+  try {
+    error_slot().clear();
+    body();
+    return DPAH_OK;
+  } catch (const Error& e) {
+    error_slot() = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    error_slot() = "host allocation failed";
+    return DPAH_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    error_slot() = e.what();
+    return DPAH_ERR_INVALID;
+  }
+}
 
 struct TimedKernel {
   uint64_t launches = 0;
@@ -153,6 +176,8 @@ struct DeviceBuffer {
 // Launchers (advect.hip / compact.hip).
 struct AdvectLaunch {
   const FieldView* f;
+  int seed_kind;          // SeedKind
+  SeedLattice lat;
   const float* seeds;
   const uint64_t* rem_in;
   uint64_t uniform_rem;

@@ -143,6 +143,45 @@ int dpah_advect_endpoints_device(dpah_context* ctx, const dpah_field* field, con
                                  int integrator, float* positions_out, uint64_t* remaining_out,
                                  uint8_t* status_out, uint64_t* total_steps);
 
+/* ftle_estimator::estimate (ftle_estimator.cpp:7-65) on device data: the final
+ * positions are scattered onto the strided lattice of the original positions
+ * (shape size_t(n_i / stride_i), spacing h_i * stride_i, cell of
+ * floor((p - offset) / spacing)), differentiated by regular_grid::gradient
+ * (regular_grid.hpp:87-108: clamped neighbours, divisor 2 * spacing), and
+ * ln(sqrt(lmax(J^T J))) / |step_size * iterations| is written per lattice cell.
+ * field_shape/offset/spacing describe the advected field; stride is in cells.
+ * out_shape always receives the lattice shape; ftle_out (device, may be NULL
+ * to query the shape) receives out_shape[0]*out_shape[1]*out_shape[2] floats.
+ * Two particles landing in one cell: the higher particle index wins (dpa leaves
+ * the order to TBB). */
+int dpah_ftle_estimate_device(dpah_context* ctx, const uint64_t field_shape[3],
+                              const float offset[3], const float spacing[3],
+                              const float stride[3], uint64_t iterations, float step_size,
+                              const float* original_positions, const float* final_positions,
+                              uint64_t count, float* ftle_out, uint64_t out_shape[3]);
+
+/* One rank's FTLE stage as dpa's pipeline runs it (pipeline.cpp:65-72, 91-147,
+ * 165-167): lattice seeds of uniform_seed_generator::generate
+ * (uniform_seed_generator.cpp:16-45) with stride in cells over the whole field,
+ * endpoint advection for `iterations` steps, then ftle_estimator::estimate.
+ * ftle_out is host memory (NULL: only out_shape is filled). */
+int dpah_ftle_lattice(dpah_context* ctx, const dpah_field* field, const float stride[3],
+                      uint64_t iterations, float step_size, int integrator, float* ftle_out,
+                      uint64_t out_shape[3], uint64_t* total_steps);
+
+/* FTLE with auxiliary neighbours (no counterpart in dpa; replaces the scatter
+ * and one-sided borders of ftle_estimator.cpp:39-45 by central differences
+ * that hold at the border): for every lattice point p = origin + spacing * i
+ * (C order over lattice_shape) six particles start at p -/+ delta_d along each
+ * axis d; J[:, d] = (phi(p + delta_d) - phi(p - delta_d)) / (2 delta_d);
+ * ftle_out (device, one float per point) = ln(sqrt(lmax(J^T J))) /
+ * |step_size * iterations|. */
+int dpah_ftle_aux_device(dpah_context* ctx, const dpah_field* field, const float origin[3],
+                         const float lattice_spacing[3], const uint64_t lattice_shape[3],
+                         const float delta[3], uint64_t iterations, float step_size,
+                         float start_time, int integrator, float* ftle_out,
+                         uint64_t* total_steps);
+
 /* Kernel timing recorded with HIP events on the context's stream (enabled by
  * dpah_options.kernel_timing): launches and summed milliseconds of the named
  * kernel ("advect_persistent", "advect_simple", "compact_write", ...). */

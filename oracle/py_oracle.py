@@ -249,3 +249,17 @@ def ftle_aux(finals, delta, T) -> np.ndarray:
 def sym3_max_eigenvalue(m) -> float:
     a = np.ascontiguousarray(m, np.float64).reshape(9)
     return float(load().dpo_sym3_max_eigenvalue(_p(a)))
+
+
+def aux_seeds(origin, spacing, shape, delta) -> np.ndarray:
+    """Seeds of the auxiliary-neighbour FTLE: for each lattice point
+    p = origin + spacing * i (float32, C order) the six points p -/+ delta_d
+    along axes 0, 1, 2, in that order; shape [points * 6, 3]."""
+    o, h, dl = _f32(origin), _f32(spacing), _f32(delta)
+    idx = np.indices(tuple(int(s) for s in shape)).reshape(3, -1).T
+    base = (o[None, :] + (h[None, :] * idx.astype(np.float32)).astype(np.float32)).astype(np.float32)
+    out = np.repeat(base[:, None, :], 6, axis=1)
+    for d in range(3):
+        out[:, 2 * d, d] = (base[:, d] - dl[d]).astype(np.float32)
+        out[:, 2 * d + 1, d] = (base[:, d] + dl[d]).astype(np.float32)
+    return out.reshape(-1, 3)
