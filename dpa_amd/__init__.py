@@ -44,6 +44,12 @@ class _AdvectParams(C.Structure):
                 ("use_64_bit_indices", C.c_int)]
 
 
+class _Job(C.Structure):
+    _fields_ = [("field", C.c_void_p), ("seeds", C.c_void_p), ("remaining", C.c_void_p),
+                ("uniform_remaining", C.c_uint64), ("count", C.c_uint64),
+                ("params", _AdvectParams)]
+
+
 _lib = None
 
 
@@ -73,6 +79,7 @@ def load_library() -> C.CDLL:
         "dpah_field_release": (C.c_int, [vp]),
         "dpah_advect": (C.c_int, [vp, vp, vp, vp, C.c_uint64, C.c_uint64,
                                   C.POINTER(_AdvectParams), C.POINTER(vp)]),
+        "dpah_advect_batch": (C.c_int, [vp, C.POINTER(_Job), C.c_uint64, C.POINTER(vp)]),
         "dpah_result_round_count": (C.c_int, [vp, _u64p]),
         "dpah_result_round_sizes": (C.c_int, [vp, C.c_uint64, _u64p, _u64p, C.POINTER(C.c_int)]),
         "dpah_result_copy_round": (C.c_int, [vp, C.c_uint64, vp, vp, vp]),
@@ -199,6 +206,31 @@ class Context:
         _check(self.lib.dpah_advect(self.h, field.h, _ptr(seeds), _ptr(rem),
                                     int(uniform_remaining), n, C.byref(p), C.byref(h)))
         return Result(self, h, n)
+
+    def advect_batch(self, jobs: list[dict]) -> list["Result"]:
+        """jobs: dicts with field, seeds and the keyword arguments of advect()."""
+        keep = []
+        arr = (_Job * max(len(jobs), 1))()
+        for i, j in enumerate(jobs):
+            seeds = np.ascontiguousarray(j["seeds"], dtype=np.float32).reshape(-1, 3)
+            rem = j.get("remaining")
+            rem = None if rem is None else np.ascontiguousarray(rem, dtype=np.uint64)
+            keep += [seeds, rem]
+            n = seeds.shape[0]
+            ppr = j.get("particles_per_round") or max(n, 1)
+            arr[i].field = j["field"].h
+            arr[i].seeds = seeds.ctypes.data if n else None
+            arr[i].remaining = None if rem is None else rem.ctypes.data
+            arr[i].uniform_remaining = int(j.get("uniform_remaining", 0))
+            arr[i].count = n
+            arr[i].params = _AdvectParams(int(ppr), float(j.get("step_size", 0.1)),
+                                          float(j.get("start_time", 0.0)),
+                                          int(j.get("integrator", INTEGRATOR_DPA_RK4)),
+                                          1 if j.get("record", True) else 0,
+                                          1 if j.get("use_64_bit", False) else 0)
+        hs = (C.c_void_p * max(len(jobs), 1))()
+        _check(self.lib.dpah_advect_batch(self.h, arr, len(jobs), hs))
+        return [Result(self, C.c_void_p(hs[i]), arr[i].count) for i in range(len(jobs))]
 
     def advect_endpoints_device(self, field: "Field", d_seeds: int, count: int, d_pos_out: int,
                                 d_remaining: int | None = None, uniform_remaining: int = 0,

@@ -49,6 +49,8 @@ __global__ void k_reset_counters(unsigned long long* c) {
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   AdvectArgs a{};
   a.f = *L.f;
+  a.segs = L.segs;
+  a.nseg = L.nseg;
   a.seed_kind = L.seed_kind;
   a.lat = L.lat;
   a.seeds = L.seeds;

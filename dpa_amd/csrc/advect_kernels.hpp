@@ -28,8 +28,22 @@ struct SeedLattice {
   uint64_t first;             // lattice index of particle / point 0 (chunked launches)
 };
 
+// A contiguous run of particles with one slot layout: one dpa round of one job.
+// The queue hands out ordinals q; segment s owns [qbase, qbase + count) and maps
+// ordinal q to particle first + (q - qbase), whose slot row starts at
+// slots + slot_base + (q - qbase) * stride * 3.
+struct Segment {
+  uint64_t qbase;
+  uint64_t first;
+  uint64_t count;
+  uint64_t slot_base;   // in floats
+  uint64_t stride;      // slots per row
+};
+
 struct AdvectArgs {
   FieldView f;
+  const Segment*  segs;         // nseg > 0: batched form; 0: one implicit segment
+  uint32_t        nseg;
   int             seed_kind;
   SeedLattice     lat;
   const float*    seeds;        // [n][3] (kSeedArray)
@@ -70,6 +84,7 @@ __device__ __forceinline__ void add_wave_steps(unsigned long long* total, uint64
 
 // Particle registers carried across the step loop.
 struct Lane {
+  float* row;      // record: this particle's slot row
   uint64_t pid;
   uint64_t rem;
   uint32_t it;
@@ -111,15 +126,35 @@ __device__ __forceinline__ void seed_position(const AdvectArgs& a, uint64_t pid,
   }
 }
 
+// Queue ordinal -> particle index and slot row (binary search over segments).
+__device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, float*& row) {
+  if (a.nseg == 0) {
+    row = a.slots + 3 * q * a.stride;
+    return q;
+  }
+  uint32_t lo = 0, hi = a.nseg;            // last segment with qbase <= q
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.segs[mid].qbase <= q) lo = mid; else hi = mid;
+  }
+  const Segment& s = a.segs[lo];
+  const uint64_t local = q - s.qbase;
+  row = a.slots + s.slot_base + 3 * local * s.stride;
+  return s.first + local;
+}
+
 template <bool kRecord>
-__device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uint64_t pid) {
+__device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uint64_t q) {
+  float* row = nullptr;
+  const uint64_t pid = resolve(a, q, row);
   L.pid = pid;
   seed_position(a, pid, L.p0, L.p1, L.p2);
   L.rem = a.rem_in ? a.rem_in[pid] : a.uniform_rem;
   L.it = 0;
   if constexpr (kRecord) {
+    L.row = row;
     L.special = is_sentinel(L.p0, L.p1, L.p2);
-    put3(a.slots + 3 * (pid * a.stride), L.p0, L.p1, L.p2);
+    put3(row, L.p0, L.p1, L.p2);
   }
 }
 
@@ -127,7 +162,7 @@ template <bool kRecord>
 __device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const Lane& L, uint8_t st) {
   const uint64_t pid = L.pid;
   if constexpr (kRecord) {
-    put3(a.slots + 3 * (pid * a.stride + (uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
+    put3(L.row + 3 * ((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
     if (a.special_out) a.special_out[pid] = L.special ? 1 : 0;
   }
   put3(a.pos_out + 3 * pid, L.p0, L.p1, L.p2);
@@ -158,7 +193,7 @@ __device__ __forceinline__ uint8_t step_once(const AdvectArgs& a, Lane& L) {
   L.rem -= 1;
   L.it += 1;
   if constexpr (kRecord) {
-    put3(a.slots + 3 * (L.pid * a.stride + (uint64_t)L.it), L.p0, L.p1, L.p2);
+    put3(L.row + 3 * (uint64_t)L.it, L.p0, L.p1, L.p2);
     L.special |= is_sentinel(L.p0, L.p1, L.p2);
   }
   return L.rem == 0 ? kCompleted : kActive;
@@ -186,6 +221,7 @@ template <bool kRecord, int kIntegrator, bool kUnsteady>
 __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   const uint32_t lane = threadIdx.x & 63;
   Lane L;
+  L.row = nullptr;
   L.pid = ~0ull;
   L.rem = 0; L.it = 0; L.p0 = L.p1 = L.p2 = 0.0f; L.special = false;
   bool open = true;       // wave-uniform: the queue still has particles

@@ -178,6 +178,8 @@ struct AdvectLaunch {
   const FieldView* f;
   int seed_kind;          // SeedKind
   SeedLattice lat;
+  const Segment* segs;    // device table (batched form) or null
+  uint32_t nseg;
   const float* seeds;
   const uint64_t* rem_in;
   uint64_t uniform_rem;

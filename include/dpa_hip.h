@@ -117,6 +117,25 @@ int dpah_advect(dpah_context* ctx, const dpah_field* field, const float* seeds,
                 const uint64_t* remaining, uint64_t uniform_remaining, uint64_t count,
                 const dpah_advect_params* params, dpah_result** out);
 
+/* One seed set of a batch: the arguments of one dpah_advect call. */
+typedef struct dpah_job {
+  const dpah_field*  field;
+  const float*       seeds;              /* host, count x 3 */
+  const uint64_t*    remaining;          /* host, count, or NULL */
+  uint64_t           uniform_remaining;
+  uint64_t           count;
+  dpah_advect_params params;
+} dpah_job;
+
+/* Many independent advection stages at once (no counterpart in dpa, whose
+ * ranks balance load with diffusion between neighbours,
+ * particle_advector.cpp:65-260).  Jobs sharing field, step size, start time,
+ * integrator and record flag run in ONE persistent launch whose on-device work
+ * queue holds every round of every job, longest rows first.  results[i] gets a
+ * result equal, bit for bit, to dpah_advect on jobs[i] alone. */
+int dpah_advect_batch(dpah_context* ctx, const dpah_job* jobs, uint64_t job_count,
+                      dpah_result** results);
+
 /* The per-round integral_curve (integral_curves.hpp:11-18) and the inactive
  * particle set (particle_advector.hpp:85-89) of a finished advection.
  * index_count counts index entries (two per segment), like indices.size(). */
