@@ -50,6 +50,36 @@ class _Job(C.Structure):
                 ("params", _AdvectParams)]
 
 
+class _ServerOptions(C.Structure):
+    _fields_ = [("memory_budget_bytes", C.c_uint64), ("max_active", C.c_uint32),
+                ("steps_per_tick", C.c_uint32), ("mode", C.c_int)]
+
+
+class _Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("field", C.c_void_p), ("seeds", C.c_void_p),
+                ("remaining", C.c_void_p), ("uniform_remaining", C.c_uint64),
+                ("count", C.c_uint64), ("params", _AdvectParams), ("origin", C.c_float * 3),
+                ("lattice_spacing", C.c_float * 3), ("lattice_shape", C.c_uint64 * 3),
+                ("delta", C.c_float * 3), ("iterations", C.c_uint64)]
+
+
+class _RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int), ("queue_seconds", C.c_double),
+                ("latency_seconds", C.c_double), ("particle_steps", C.c_uint64),
+                ("footprint_bytes", C.c_uint64)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("cancelled", C.c_uint64), ("failed", C.c_uint64), ("pending", C.c_uint64),
+                ("ticks", C.c_uint64), ("particle_steps", C.c_uint64),
+                ("bytes_in_use", C.c_uint64), ("busy_seconds", C.c_double)]
+
+
+REQUEST_ADVECT, REQUEST_FTLE_AUX = 0, 1
+SERVE_BATCHED, SERVE_PER_REQUEST = 0, 1
+TICKET_PENDING, TICKET_RUNNING, TICKET_DONE, TICKET_FAILED, TICKET_CANCELLED = range(5)
+
 _lib = None
 
 
@@ -95,6 +125,16 @@ def load_library() -> C.CDLL:
                                         _u64p, _u64p]),
         "dpah_ftle_aux_device": (C.c_int, [vp, vp, _f32p, _f32p, _u64p, _f32p, C.c_uint64,
                                            C.c_float, C.c_float, C.c_int, vp, _u64p]),
+        "dpah_server_create": (C.c_int, [vp, C.POINTER(_ServerOptions), C.POINTER(vp)]),
+        "dpah_server_submit": (C.c_int, [vp, C.POINTER(_Request), _u64p]),
+        "dpah_server_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(C.c_int)]),
+        "dpah_server_wait": (C.c_int, [vp, C.c_uint64, C.c_int64, C.POINTER(vp)]),
+        "dpah_server_request_info": (C.c_int, [vp, C.c_uint64, C.POINTER(_RequestInfo)]),
+        "dpah_server_cancel": (C.c_int, [vp, C.c_uint64]),
+        "dpah_server_forget": (C.c_int, [vp, C.c_uint64]),
+        "dpah_server_get_stats": (C.c_int, [vp, C.POINTER(_ServerStats)]),
+        "dpah_server_destroy": (C.c_int, [vp]),
+        "dpah_result_ftle": (C.c_int, [vp, vp, _u64p]),
         "dpah_kernel_timing": (C.c_int, [vp, C.c_char_p, _u64p, C.POINTER(C.c_double)]),
         "dpah_kernel_timing_reset": (C.c_int, [vp]),
     }
@@ -343,3 +383,99 @@ class Result:
         v = C.c_uint64()
         _check(self.ctx.lib.dpah_result_total_steps(self.h, C.byref(v)))
         return v.value
+
+
+class Server:
+    """A dpah_server: continuous batching (mode SERVE_BATCHED) or the
+    per-request comparator (SERVE_PER_REQUEST)."""
+
+    def __init__(self, ctx: Context, mode: int = SERVE_BATCHED, memory_budget: int = 0,
+                 max_active: int = 0, steps_per_tick: int = 0):
+        self.ctx, self.lib = ctx, ctx.lib
+        o = _ServerOptions(memory_budget, max_active, steps_per_tick, mode)
+        h = C.c_void_p()
+        _check(self.lib.dpah_server_create(ctx.h, C.byref(o), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self) -> None:
+        if self.h:
+            _check(self.lib.dpah_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit_advect(self, field: Field, seeds, remaining=None, uniform_remaining=0,
+                      particles_per_round=None, step_size=0.1, start_time=0.0,
+                      integrator=INTEGRATOR_DPA_RK4, record=True, use_64_bit=False) -> int:
+        seeds = np.ascontiguousarray(seeds, dtype=np.float32).reshape(-1, 3)
+        rem = None if remaining is None else np.ascontiguousarray(remaining, dtype=np.uint64)
+        n = seeds.shape[0]
+        r = _Request()
+        r.kind = REQUEST_ADVECT
+        r.field = field.h
+        r.seeds = seeds.ctypes.data if n else None
+        r.remaining = None if rem is None else rem.ctypes.data
+        r.uniform_remaining = int(uniform_remaining)
+        r.count = n
+        r.params = _AdvectParams(int(particles_per_round or max(n, 1)), float(step_size),
+                                 float(start_time), int(integrator), 1 if record else 0,
+                                 1 if use_64_bit else 0)
+        t = C.c_uint64()
+        _check(self.lib.dpah_server_submit(self.h, C.byref(r), C.byref(t)))
+        return t.value
+
+    def submit_ftle(self, field: Field, origin, spacing, shape, delta, iterations, step_size,
+                    start_time=0.0, integrator=INTEGRATOR_DPA_RK4) -> int:
+        r = _Request()
+        r.kind = REQUEST_FTLE_AUX
+        r.field = field.h
+        r.params = _AdvectParams(1, float(step_size), float(start_time), int(integrator), 0, 0)
+        r.origin[:] = [float(x) for x in np.asarray(origin, np.float32)]
+        r.lattice_spacing[:] = [float(x) for x in np.asarray(spacing, np.float32)]
+        r.lattice_shape[:] = [int(x) for x in shape]
+        r.delta[:] = [float(x) for x in np.asarray(delta, np.float32)]
+        r.iterations = int(iterations)
+        t = C.c_uint64()
+        _check(self.lib.dpah_server_submit(self.h, C.byref(r), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int) -> int:
+        s = C.c_int()
+        _check(self.lib.dpah_server_poll(self.h, ticket, C.byref(s)))
+        return s.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1, n: int = 0) -> Result:
+        h = C.c_void_p()
+        _check(self.lib.dpah_server_wait(self.h, ticket, timeout_ms, C.byref(h)))
+        return Result(self.ctx, h, n)
+
+    def info(self, ticket: int) -> dict:
+        i = _RequestInfo()
+        _check(self.lib.dpah_server_request_info(self.h, ticket, C.byref(i)))
+        return dict(state=i.state, queue_seconds=i.queue_seconds,
+                    latency_seconds=i.latency_seconds, particle_steps=i.particle_steps,
+                    footprint_bytes=i.footprint_bytes)
+
+    def cancel(self, ticket: int) -> None:
+        _check(self.lib.dpah_server_cancel(self.h, ticket))
+
+    def forget(self, ticket: int) -> None:
+        _check(self.lib.dpah_server_forget(self.h, ticket))
+
+    def stats(self) -> dict:
+        s = _ServerStats()
+        _check(self.lib.dpah_server_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _ServerStats._fields_}
+
+
+def result_ftle(res: Result) -> np.ndarray:
+    n = C.c_uint64()
+    _check(res.ctx.lib.dpah_result_ftle(res.h, None, C.byref(n)))
+    out = np.empty(n.value, np.float32)
+    _check(res.ctx.lib.dpah_result_ftle(res.h, _ptr(out), C.byref(n)))
+    return out

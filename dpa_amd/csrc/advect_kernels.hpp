@@ -84,6 +84,7 @@ __device__ __forceinline__ void add_wave_steps(unsigned long long* total, uint64
 
 // Particle registers carried across the step loop.
 struct Lane {
+  CellCache cc;    // corners of the last sampled cell
   float* row;      // record: this particle's slot row
   uint64_t pid;
   uint64_t rem;
@@ -148,6 +149,7 @@ __device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uin
   float* row = nullptr;
   const uint64_t pid = resolve(a, q, row);
   L.pid = pid;
+  cache_clear(L.cc);
   seed_position(a, pid, L.p0, L.p1, L.p2);
   L.rem = a.rem_in ? a.rem_in[pid] : a.uniform_rem;
   L.it = 0;
@@ -172,23 +174,25 @@ __device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const L
   return L.it;
 }
 
-// One iteration of the loop at particle_advector.cpp:336-389.  Returns the
-// retirement status, or kActive while the particle keeps going.
+// One iteration of the loop at particle_advector.cpp:336-389 on field f with
+// step coefficients c and start time t0.  Returns the retirement status, or
+// kActive while the particle keeps going.
 template <bool kRecord, int kIntegrator, bool kUnsteady>
-__device__ __forceinline__ uint8_t step_once(const AdvectArgs& a, Lane& L) {
+__device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef& c, float t0,
+                                             Lane& L) {
   if (L.rem == 0) return kCompleted;
   float t = 0.0f;
   if constexpr (kUnsteady || kIntegrator == kRk4) {
-    const float ft = (float)L.it * a.coef.dt;   // iteration_index * step_size_
-    t = a.t0 + ft;
+    const float ft = (float)L.it * c.dt;   // iteration_index * step_size_
+    t = t0 + ft;
   }
   V3 v;
-  if (!sample<kUnsteady>(a.f, L.p0, L.p1, L.p2, t, v)) return kExited;
+  if (!sample<kUnsteady>(f, L.p0, L.p1, L.p2, t, v, L.cc)) return kExited;
   if (is_zero(v)) return kZero;
   if constexpr (kIntegrator == kRk4) {
-    if (!step_rk4<kUnsteady>(a.f, L.p0, L.p1, L.p2, v, t, a.coef)) return kExited;
+    if (!step_rk4<kUnsteady>(f, L.p0, L.p1, L.p2, v, t, c, L.cc)) return kExited;
   } else {
-    step_dpa(L.p0, L.p1, L.p2, v, a.coef);
+    step_dpa(L.p0, L.p1, L.p2, v, c);
   }
   L.rem -= 1;
   L.it += 1;
@@ -197,6 +201,11 @@ __device__ __forceinline__ uint8_t step_once(const AdvectArgs& a, Lane& L) {
     L.special |= is_sentinel(L.p0, L.p1, L.p2);
   }
   return L.rem == 0 ? kCompleted : kActive;
+}
+
+template <bool kRecord, int kIntegrator, bool kUnsteady>
+__device__ __forceinline__ uint8_t step_once(const AdvectArgs& a, Lane& L) {
+  return step_lane<kRecord, kIntegrator, kUnsteady>(a.f, a.coef, a.t0, L);
 }
 
 // Per-request form: one thread per particle, runs it to retirement.
@@ -221,6 +230,7 @@ template <bool kRecord, int kIntegrator, bool kUnsteady>
 __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   const uint32_t lane = threadIdx.x & 63;
   Lane L;
+  cache_clear(L.cc);
   L.row = nullptr;
   L.pid = ~0ull;
   L.rem = 0; L.it = 0; L.p0 = L.p1 = L.p2 = 0.0f; L.special = false;

@@ -73,6 +73,7 @@ void dpah_context::resolve_timings() {
 }
 
 dpah_result::~dpah_result() {
+  if (d_ftle) ctx->release(d_ftle, ftle_count * sizeof(float));
   for (auto& r : rounds)
     for (auto& c : r.chunks)
       if (c.vertices) ctx->release(c.vertices, c.bytes);
@@ -263,11 +264,11 @@ void check_integrator(int integrator) {
 
 }  // namespace
 
-DPAH_EXPORT int dpah_advect(dpah_context* ctx, const dpah_field* field, const float* seeds,
-                            const uint64_t* remaining, uint64_t uniform_remaining, uint64_t count,
-                            const dpah_advect_params* params, dpah_result** out) {
-  return guarded([&] {
-    DPAH_REQUIRE(ctx && field && params && out, "null argument");
+dpah_result* dpah::run_advect(dpah_context* ctx, const dpah_field* field, const float* seeds,
+                              const uint64_t* remaining, uint64_t uniform_remaining,
+                              uint64_t count, const dpah_advect_params* params, bool persistent) {
+  {
+    DPAH_REQUIRE(ctx && field && params, "null argument");
     DPAH_REQUIRE(field->ctx == ctx, "field belongs to another context");
     DPAH_REQUIRE(count == 0 || seeds, "seeds is null");
     DPAH_REQUIRE(count == 0 || params->particles_per_round > 0,
@@ -308,7 +309,7 @@ DPAH_EXPORT int dpah_advect(dpah_context* ctx, const dpah_field* field, const fl
     L.dt = params->step_size;
     L.t0 = params->start_time;
     L.integrator = params->integrator;
-    L.persistent = true;
+    L.persistent = persistent;
 
     uint64_t total_steps = 0;
     auto read_steps = [&] {
@@ -386,7 +387,16 @@ DPAH_EXPORT int dpah_advect(dpah_context* ctx, const dpah_field* field, const fl
     }
     ctx->sync();
     res->total_steps = total_steps;
-    *out = res.release();
+    return res.release();
+  }
+}
+
+DPAH_EXPORT int dpah_advect(dpah_context* ctx, const dpah_field* field, const float* seeds,
+                            const uint64_t* remaining, uint64_t uniform_remaining, uint64_t count,
+                            const dpah_advect_params* params, dpah_result** out) {
+  return guarded([&] {
+    DPAH_REQUIRE(out, "null argument");
+    *out = run_advect(ctx, field, seeds, remaining, uniform_remaining, count, params, true);
   });
 }
 

@@ -75,7 +75,7 @@ __device__ __forceinline__ void put_index(const CompactArgs& a, uint64_t pos, ui
 }
 
 // Counts for one row (thread per row).
-__global__ __launch_bounds__(256) void k_compact_count(CompactArgs a) {
+static __global__ __launch_bounds__(256) void k_compact_count(CompactArgs a) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.count) return;
   const uint64_t k = a.steps[p];
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void k_compact_count(CompactArgs a) {
 
 // Serial walk of a row that holds sentinel-valued vertices: literal prune,
 // index and colour rules on the row's kept vertices.
-__device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
+static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
   const uint64_t k = a.steps[p];
   const float* row = a.slots + 3 * p * a.stride;
   const uint64_t vb = a.vcount[p], ib = a.icount[p];
@@ -142,7 +142,7 @@ __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
 }
 
 // Writes: one wave per row, lanes stride over the row's slots.
-__global__ __launch_bounds__(256) void k_compact_write(CompactArgs a) {
+static __global__ __launch_bounds__(256) void k_compact_write(CompactArgs a) {
   const uint64_t p = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t lane = threadIdx.x & 63;
   if (p >= a.count) return;
@@ -202,7 +202,7 @@ __device__ __forceinline__ void block_exclusive_scan2(uint64_t& x, uint64_t& y, 
   x = ix - x; y = iy - y;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint64_t* a, const uint64_t* b,
+static __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint64_t* a, const uint64_t* b,
                                                               uint64_t n, uint64_t* sums) {
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
   uint64_t x = 0, y = 0;
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint64_t* a,
 }
 
 // Single block: exclusive scan over block sums; totals to sums[2*blocks..].
-__global__ __launch_bounds__(kScanThreads) void k_scan_sums(uint64_t* sums, uint64_t blocks) {
+static __global__ __launch_bounds__(kScanThreads) void k_scan_sums(uint64_t* sums, uint64_t blocks) {
   __shared__ uint64_t carry[2];
   if (threadIdx.x == 0) { carry[0] = 0; carry[1] = 0; }
   __syncthreads();
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_sums(uint64_t* sums, uint
   if (threadIdx.x == 0) { sums[2 * blocks] = carry[0]; sums[2 * blocks + 1] = carry[1]; }
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint64_t* a, uint64_t* b, uint64_t n,
+static __global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint64_t* a, uint64_t* b, uint64_t n,
                                                              const uint64_t* sums) {
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
   uint64_t va[kScanItems], vb[kScanItems];

@@ -75,23 +75,39 @@ __device__ __forceinline__ V3 node(const FieldView& f, const float* base, int64_
   return V3{p[0], p[1], p[2]};
 }
 
+// The eight corners of the cell a particle last sampled, kept in registers.
+// A particle moves a fraction of a cell per step, so most steps sample the
+// same cell again and need no memory access at all; only lanes that cross
+// into a new cell reload.  The values are the grid's, so results are
+// unchanged.  c0 < 0 marks an empty cache.
+struct CellCache {
+  int32_t c0, c1, c2;
+  V3 L0, L1, L2, L3, L4, L5, L6, L7;
+};
+
+__device__ __forceinline__ void cache_clear(CellCache& cc) { cc.c0 = -1; }
+
 // Trilinear sample of one time slice, corner order and lerp cascade of
 // regular_grid.hpp:59-74: list index 4a+2b+c, axis 2 first, then 1, then 0.
 __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
-                                        const Axis& a0, const Axis& a1, const Axis& a2) {
-  const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
-  const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
-  const V3 L0 = node(f, slice, b);
-  const V3 L1 = node(f, slice, b + 1);
-  const V3 L2 = node(f, slice, b + n2);
-  const V3 L3 = node(f, slice, b + n2 + 1);
-  const V3 L4 = node(f, slice, b + n12);
-  const V3 L5 = node(f, slice, b + n12 + 1);
-  const V3 L6 = node(f, slice, b + n12 + n2);
-  const V3 L7 = node(f, slice, b + n12 + n2 + 1);
+                                        const Axis& a0, const Axis& a1, const Axis& a2,
+                                        CellCache& cc) {
+  if (a0.c != cc.c0 || a1.c != cc.c1 || a2.c != cc.c2) {
+    const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
+    const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
+    cc.L0 = node(f, slice, b);
+    cc.L1 = node(f, slice, b + 1);
+    cc.L2 = node(f, slice, b + n2);
+    cc.L3 = node(f, slice, b + n2 + 1);
+    cc.L4 = node(f, slice, b + n12);
+    cc.L5 = node(f, slice, b + n12 + 1);
+    cc.L6 = node(f, slice, b + n12 + n2);
+    cc.L7 = node(f, slice, b + n12 + n2 + 1);
+    cc.c0 = a0.c; cc.c1 = a1.c; cc.c2 = a2.c;
+  }
   const float w2 = a2.w, m2 = 1.0f - w2;
-  const V3 A0 = lerp3(L0, L1, w2, m2), A1 = lerp3(L2, L3, w2, m2);
-  const V3 A2 = lerp3(L4, L5, w2, m2), A3 = lerp3(L6, L7, w2, m2);
+  const V3 A0 = lerp3(cc.L0, cc.L1, w2, m2), A1 = lerp3(cc.L2, cc.L3, w2, m2);
+  const V3 A2 = lerp3(cc.L4, cc.L5, w2, m2), A3 = lerp3(cc.L6, cc.L7, w2, m2);
   const float w1 = a1.w, m1 = 1.0f - w1;
   const V3 B0 = lerp3(A0, A1, w1, m1), B1 = lerp3(A2, A3, w1, m1);
   const float w0 = a0.w, m0 = 1.0f - w0;
@@ -126,7 +142,7 @@ __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, con
 // time-variant field) lies outside; v is the stored (unswizzled) vector.
 template <bool kUnsteady>
 __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, float p2, float t,
-                                       V3& v) {
+                                       V3& v, CellCache& cc) {
   Axis a0, a1, a2;
   if (!locate(p0, f.o0, f.h0, f.n0, a0)) return false;
   if (!locate(p1, f.o1, f.h1, f.n1, a1)) return false;
@@ -134,10 +150,11 @@ __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, f
   if constexpr (kUnsteady) {
     Axis at;
     if (!locate(t, f.to, f.th, f.nt, at)) return false;
+    (void)cc;
     v = quadlinear(f, a0, a1, a2, at);
   } else {
     (void)t;
-    v = trilinear(f, f.data, a0, a1, a2);
+    v = trilinear(f, f.data, a0, a1, a2, cc);
   }
   return true;
 }
@@ -185,27 +202,27 @@ __device__ __forceinline__ void step_dpa(float& p0, float& p1, float& p2, const 
 // the position unchanged.
 template <bool kUnsteady>
 __device__ __forceinline__ bool step_rk4(const FieldView& f, float& p0, float& p1, float& p2,
-                                         const V3& v, float t, const StepCoef& c) {
+                                         const V3& v, float t, const StepCoef& c, CellCache& cc) {
   const float k10 = v.c, k11 = v.b, k12 = v.a;
   float x0, x1, x2;
   { const float s = c.half_dt * k10; x0 = p0 + s; }
   { const float s = c.half_dt * k11; x1 = p1 + s; }
   { const float s = c.half_dt * k12; x2 = p2 + s; }
   V3 s2;
-  if (!sample<kUnsteady>(f, x0, x1, x2, t + c.half_dt, s2)) return false;
+  if (!sample<kUnsteady>(f, x0, x1, x2, t + c.half_dt, s2, cc)) return false;
   const float k20 = s2.c, k21 = s2.b, k22 = s2.a;
   { float y = p0; const float a = c.zero_dt * k10; y = y + a; const float b = c.half_dt * k20; x0 = y + b; }
   { float y = p1; const float a = c.zero_dt * k11; y = y + a; const float b = c.half_dt * k21; x1 = y + b; }
   { float y = p2; const float a = c.zero_dt * k12; y = y + a; const float b = c.half_dt * k22; x2 = y + b; }
   V3 s3;
-  if (!sample<kUnsteady>(f, x0, x1, x2, t + c.half_dt, s3)) return false;
+  if (!sample<kUnsteady>(f, x0, x1, x2, t + c.half_dt, s3, cc)) return false;
   const float k30 = s3.c, k31 = s3.b, k32 = s3.a;
   const float one_dt = 1.0f * c.dt;
   { float y = p0; const float a = c.zero_dt * k10; y = y + a; const float b = c.zero_dt * k20; y = y + b; const float d = one_dt * k30; x0 = y + d; }
   { float y = p1; const float a = c.zero_dt * k11; y = y + a; const float b = c.zero_dt * k21; y = y + b; const float d = one_dt * k31; x1 = y + d; }
   { float y = p2; const float a = c.zero_dt * k12; y = y + a; const float b = c.zero_dt * k22; y = y + b; const float d = one_dt * k32; x2 = y + d; }
   V3 s4;
-  if (!sample<kUnsteady>(f, x0, x1, x2, t + one_dt, s4)) return false;
+  if (!sample<kUnsteady>(f, x0, x1, x2, t + one_dt, s4, cc)) return false;
   const float k40 = s4.c, k41 = s4.b, k42 = s4.a;
   auto fin = [&](float x, float q1, float q2, float q3, float q4) {
     const float a = c.a16 * q1; x = x + a;

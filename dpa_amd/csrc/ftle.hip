@@ -1,6 +1,7 @@
 // ftle.hip -- FTLE entry points: dpa's lattice estimator and the
 // auxiliary-neighbour form (ftle_kernels.hpp).
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <vector>
 
@@ -162,62 +163,87 @@ DPAH_EXPORT int dpah_ftle_lattice(dpah_context* ctx, const dpah_field* field, co
   });
 }
 
+void dpah::ftle_aux_into(dpah_context* ctx, const dpah_field* field, const float origin[3],
+                         const float lattice_spacing[3], const uint64_t lattice_shape[3],
+                         const float delta[3], uint64_t iterations, float step_size,
+                         float start_time, int integrator, float* ftle_out, bool persistent,
+                         uint64_t* total_steps) {
+  DPAH_REQUIRE(ctx && field && origin && lattice_spacing && lattice_shape && delta,
+               "null argument");
+  DPAH_REQUIRE(field->ctx == ctx, "field belongs to another context");
+  DPAH_REQUIRE(integrator == DPAH_INTEGRATOR_DPA_RK4 || integrator == DPAH_INTEGRATOR_RK4,
+               "unknown integrator");
+  DPAH_REQUIRE(iterations < (1ull << 32) - 2, "iterations must be below 2^32 - 2");
+  const uint64_t points = lattice_shape[0] * lattice_shape[1] * lattice_shape[2];
+  DPAH_REQUIRE(points == 0 || ftle_out, "ftle_out is null");
+  SeedLattice lat{};
+  lat.lo0 = origin[0]; lat.lo1 = origin[1]; lat.lo2 = origin[2];
+  lat.st0 = lattice_spacing[0]; lat.st1 = lattice_spacing[1]; lat.st2 = lattice_spacing[2];
+  lat.d0 = lattice_shape[0]; lat.d1 = lattice_shape[1]; lat.d2 = lattice_shape[2];
+  lat.dl0 = delta[0]; lat.dl1 = delta[1]; lat.dl2 = delta[2];
+  // points are processed in chunks of at most 2^25 (2.4 GB of flow map)
+  const uint64_t chunk = std::min<uint64_t>(points, 1ull << 25);
+  DeviceBuffer finals(ctx, std::max<uint64_t>(chunk, 1) * 18 * sizeof(float));
+  uint64_t steps = 0;
+  for (uint64_t p0 = 0; p0 < points; p0 += chunk) {
+    const uint64_t pn = std::min(chunk, points - p0);
+    AdvectLaunch L{};
+    L.f = &field->view;
+    L.seed_kind = kSeedAux;
+    L.lat = lat;
+    L.lat.first = p0;
+    L.uniform_rem = iterations;
+    L.pos_out = finals.as<float>();
+    L.n = 6 * pn;
+    L.dt = step_size;
+    L.t0 = start_time;
+    L.integrator = integrator;
+    L.persistent = persistent;
+    launch_advect(ctx, L);
+    FtleAuxArgs a{};
+    a.finals = finals.as<float>();
+    a.points = pn;
+    a.twod0 = 2.0f * delta[0]; a.twod1 = 2.0f * delta[1]; a.twod2 = 2.0f * delta[2];
+    a.T_abs = ftle_time(step_size, iterations);
+    a.out = ftle_out + p0;
+    ctx->timed("ftle_aux_reduce", [&] {
+      hipLaunchKernelGGL(k_ftle_aux_reduce, dim3(grid_for(pn)), dim3(256), 0, ctx->stream, a);
+    });
+    DPAH_HIP(hipGetLastError());
+    unsigned long long s = 0;
+    DPAH_HIP(hipMemcpyAsync(&s, ctx->d_counters + 1, sizeof(s), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    ctx->sync();
+    steps += s;
+  }
+  if (total_steps) *total_steps = steps;
+}
+
+dpah_result* dpah::run_ftle_aux(dpah_context* ctx, const dpah_field* field, const float origin[3],
+                                const float lattice_spacing[3], const uint64_t lattice_shape[3],
+                                const float delta[3], uint64_t iterations, float step_size,
+                                float start_time, int integrator, bool persistent) {
+  auto res = std::make_unique<dpah_result>();
+  res->ctx = ctx;
+  res->ftle_count = lattice_shape[0] * lattice_shape[1] * lattice_shape[2];
+  res->d_ftle = static_cast<float*>(ctx->alloc(std::max<uint64_t>(res->ftle_count, 1) * 4));
+  uint64_t steps = 0;
+  ftle_aux_into(ctx, field, origin, lattice_spacing, lattice_shape, delta, iterations, step_size,
+                start_time, integrator, res->d_ftle, persistent, &steps);
+  res->total_steps = steps;
+  return res.release();
+}
+
 DPAH_EXPORT int dpah_ftle_aux_device(dpah_context* ctx, const dpah_field* field,
                                      const float origin[3], const float lattice_spacing[3],
                                      const uint64_t lattice_shape[3], const float delta[3],
                                      uint64_t iterations, float step_size, float start_time,
                                      int integrator, float* ftle_out, uint64_t* total_steps) {
   return guarded([&] {
-    DPAH_REQUIRE(ctx && field && origin && lattice_spacing && lattice_shape && delta,
-                 "null argument");
-    DPAH_REQUIRE(field->ctx == ctx, "field belongs to another context");
-    DPAH_REQUIRE(integrator == DPAH_INTEGRATOR_DPA_RK4 || integrator == DPAH_INTEGRATOR_RK4,
-                 "unknown integrator");
-    DPAH_REQUIRE(iterations < (1ull << 32) - 2, "iterations must be below 2^32 - 2");
+    DPAH_REQUIRE(ctx, "null argument");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     DPAH_HIP(hipSetDevice(ctx->device));
-    const uint64_t points = lattice_shape[0] * lattice_shape[1] * lattice_shape[2];
-    DPAH_REQUIRE(points == 0 || ftle_out, "ftle_out is null");
-    SeedLattice lat{};
-    lat.lo0 = origin[0]; lat.lo1 = origin[1]; lat.lo2 = origin[2];
-    lat.st0 = lattice_spacing[0]; lat.st1 = lattice_spacing[1]; lat.st2 = lattice_spacing[2];
-    lat.d0 = lattice_shape[0]; lat.d1 = lattice_shape[1]; lat.d2 = lattice_shape[2];
-    lat.dl0 = delta[0]; lat.dl1 = delta[1]; lat.dl2 = delta[2];
-    // points are processed in chunks of at most 2^25 (2.4 GB of flow map)
-    const uint64_t chunk = std::min<uint64_t>(points, 1ull << 25);
-    DeviceBuffer finals(ctx, std::max<uint64_t>(chunk, 1) * 18 * sizeof(float));
-    uint64_t steps = 0;
-    for (uint64_t p0 = 0; p0 < points; p0 += chunk) {
-      const uint64_t pn = std::min(chunk, points - p0);
-      AdvectLaunch L{};
-      L.f = &field->view;
-      L.seed_kind = kSeedAux;
-      L.lat = lat;
-      L.lat.first = p0;
-      L.uniform_rem = iterations;
-      L.pos_out = finals.as<float>();
-      L.n = 6 * pn;
-      L.dt = step_size;
-      L.t0 = start_time;
-      L.integrator = integrator;
-      L.persistent = true;
-      launch_advect(ctx, L);
-      FtleAuxArgs a{};
-      a.finals = finals.as<float>();
-      a.points = pn;
-      a.twod0 = 2.0f * delta[0]; a.twod1 = 2.0f * delta[1]; a.twod2 = 2.0f * delta[2];
-      a.T_abs = ftle_time(step_size, iterations);
-      a.out = ftle_out + p0;
-      ctx->timed("ftle_aux_reduce", [&] {
-        hipLaunchKernelGGL(k_ftle_aux_reduce, dim3(grid_for(pn)), dim3(256), 0, ctx->stream, a);
-      });
-      DPAH_HIP(hipGetLastError());
-      unsigned long long s = 0;
-      DPAH_HIP(hipMemcpyAsync(&s, ctx->d_counters + 1, sizeof(s), hipMemcpyDeviceToHost,
-                              ctx->stream));
-      ctx->sync();
-      steps += s;
-    }
-    if (total_steps) *total_steps = steps;
+    ftle_aux_into(ctx, field, origin, lattice_spacing, lattice_shape, delta, iterations, step_size,
+                  start_time, integrator, ftle_out, true, total_steps);
   });
 }

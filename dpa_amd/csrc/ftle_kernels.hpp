@@ -64,7 +64,7 @@ struct FtleAuxArgs {
   float* out;            // [points]
 };
 
-__global__ __launch_bounds__(256) void k_ftle_aux_reduce(FtleAuxArgs a) {
+static __global__ __launch_bounds__(256) void k_ftle_aux_reduce(FtleAuxArgs a) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.points) return;
   const float* f = a.finals + 18 * p;
@@ -115,7 +115,7 @@ __device__ __forceinline__ bool flow_cell(const FtleScatterArgs& a, uint64_t pid
   return true;
 }
 
-__global__ __launch_bounds__(256) void k_ftle_clear(FtleScatterArgs a) {
+static __global__ __launch_bounds__(256) void k_ftle_clear(FtleScatterArgs a) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t m = a.m0 * a.m1 * a.m2;
   if (i < m) {
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_ftle_clear(FtleScatterArgs a) {
 
 // Two passes make the scatter deterministic: on a collision the highest
 // particle index wins (dpa's TBB scatter leaves that order unspecified).
-__global__ __launch_bounds__(256) void k_ftle_claim(FtleScatterArgs a) {
+static __global__ __launch_bounds__(256) void k_ftle_claim(FtleScatterArgs a) {
   const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (pid >= a.count) return;
   uint64_t lin;
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_ftle_claim(FtleScatterArgs a) {
   else atomicAdd(a.misses, 1ull);
 }
 
-__global__ __launch_bounds__(256) void k_ftle_scatter(FtleScatterArgs a) {
+static __global__ __launch_bounds__(256) void k_ftle_scatter(FtleScatterArgs a) {
   const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (pid >= a.count) return;
   uint64_t lin;
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_ftle_scatter(FtleScatterArgs a) {
 
 // regular_grid::gradient (regular_grid.hpp:95-105): clamped neighbours, the
 // divisor stays fl(2 * s) at the borders; then C, lmax and the FTLE.
-__global__ __launch_bounds__(256) void k_ftle_gradient(FtleScatterArgs a) {
+static __global__ __launch_bounds__(256) void k_ftle_gradient(FtleScatterArgs a) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t m = a.m0 * a.m1 * a.m2;
   if (i >= m) return;

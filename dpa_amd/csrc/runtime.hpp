@@ -145,6 +145,8 @@ struct dpah_result {
   std::vector<uint64_t> final_rem;
   std::vector<uint8_t> final_status;
   uint64_t total_steps = 0;
+  float* d_ftle = nullptr;        // FTLE requests: device volume
+  uint64_t ftle_count = 0;
   ~dpah_result();
 };
 
@@ -197,6 +199,23 @@ struct AdvectLaunch {
   bool persistent;
 };
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
+
+// One rank's advection stage (dpah_advect); persistent=false uses one thread
+// per particle (the per-request form).
+dpah_result* run_advect(dpah_context* ctx, const dpah_field* field, const float* seeds,
+                        const uint64_t* remaining, uint64_t uniform_remaining, uint64_t count,
+                        const dpah_advect_params* params, bool persistent);
+
+// Auxiliary-neighbour FTLE into a device volume; run_ftle_aux returns it in a
+// result (the serving engine's per-request form).
+void ftle_aux_into(dpah_context* ctx, const dpah_field* field, const float origin[3],
+                   const float lattice_spacing[3], const uint64_t lattice_shape[3],
+                   const float delta[3], uint64_t iterations, float step_size, float start_time,
+                   int integrator, float* ftle_out, bool persistent, uint64_t* total_steps);
+dpah_result* run_ftle_aux(dpah_context* ctx, const dpah_field* field, const float origin[3],
+                          const float lattice_spacing[3], const uint64_t lattice_shape[3],
+                          const float delta[3], uint64_t iterations, float step_size,
+                          float start_time, int integrator, bool persistent);
 
 // Prunes, indexes and colours `count` slot rows; allocates the chunk's outputs.
 Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const uint32_t* steps,

@@ -201,6 +201,86 @@ int dpah_ftle_aux_device(dpah_context* ctx, const dpah_field* field, const float
                          float start_time, int integrator, float* ftle_out,
                          uint64_t* total_steps);
 
+/* ------------------------------------------------------------- serving ----
+ * No counterpart in dpa (a batch pipeline): a request loop on one GPU that
+ * keeps fields resident, admits advection and FTLE queries by memory and
+ * batches them continuously.  One engine thread owns the device.  In
+ * DPAH_SERVE_BATCHED mode every tick advances the particles of all admitted
+ * requests by up to steps_per_tick steps in one launch per (field, integrator,
+ * record) group; requests join and leave between ticks.  DPAH_SERVE_PER_REQUEST
+ * runs each request alone in arrival order (allocate, upload, one
+ * non-persistent launch, compact, release): the comparator.
+ * A request's result is the same alone and in any mix, bit for bit. */
+#define DPAH_REQUEST_ADVECT   0   /* dpah_advect arguments                    */
+#define DPAH_REQUEST_FTLE_AUX 1   /* dpah_ftle_aux_device arguments           */
+
+#define DPAH_SERVE_BATCHED     0
+#define DPAH_SERVE_PER_REQUEST 1
+
+#define DPAH_TICKET_PENDING   0
+#define DPAH_TICKET_RUNNING   1
+#define DPAH_TICKET_DONE      2
+#define DPAH_TICKET_FAILED    3
+#define DPAH_TICKET_CANCELLED 4
+
+typedef struct dpah_server dpah_server;
+
+typedef struct dpah_server_options {
+  uint64_t memory_budget_bytes;  /* device bytes for admitted requests; 0: half the context's */
+  uint32_t max_active;           /* requests admitted at once; 0: 256                        */
+  uint32_t steps_per_tick;       /* per particle per tick; 0: 256                            */
+  int      mode;                 /* DPAH_SERVE_*                                              */
+} dpah_server_options;
+
+typedef struct dpah_request {
+  int                kind;               /* DPAH_REQUEST_*                              */
+  const dpah_field*  field;
+  /* DPAH_REQUEST_ADVECT (host arrays, copied at submit) */
+  const float*       seeds;
+  const uint64_t*    remaining;
+  uint64_t           uniform_remaining;
+  uint64_t           count;
+  dpah_advect_params params;             /* FTLE uses step_size, start_time, integrator */
+  /* DPAH_REQUEST_FTLE_AUX */
+  float              origin[3];
+  float              lattice_spacing[3];
+  uint64_t           lattice_shape[3];
+  float              delta[3];
+  uint64_t           iterations;
+} dpah_request;
+
+typedef struct dpah_request_info {
+  int      state;                /* DPAH_TICKET_*                         */
+  double   queue_seconds;        /* submit -> admitted                    */
+  double   latency_seconds;      /* submit -> result ready (on device)    */
+  uint64_t particle_steps;
+  uint64_t footprint_bytes;
+} dpah_request_info;
+
+typedef struct dpah_server_stats {
+  uint64_t submitted, completed, rejected, cancelled, failed, pending, ticks;
+  uint64_t particle_steps;
+  uint64_t bytes_in_use;
+  double   busy_seconds;
+} dpah_server_stats;
+
+int dpah_server_create(dpah_context* ctx, const dpah_server_options* options, dpah_server** out);
+/* Queues a request; DPAH_ERR_ADMISSION if it can never fit the budget. */
+int dpah_server_submit(dpah_server* server, const dpah_request* request, uint64_t* ticket);
+int dpah_server_poll(dpah_server* server, uint64_t ticket, int* state);
+/* Waits (timeout_ms < 0: forever) and hands over the result (free it with
+ * dpah_result_free); DPAH_ERR_TIMEOUT / DPAH_ERR_CANCELLED / the failure code. */
+int dpah_server_wait(dpah_server* server, uint64_t ticket, int64_t timeout_ms, dpah_result** out);
+int dpah_server_request_info(dpah_server* server, uint64_t ticket, dpah_request_info* info);
+int dpah_server_cancel(dpah_server* server, uint64_t ticket);
+/* Drops a finished ticket's bookkeeping (and its result if not taken). */
+int dpah_server_forget(dpah_server* server, uint64_t ticket);
+int dpah_server_get_stats(dpah_server* server, dpah_server_stats* stats);
+/* Cancels what is still queued or running, stops the engine, frees. */
+int dpah_server_destroy(dpah_server* server);
+/* FTLE volume of a DPAH_REQUEST_FTLE_AUX result (out may be NULL: count only). */
+int dpah_result_ftle(const dpah_result* r, float* out, uint64_t* count);
+
 /* Kernel timing recorded with HIP events on the context's stream (enabled by
  * dpah_options.kernel_timing): launches and summed milliseconds of the named
  * kernel ("advect_persistent", "advect_simple", "compact_write", ...). */
