@@ -1,13 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark of the particle-advection hot path on one MI355X.
 
-Workload (one "step" of this bench = one pass of the hot path over one batch):
-steady endpoint advection on a 256^3 float32 Gaussian-vortex field (64
+Default workload (one "step" of this bench = one pass of the hot path over one
+batch): steady endpoint advection on a 256^3 float32 Gaussian-vortex field (64
 vortex tubes + uniform drift 0.1, spacing 1), 2^20 seeds uniform in
 [0, 255)^3, dpa's runge_kutta_4 with dt = 0.25 for 1000 iterations, endpoints
 only, particles leaving the grid retire.  Field and seeds are resident in HBM
 before the timed region; each pass reads the seeds and writes final positions,
-remaining iterations and status.
+remaining iterations and status.  A short served-query leg (continuous
+batching vs one launch per request on the same Poisson trace) adds p50/p99
+latency.
+
+Other configurations: --workload cfg1 | cfg3 | cfg3-dpa | cfg4 | cfg5.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -18,6 +22,7 @@ import json
 import os
 import platform
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -55,43 +60,111 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(field, seeds, iters, dt, target_s: float):
-    """Oracle (CPU port of dpa's loop) on a bounded sample of the same
-    workload, on the host's cores with OpenMP schedule(dynamic,1)."""
+def host_threads() -> int:
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline_endpoints(field, offset, spacing, seeds, iters, dt, target_s: float,
+                           what: str):
+    """Oracle (CPU port of dpa's loop) on a bounded sample of the same workload,
+    OpenMP schedule(dynamic,1) over the host's cores."""
     from oracle import py_oracle as po
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    fld = po.Field(field, (0, 0, 0), (1, 1, 1))
+    threads = host_threads()
+    fld = po.Field(field, offset, spacing)
     probe = seeds[: 2048]
     t = time.perf_counter()
     r = po.advect_endpoints(fld, probe, iters, dt, threads=threads)
     el = time.perf_counter() - t
+    per_seed = max(r["total_steps"] / len(probe), 1e-9)
     rate = r["total_steps"] / max(el, 1e-9)
-    count = int(min(len(seeds), max(2048, rate * target_s / (r["total_steps"] / len(probe)))))
+    count = int(min(len(seeds), max(2048, rate * target_s / per_seed)))
     sample = seeds[: count]
     t = time.perf_counter()
     r = po.advect_endpoints(fld, sample, iters, dt, threads=threads)
     el = time.perf_counter() - t
     return {"value": r["total_steps"] / el, "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"first {count} of the 2^20 seeds, full 1000 iterations, same field "
+            "sample": f"first {count} of the {what} seeds, all {iters} iterations, same field "
                       f"({r['total_steps']} particle-steps in {el:.2f} s; host {cpu_model()})"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--seeds", type=int, default=1 << 20)
-    ap.add_argument("--iterations", type=int, default=1000)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(seeds shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
-        return 2
+def roofline(steps_per_launch, avg_ms, bytes_per_step=BYTES_PER_STEP, traffic=None):
+    achieved = steps_per_launch * bytes_per_step / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_avg_ms": avg_ms,
+            "bytes_per_step": bytes_per_step}
 
+
+# ----------------------------------------------------------------- serving
+
+def serve_trace(ctx, fields, trace, mode, steps_per_tick=256):
+    """Replays a trace against a server; returns latency / throughput figures."""
+    import dpa_amd
+    lat, steps = [], 0
+    with dpa_amd.Server(ctx, mode=mode, steps_per_tick=steps_per_tick) as server:
+        tickets = []
+        done = threading.Event()
+        results = {}
+
+        def collector():
+            i = 0
+            while i < len(trace):
+                while i >= len(tickets):
+                    time.sleep(0.0005)
+                t = tickets[i]
+                server.wait(t, -1).free()
+                results[i] = server.info(t)
+                server.forget(t)
+                i += 1
+            done.set()
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t0 = time.perf_counter()
+        for r in trace:
+            delay = r["at"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            f = fields[r["field"]]
+            if r["kind"] == "ftle":
+                tickets.append(server.submit_ftle(f, r["origin"], r["spacing"], r["shape"],
+                                                  r["delta"], r["iterations"], r["step_size"],
+                                                  start_time=r["start_time"]))
+            else:
+                tickets.append(server.submit_advect(f, r["seeds"],
+                                                    uniform_remaining=r["uniform_remaining"],
+                                                    step_size=r["step_size"],
+                                                    start_time=r["start_time"],
+                                                    record=r["record"]))
+        done.wait()
+        th.join()
+        t_end = time.perf_counter()
+    for i in range(len(trace)):
+        lat.append(results[i]["latency_seconds"])
+        steps += results[i]["particle_steps"]
+    lat = np.array(lat)
+    span = t_end - t0
+    return {"requests": len(trace), "particle_steps": int(steps),
+            "throughput": steps / span, "makespan_s": span,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3)}
+
+
+def serving_leg(ctx, fields, trace, steps_per_tick=256):
+    import dpa_amd
+    b = serve_trace(ctx, fields, trace, dpa_amd.SERVE_BATCHED, steps_per_tick)
+    log(f"served batched: {b['throughput']:.3e} steps/s p50 {b['p50_ms']:.1f} ms "
+        f"p99 {b['p99_ms']:.1f} ms")
+    p = serve_trace(ctx, fields, trace, dpa_amd.SERVE_PER_REQUEST)
+    log(f"per-request:    {p['throughput']:.3e} steps/s p50 {p['p50_ms']:.1f} ms "
+        f"p99 {p['p99_ms']:.1f} ms")
+    return {"batched": b, "per_request_launch": p,
+            "throughput_ratio": b["throughput"] / max(p["throughput"], 1e-9)}
+
+
+# --------------------------------------------------------------- workloads
+
+def run_cfg2(args):
     import dpa_amd
     from dpa_amd import workloads as W
 
@@ -110,9 +183,9 @@ def main() -> int:
     d_st = ctx.alloc(n)
 
     def one_pass(wait: bool):
-        return ctx.advect_endpoints_device(fld, d_seeds, n, d_pos, uniform_remaining=args.iterations,
-                                           step_size=dt, d_rem_out=d_rem, d_status_out=d_st,
-                                           wait=wait)
+        return ctx.advect_endpoints_device(fld, d_seeds, n, d_pos,
+                                           uniform_remaining=args.iterations, step_size=dt,
+                                           d_rem_out=d_rem, d_status_out=d_st, wait=wait)
 
     steps_per_pass = None
     for i in range(max(args.warmup, 1)):
@@ -125,56 +198,265 @@ def main() -> int:
         one_pass(False)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
+    launches, total_ms = ctx.kernel_timing("advect_persistent")
     last = one_pass(True)
     if last != steps_per_pass:
         raise RuntimeError(f"step count changed between passes: {last} != {steps_per_pass}")
-    launches, total_ms = ctx.kernel_timing("advect_persistent")
-    # the check pass above is the last timed launch: average over all of them
     avg_ms = total_ms / max(launches, 1)
     value = args.steps * steps_per_pass / elapsed
     log(f"timed {args.steps} passes in {elapsed:.3f} s: {value:.4e} particle-steps/s; "
         f"kernel avg {avg_ms:.3f} ms over {launches} launches")
-    achieved = steps_per_pass * BYTES_PER_STEP / (avg_ms * 1e-3) / 1e9
-    traffic = read_traffic()
-
     for p in (d_seeds, d_pos, d_rem, d_st):
         ctx.free(p)
+
+    serving = None
+    if not args.no_serve:
+        # short served-query leg on a time-variant version of the same field
+        fu = ctx.field_unsteady(W.moving_field(field, 16), (0, 0, 0), (1, 1, 1), 0.0, 1.0)
+        trace = W.serving_trace(args.serve_rate, args.serve_requests, fields=1,
+                                ftle_shape=(16, 64, 64))
+        serving = serving_leg(ctx, [fu], trace)
+        serving["trace"] = (f"{args.serve_requests} Poisson arrivals at {args.serve_rate} req/s on "
+                            "a 256^3 x 16-slice time-variant field: 80% recorded pathlines "
+                            "(1k-262k seeds, 128-2048 steps, dt 0.005), 20% FTLE slabs "
+                            "(16x64x64 lattice, +-0.5 cell, 200 steps, dt 0.02)")
+        fu.release()
     fld.release()
     ctx.close()
 
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle on a bounded sample)")
-        cpu = cpu_baseline(field, seeds, args.iterations, dt, args.cpu_seconds)
+        cpu = cpu_baseline_endpoints(field, (0, 0, 0), (1, 1, 1), seeds, args.iterations, dt,
+                                     args.cpu_seconds, "2^20")
         log(f"cpu: {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads")
 
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded Gaussian-vortex field and uniform seeds; no dataset)",
-        "config": {
-            "workload": "large steady endpoint advection: 256^3 f32 Gaussian-vortex field "
-                        "(64 tubes + drift 0.1), 2^20 uniform seeds, dpa runge_kutta_4 "
-                        "dt 0.25 x 1000 iterations, endpoints only, exits retire",
-            "seeds": n, "iterations": args.iterations, "particle_steps_per_pass": steps_per_pass,
-        },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_advect_persistent", "kernel_avg_ms": avg_ms,
-            "bytes_per_step": BYTES_PER_STEP,
-        },
-        "cpu_baseline": cpu,
-    }
+    rf = roofline(steps_per_pass, avg_ms, traffic=read_traffic())
+    rf["kernel"] = "k_advect_persistent"
+    out = base_line(args, value, elapsed, "f32")
+    out["config"] = {
+        "workload": "large steady endpoint advection: 256^3 f32 Gaussian-vortex field "
+                    "(64 tubes + drift 0.1), 2^20 uniform seeds, dpa runge_kutta_4 "
+                    "dt 0.25 x 1000 iterations, endpoints only, exits retire",
+        "seeds": n, "iterations": args.iterations, "particle_steps_per_pass": steps_per_pass}
+    out["roofline"] = rf
+    out["cpu_baseline"] = cpu
+    out["serving"] = serving
+    return out
+
+
+def base_line(args, value, elapsed, dtype):
+    return {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (seeded fields and seeds; no dataset)"}
+
+
+def run_cfg1(args):
+    """ABC 64^3, 4096 seeds x 256 steps, record + prune + indices + colours."""
+    import dpa_amd
+    from dpa_amd import workloads as W
+    c = W.cfg1()
+    ctx = dpa_amd.Context(0, kernel_timing=True)
+    fld = ctx.field(c["field"], c["offset"], c["spacing"])
+
+    def one():
+        r = ctx.advect(fld, c["seeds"], uniform_remaining=c["iterations"],
+                       particles_per_round=c["particles_per_round"], step_size=c["step_size"],
+                       record=True)
+        s = r.total_steps
+        r.free()
+        return s
+
+    steps = None
+    for _ in range(max(args.warmup, 1)):
+        steps = one()
+    ctx.kernel_timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one()
+    elapsed = time.perf_counter() - t0
+    na, ta = ctx.kernel_timing("advect_persistent")
+    nc, tc = ctx.kernel_timing("compact_write")
+    fld.release()
+    ctx.close()
+    value = args.steps * steps / elapsed
+    adv_ms = ta / max(na, 1)
+    out = base_line(args, value, elapsed, "f32")
+    out["config"] = {"workload": "cfg1 ABC 64^3, 4096 seeds x 256 steps dt 0.05 h, record, u32; "
+                                 "value includes host upload / download and allocation",
+                     "particle_steps_per_pass": steps}
+    out["kernels"] = {"advect_persistent_ms": adv_ms, "compact_write_ms": tc / max(nc, 1),
+                      "device_steps_per_s": steps / (adv_ms * 1e-3)}
+    out["roofline"] = roofline(steps, adv_ms, bytes_per_step=108)
+    if not args.no_cpu:
+        from oracle import py_oracle as po
+        th = host_threads()
+        t = time.perf_counter()
+        r = po.run_rounds(po.Field(c["field"], c["offset"], c["spacing"]), c["seeds"],
+                          c["iterations"], 4096, c["step_size"], record=True, threads=th)
+        el = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": r["total_steps"] / el, "unit": "particle-steps/s",
+                               "cores": th, "kind": "port",
+                               "sample": "whole workload incl. prune / indices / colours"}
+    return out
+
+
+def run_cfg3(args, dpa_mode=False):
+    """FTLE on ABC 512x512x128: auxiliary-neighbour form (default) or dpa's
+    lattice estimator."""
+    import dpa_amd
+    from dpa_amd import workloads as W
+    c = W.cfg3()
+    ctx = dpa_amd.Context(0, kernel_timing=True)
+    fld = ctx.field(c["field"], c["offset"], c["spacing"])
+    shape = c["lattice_shape"]
+    points = int(np.prod(shape))
+    d_out = ctx.alloc(4 * points)
+
+    def one():
+        if dpa_mode:
+            v, s = ctx.ftle_lattice(fld, (1.0, 1.0, 1.0), c["iterations"], c["step_size"])
+            return s
+        return ctx.ftle_aux_device(fld, c["lattice_origin"], c["lattice_spacing"], shape,
+                                   c["delta"], c["iterations"], c["step_size"], d_out)
+
+    steps = None
+    for _ in range(max(args.warmup, 1)):
+        steps = one()
+    ctx.kernel_timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one()
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    na, ta = ctx.kernel_timing("advect_persistent")
+    ctx.free(d_out)
+    fld.release()
+    ctx.close()
+    adv_ms = ta / max(na, 1)
+    out = base_line(args, args.steps * steps / elapsed, elapsed, "f32")
+    out["config"] = {"workload": ("cfg3 FTLE dpa lattice mode" if dpa_mode else
+                                  "cfg3 FTLE auxiliary +-0.5 cell neighbours") +
+                                 ", ABC 512x512x128, T=4 at dt 0.02",
+                     "lattice_points": points, "particle_steps_per_pass": steps,
+                     "points_per_s": args.steps * points / elapsed}
+    out["roofline"] = roofline(steps, adv_ms)
+    out["roofline"]["kernel"] = "k_advect_persistent"
+    return out
+
+
+def run_cfg4(args):
+    """64 uneven recorded jobs on the vortex field: one persistent launch
+    (dpah_advect_batch) against the per-request form on the same jobs."""
+    import dpa_amd
+    from dpa_amd import workloads as W
+    field = W.vortex_field(256)
+    jobs = W.cfg4_jobs()
+    ctx = dpa_amd.Context(0, kernel_timing=True)
+    fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
+    for j in jobs:
+        j["field"] = fld
+
+    def batched():
+        res = ctx.advect_batch(jobs)
+        s = sum(r.total_steps for r in res)
+        for r in res:
+            r.free()
+        return s
+
+    steps = batched()                    # warmup
+    ctx.kernel_timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batched()
+    elapsed = time.perf_counter() - t0
+    na, ta = ctx.kernel_timing("advect_persistent")
+    nc, tc = ctx.kernel_timing("compact_write")
+    log(f"batched: {args.steps * steps / elapsed:.3e} steps/s")
+    # per-request form: each job alone, one thread per particle
+    t0 = time.perf_counter()
+    with dpa_amd.Server(ctx, mode=dpa_amd.SERVE_PER_REQUEST) as server:
+        ts = [server.submit_advect(fld, j["seeds"], uniform_remaining=j["uniform_remaining"],
+                                   step_size=j["step_size"], record=True) for j in jobs]
+        for t in ts:
+            server.wait(t, -1).free()
+    per_req = time.perf_counter() - t0
+    log(f"per-request: {steps / per_req:.3e} steps/s")
+    fld.release()
+    ctx.close()
+    value = args.steps * steps / elapsed
+    out = base_line(args, value, elapsed, "f32")
+    out["config"] = {"workload": "cfg4 64 recorded jobs (256..262144 seeds, 64..4096 steps) on "
+                                 "the 256^3 vortex field, one persistent launch + compaction; "
+                                 "host upload / download included",
+                     "jobs": len(jobs), "seeds": int(sum(len(j['seeds']) for j in jobs)),
+                     "particle_steps_per_pass": steps}
+    out["kernels"] = {"advect_persistent_ms": ta / max(na, 1),
+                      "compact_write_ms_total_per_pass": tc / max(args.steps, 1)}
+    out["per_request_launch"] = {"value": steps / per_req, "seconds": per_req}
+    out["vs_per_request"] = value / (steps / per_req)
+    out["roofline"] = roofline(steps, ta / max(na, 1), bytes_per_step=108)
+    return out
+
+
+def run_cfg5(args):
+    """4 resident time-variant fields (16 slices of 256^3), Poisson traces at
+    several rates, continuous batching vs per-request launch."""
+    import dpa_amd
+    from dpa_amd import workloads as W
+    base = W.vortex_field(256)
+    ctx = dpa_amd.Context(0, kernel_timing=True)
+    fields = []
+    for k in range(4):
+        shifted = np.roll(base, shift=(37 * k, 71 * k, 13 * k), axis=(0, 1, 2))
+        fields.append(ctx.field_unsteady(W.moving_field(shifted, 16, phase=k), (0, 0, 0),
+                                         (1, 1, 1), 0.0, 1.0))
+        log(f"field {k} resident")
+    rates = [float(r) for r in args.rates.split(",")]
+    sweep = []
+    for rate in rates:
+        trace = W.serving_trace(rate, args.serve_requests)
+        leg = serving_leg(ctx, fields, trace)
+        leg["rate"] = rate
+        sweep.append(leg)
+    for f in fields:
+        f.release()
+    ctx.close()
+    best = max(s["batched"]["throughput"] for s in sweep)
+    out = base_line(args, best, 0.0, "f32")
+    out["steps"] = 0
+    out["config"] = {"workload": "cfg5 served pathlines + FTLE slabs on 4 resident 256^3 x 16 "
+                                 "time-variant fields (12.9 GB)", "requests_per_rate": args.serve_requests}
+    out["sweep"] = sweep
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg2",
+                    choices=["cfg1", "cfg2", "cfg3", "cfg3-dpa", "cfg4", "cfg5"])
+    ap.add_argument("--seeds", type=int, default=1 << 20)
+    ap.add_argument("--iterations", type=int, default=1000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--rates", default="50,100,200,500")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(seeds shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        return 2
+    run = {"cfg1": run_cfg1, "cfg2": run_cfg2, "cfg3": run_cfg3,
+           "cfg3-dpa": lambda a: run_cfg3(a, dpa_mode=True), "cfg4": run_cfg4,
+           "cfg5": run_cfg5}[args.workload]
+    out = run(args)
     print(json.dumps(out), flush=True)
     return 0
 

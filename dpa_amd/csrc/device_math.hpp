@@ -81,8 +81,9 @@ __device__ __forceinline__ V3 node(const FieldView& f, const float* base, int64_
 // into a new cell reload.  The values are the grid's, so results are
 // unchanged.  c0 < 0 marks an empty cache.
 struct CellCache {
-  int32_t c0, c1, c2;
-  V3 L0, L1, L2, L3, L4, L5, L6, L7;
+  int32_t c0, c1, c2, ct;
+  V3 L0, L1, L2, L3, L4, L5, L6, L7;   // corners of slice ct (the only slice if steady)
+  V3 M0, M1, M2, M3, M4, M5, M6, M7;   // corners of slice ct + 1 (time-variant fields)
 };
 
 __device__ __forceinline__ void cache_clear(CellCache& cc) { cc.c0 = -1; }
@@ -116,22 +117,34 @@ __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
 
 // Four-dimensional sample of a time-variant field (regular_grid<vector3,4>,
 // regular_fields.hpp:27): the 16 corners in permute_for order have time as the
-// innermost axis, so the cascade lerps time first, then axes 2, 1, 0.
+// innermost axis, so the cascade lerps time first, then axes 2, 1, 0.  The
+// cache holds the corners of both bracketing slices.
 __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, const Axis& a1,
-                                         const Axis& a2, const Axis& at) {
-  const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
-  const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
-  const float* s0 = f.data + 3 * (int64_t)at.c * f.slice;
-  const float* s1 = s0 + 3 * f.slice;
+                                         const Axis& a2, const Axis& at, CellCache& cc) {
+  if (a0.c != cc.c0 || a1.c != cc.c1 || a2.c != cc.c2 || at.c != cc.ct) {
+    const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
+    const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
+    const float* s0 = f.data + 3 * (int64_t)at.c * f.slice;
+    const float* s1 = s0 + 3 * f.slice;
+    cc.L0 = node(f, s0, b);            cc.M0 = node(f, s1, b);
+    cc.L1 = node(f, s0, b + 1);        cc.M1 = node(f, s1, b + 1);
+    cc.L2 = node(f, s0, b + n2);       cc.M2 = node(f, s1, b + n2);
+    cc.L3 = node(f, s0, b + n2 + 1);   cc.M3 = node(f, s1, b + n2 + 1);
+    cc.L4 = node(f, s0, b + n12);      cc.M4 = node(f, s1, b + n12);
+    cc.L5 = node(f, s0, b + n12 + 1);  cc.M5 = node(f, s1, b + n12 + 1);
+    cc.L6 = node(f, s0, b + n12 + n2); cc.M6 = node(f, s1, b + n12 + n2);
+    cc.L7 = node(f, s0, b + n12 + n2 + 1);
+    cc.M7 = node(f, s1, b + n12 + n2 + 1);
+    cc.c0 = a0.c; cc.c1 = a1.c; cc.c2 = a2.c; cc.ct = at.c;
+  }
   const float wt = at.w, mt = 1.0f - wt;
-  const int64_t off[8] = {b, b + 1, b + n2, b + n2 + 1, b + n12, b + n12 + 1, b + n12 + n2,
-                          b + n12 + n2 + 1};
-  V3 L[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) L[k] = lerp3(node(f, s0, off[k]), node(f, s1, off[k]), wt, mt);
+  const V3 T0 = lerp3(cc.L0, cc.M0, wt, mt), T1 = lerp3(cc.L1, cc.M1, wt, mt);
+  const V3 T2 = lerp3(cc.L2, cc.M2, wt, mt), T3 = lerp3(cc.L3, cc.M3, wt, mt);
+  const V3 T4 = lerp3(cc.L4, cc.M4, wt, mt), T5 = lerp3(cc.L5, cc.M5, wt, mt);
+  const V3 T6 = lerp3(cc.L6, cc.M6, wt, mt), T7 = lerp3(cc.L7, cc.M7, wt, mt);
   const float w2 = a2.w, m2 = 1.0f - w2;
-  const V3 A0 = lerp3(L[0], L[1], w2, m2), A1 = lerp3(L[2], L[3], w2, m2);
-  const V3 A2 = lerp3(L[4], L[5], w2, m2), A3 = lerp3(L[6], L[7], w2, m2);
+  const V3 A0 = lerp3(T0, T1, w2, m2), A1 = lerp3(T2, T3, w2, m2);
+  const V3 A2 = lerp3(T4, T5, w2, m2), A3 = lerp3(T6, T7, w2, m2);
   const float w1 = a1.w, m1 = 1.0f - w1;
   const V3 B0 = lerp3(A0, A1, w1, m1), B1 = lerp3(A2, A3, w1, m1);
   const float w0 = a0.w, m0 = 1.0f - w0;
@@ -150,8 +163,7 @@ __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, f
   if constexpr (kUnsteady) {
     Axis at;
     if (!locate(t, f.to, f.th, f.nt, at)) return false;
-    (void)cc;
-    v = quadlinear(f, a0, a1, a2, at);
+    v = quadlinear(f, a0, a1, a2, at, cc);
   } else {
     (void)t;
     v = trilinear(f, f.data, a0, a1, a2, cc);

@@ -153,3 +153,80 @@ def cfg2(count=1 << 20, n=256):
     return dict(field=vortex_field(n), spacing=np.ones(3, np.float32),
                 offset=np.zeros(3, np.float32), seeds=cfg2_seeds(count), iterations=1000,
                 step_size=np.float32(0.25), record=False)
+
+
+def cfg3(shape=(128, 512, 512)):
+    """FTLE: ABC on 512 x 512 x 128 nodes (z = axis 0 has 128, extent 127 h)
+    with isotropic spacing h = 2 pi / 511 (origin 0); every node is a lattice
+    point with auxiliary neighbours at +-0.5 h; dt 0.02 for 200 steps (T = 4)."""
+    h = np.float32(2 * math.pi / 511)
+    field, sp = abc_field(shape, spacing=(h, h, h))
+    return dict(field=field, spacing=sp, offset=np.zeros(3, np.float32),
+                lattice_origin=np.zeros(3, np.float32), lattice_spacing=sp,
+                lattice_shape=shape, delta=(np.float32(0.5) * sp).astype(np.float32),
+                iterations=200, step_size=np.float32(0.02))
+
+
+def cfg4_jobs(count=64, n=256, seed_offset=4):
+    """64 jobs on the 256^3 vortex field: seed counts round(exp(U[ln 256,
+    ln 262144])), steps uniform integer in [64, 4096], seeds uniform in a random
+    axis-aligned box (extent U[16, 128] per axis) inside [0, 255)^3; full curves
+    recorded with 32-bit indices (dpa's rule with ppr = the job's seed count)."""
+    rng = np.random.default_rng(MASTER_SEED + seed_offset)
+    jobs = []
+    for j in range(count):
+        seeds = int(round(math.exp(rng.uniform(math.log(256), math.log(262144)))))
+        steps = int(rng.integers(64, 4097))
+        ext = rng.uniform(16, 128, 3)
+        lo = rng.uniform(0, (n - 1) - ext)
+        s = uniform_seeds(seeds, lo, lo + ext, seed_offset=1000 + j)
+        jobs.append(dict(seeds=s, uniform_remaining=steps, step_size=np.float32(0.25),
+                         record=True, use_64_bit=seeds * steps > 0xFFFFFFFF,
+                         particles_per_round=seeds))
+    return jobs
+
+
+def moving_field(base: np.ndarray, time_steps: int, shift=(0, 0, 1), phase=0.0) -> np.ndarray:
+    """Time-variant field from a steady one: slice t is the base pattern moved by
+    t * shift nodes (periodic roll) with its amplitude pulsing as
+    1 + 0.25 sin(0.7 t + phase).  [nt][n0][n1][n2][3] float32."""
+    out = np.empty((time_steps,) + base.shape, np.float32)
+    for t in range(time_steps):
+        amp = np.float32(1.0 + 0.25 * math.sin(0.7 * t + phase))
+        rolled = np.roll(base, shift=(t * shift[0], t * shift[1], t * shift[2]), axis=(0, 1, 2))
+        np.multiply(rolled, amp, out=out[t])
+    return out
+
+
+def serving_trace(rate, count, seed_offset=5, fields=4, ftle_fraction=0.2,
+                  seeds_range=(1000, 262144), steps_range=(128, 2048), n=256,
+                  ftle_shape=(16, 256, 256), time_steps=16, dt=np.float32(0.005)):
+    """Poisson arrivals at `rate` requests/s: a mix of pathline queries (seeds
+    log-uniform in seeds_range, steps uniform in steps_range, start time so the
+    pathline stays inside the time span, record) and FTLE slabs
+    (ftle_shape lattice with +-0.5 cell neighbours, T = 4 at dt 0.02, start
+    time uniform).  Returns a list of dicts with an 'at' arrival offset."""
+    rng = np.random.default_rng(MASTER_SEED + seed_offset + int(rate))
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        f = int(rng.integers(0, fields))
+        if rng.uniform() < ftle_fraction:
+            z0 = int(rng.integers(0, n - ftle_shape[0]))
+            t0 = float(rng.uniform(0, time_steps - 1 - 4.5))
+            out.append(dict(at=t, kind="ftle", field=f, origin=(float(z0), 0.0, 0.0),
+                            spacing=(1.0, 1.0, 1.0), shape=ftle_shape, delta=(0.5, 0.5, 0.5),
+                            iterations=200, step_size=np.float32(0.02), start_time=np.float32(t0)))
+        else:
+            k = int(round(math.exp(rng.uniform(math.log(seeds_range[0]), math.log(seeds_range[1])))))
+            steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+            span = float(dt) * steps
+            t0 = float(rng.uniform(0, max(time_steps - 1 - span - 0.5, 0.0)))
+            ext = rng.uniform(16, 128, 3)
+            lo = rng.uniform(0, (n - 1) - ext)
+            out.append(dict(at=t, kind="advect", field=f,
+                            seeds=uniform_seeds(k, lo, lo + ext, seed_offset=5000 + i),
+                            uniform_remaining=steps, step_size=dt, start_time=np.float32(t0),
+                            record=True))
+    return out
