@@ -70,6 +70,31 @@ def build_library(jobs: int = 4) -> Path:
     return LIB
 
 
+HOST_SOURCES = ["json.cpp", "stages.cpp", "io.cpp", "pipeline.cpp", "main.cpp"]
+HOST = HERE / "host"
+CLI = HERE / "bin" / "dpa_hip"
+HOST_CXX = os.environ.get("DPAH_HOST_CXX", "/opt/rocm/llvm/bin/clang++")
+HDF5_PREFIX = Path(os.environ.get("DPAH_HDF5_PREFIX", "/opt/conda"))
+
+
+def build_cli() -> Path:
+    """The dpa_hip command line (host C++ over libdpa_hip.so), with HDF5 I/O
+    when a serial HDF5 is found."""
+    CLI.parent.mkdir(exist_ok=True)
+    # the same host compiler as hipcc, so both sides agree on the C++ runtime
+    cmd = [HOST_CXX, "-std=c++17", "-O2", "-Wall", "-Wextra", "-fPIC",
+           *[str(HOST / s) for s in HOST_SOURCES], "-o", str(CLI.with_suffix(".tmp")),
+           f"-L{HERE}", "-ldpa_hip", "-Wl,-rpath,$ORIGIN/.."]
+    if (HDF5_PREFIX / "include" / "hdf5.h").exists():
+        cmd[4:4] = [f"-I{HDF5_PREFIX / 'include'}", "-DDPAH_WITH_HDF5"]
+        # system libraries first: the HDF5 prefix may carry an older libstdc++
+        cmd += [f"-L{HDF5_PREFIX / 'lib'}", "-lhdf5",
+                f"-Wl,-rpath,/usr/lib/x86_64-linux-gnu:{HDF5_PREFIX / 'lib'}"]
+    _run(cmd)
+    os.replace(CLI.with_suffix(".tmp"), CLI)
+    return CLI
+
+
 def build_oracle() -> Path:
     """The CPU oracle is test infrastructure; it is built here so that tests,
     the smoke check and the bench's CPU baseline find it."""
@@ -79,6 +104,7 @@ def build_oracle() -> Path:
 
 def build_all() -> None:
     build_library()
+    build_cli()
     build_oracle()
 
 

@@ -25,26 +25,68 @@ using namespace dpah;
 
 // ------------------------------------------------------------ context ----
 
-void* dpah_context::alloc(uint64_t bytes) {
+namespace {
+uint64_t round_block(uint64_t bytes) {
   if (bytes == 0) bytes = 1;
-  if (bytes_in_use.load() + bytes > memory_budget)
-    throw Error(DPAH_ERR_NOMEM, "device memory budget exceeded: " + std::to_string(bytes) +
+  const uint64_t g = bytes >= (1ull << 20) ? (2ull << 20) : 512;
+  return (bytes + g - 1) / g * g;
+}
+}  // namespace
+
+void dpah_context::trim_cache() {
+  for (auto& kv : cached) {
+    (void)hipFree(kv.second);
+    block_size.erase(kv.second);
+  }
+  cached.clear();
+  cached_bytes = 0;
+}
+
+void* dpah_context::alloc(uint64_t bytes) {
+  const uint64_t want = round_block(bytes);
+  if (caching) {
+    auto it = cached.lower_bound(want);
+    if (it != cached.end() && it->first <= 2 * want) {
+      void* p = it->second;
+      bytes_in_use += it->first;
+      cached_bytes -= it->first;
+      cached.erase(it);
+      return p;
+    }
+  }
+  if (bytes_in_use.load() + want > memory_budget)
+    throw Error(DPAH_ERR_NOMEM, "device memory budget exceeded: " + std::to_string(want) +
                                     " bytes requested, " + std::to_string(bytes_in_use.load()) +
                                     " of " + std::to_string(memory_budget) + " in use");
+  if (bytes_in_use.load() + cached_bytes + want > memory_budget) trim_cache();
   void* p = nullptr;
-  const hipError_t e = hipMalloc(&p, bytes);
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess && cached_bytes) {
+    (void)hipGetLastError();
+    trim_cache();
+    e = hipMalloc(&p, want);
+  }
   if (e != hipSuccess)
-    throw Error(DPAH_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " +
+    throw Error(DPAH_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(want) + "): " +
                                     hipGetErrorString(e));
-  bytes_in_use += bytes;
+  block_size[p] = want;
+  bytes_in_use += want;
   return p;
 }
 
-void dpah_context::release(void* p, uint64_t bytes) {
+void dpah_context::release(void* p, uint64_t /*bytes*/) {
   if (!p) return;
-  if (bytes == 0) bytes = 1;
-  (void)hipFree(p);
-  bytes_in_use -= bytes;
+  auto it = block_size.find(p);
+  if (it == block_size.end()) return;
+  const uint64_t sz = it->second;
+  bytes_in_use -= sz;
+  if (caching) {
+    cached.emplace(sz, p);
+    cached_bytes += sz;
+  } else {
+    (void)hipFree(p);
+    block_size.erase(it);
+  }
 }
 
 hipEvent_t dpah_context::take_event() {
@@ -125,6 +167,8 @@ DPAH_EXPORT int dpah_destroy(dpah_context* ctx) {
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto& a : ctx->user_allocs) ctx->release(a.first, a.second);
     if (ctx->d_counters) ctx->release(ctx->d_counters, 8 * sizeof(unsigned long long));
+    ctx->trim_cache();
+    for (auto& kv : ctx->block_size) (void)hipFree(kv.first);   // blocks never released
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });

@@ -82,8 +82,17 @@ struct dpah_context {
   int cu_count = 0;
   uint64_t memory_budget = 0;
   uint64_t slot_budget = 0;
-  std::atomic<uint64_t> bytes_in_use{0};
+  std::atomic<uint64_t> bytes_in_use{0};     // live blocks (rounded sizes)
   std::map<void*, uint64_t> user_allocs;      // dpah_device_alloc blocks and their sizes
+  // Caching allocator: released blocks are kept, keyed by size, and handed out
+  // again (stream order on the single context stream makes reuse safe), so
+  // repeated passes do not pay hipMalloc / hipFree.  `caching` false makes
+  // alloc / release go to the runtime every time (the per-request form).
+  bool caching = true;
+  std::multimap<uint64_t, void*> cached;      // size -> block
+  std::map<void*, uint64_t> block_size;       // every live or cached block
+  uint64_t cached_bytes = 0;
+  void trim_cache();
   unsigned long long* d_counters = nullptr;   // [0] queue head, [1] steps, [2..7] spare
   bool timing = false;
   std::map<std::string, dpah::TimedKernel> timings;

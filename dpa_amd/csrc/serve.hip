@@ -491,6 +491,13 @@ void dpah_server::run_per_request() {
     try {
       std::lock_guard<std::recursive_mutex> ctx_lk(ctx->mu);
       DPAH_HIP(hipSetDevice(ctx->device));
+      // the per-request form allocates and frees through the runtime every time
+      struct Uncached {
+        dpah_context* c;
+        bool was;
+        explicit Uncached(dpah_context* x) : c(x), was(x->caching) { c->caching = false; }
+        ~Uncached() { c->caching = was; }
+      } uncached(ctx);
       if (r->spec.kind == DPAH_REQUEST_FTLE_AUX) {
         r->result = run_ftle_aux(ctx, r->spec.field, r->spec.origin, r->spec.lattice_spacing,
                                  r->spec.lattice_shape, r->spec.delta, r->spec.iterations,
