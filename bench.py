@@ -97,7 +97,7 @@ def roofline(steps_per_launch, avg_ms, bytes_per_step=BYTES_PER_STEP, traffic=No
 
 # ----------------------------------------------------------------- serving
 
-def serve_trace(ctx, fields, trace, mode, steps_per_tick=256):
+def serve_trace(ctx, fields, trace, mode, steps_per_tick=0):
     """Replays a trace against a server; returns latency / throughput figures."""
     import dpa_amd
     lat, steps = [], 0
@@ -150,7 +150,7 @@ def serve_trace(ctx, fields, trace, mode, steps_per_tick=256):
             "p99_ms": float(np.percentile(lat, 99) * 1e3)}
 
 
-def serving_leg(ctx, fields, trace, steps_per_tick=256):
+def serving_leg(ctx, fields, trace, steps_per_tick=0):
     import dpa_amd
     b = serve_trace(ctx, fields, trace, dpa_amd.SERVE_BATCHED, steps_per_tick)
     log(f"served batched: {b['throughput']:.3e} steps/s p50 {b['p50_ms']:.1f} ms "

@@ -101,7 +101,12 @@ hipEvent_t dpah_context::take_event() {
 }
 
 void dpah_context::resolve_timings() {
+  std::vector<dpah::PendingTiming> still;
   for (auto& t : pending) {
+    if (hipEventQuery(t.stop) == hipErrorNotReady) {   // another stream, not finished yet
+      still.push_back(t);
+      continue;
+    }
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
       auto& k = timings[t.name];
@@ -111,7 +116,7 @@ void dpah_context::resolve_timings() {
     event_pool.push_back(t.start);
     event_pool.push_back(t.stop);
   }
-  pending.clear();
+  pending.swap(still);
 }
 
 dpah_result::~dpah_result() {

@@ -5,7 +5,13 @@
 namespace dpah {
 
 Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const uint32_t* steps,
-                   const uint8_t* special, uint64_t count, uint64_t vertex_base, int use64) {
+                   const uint8_t* special, uint64_t count, uint64_t vertex_base, int use64,
+                   hipStream_t stream) {
+  hipStream_t st = stream ? stream : ctx->stream;
+  auto sync = [&] {
+    DPAH_HIP(hipStreamSynchronize(st));
+    ctx->resolve_timings();
+  };
   Chunk ch;
   if (count == 0) return ch;
   DeviceBuffer vcount(ctx, count * sizeof(uint64_t));
@@ -25,19 +31,19 @@ Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const
   a.vertex_base = vertex_base;
 
   const unsigned g1 = (unsigned)((count + 255) / 256);
-  hipLaunchKernelGGL(k_compact_count, dim3(g1), dim3(256), 0, ctx->stream, a);
+  hipLaunchKernelGGL(k_compact_count, dim3(g1), dim3(256), 0, st, a);
   DPAH_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)blocks), dim3(kScanThreads), 0, ctx->stream,
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)blocks), dim3(kScanThreads), 0, st,
                      a.vcount, a.icount, count, sums.as<uint64_t>());
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, ctx->stream, sums.as<uint64_t>(),
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, sums.as<uint64_t>(),
                      blocks);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)blocks), dim3(kScanThreads), 0, ctx->stream,
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)blocks), dim3(kScanThreads), 0, st,
                      a.vcount, a.icount, count, sums.as<uint64_t>());
   DPAH_HIP(hipGetLastError());
   uint64_t totals[2] = {0, 0};
   DPAH_HIP(hipMemcpyAsync(totals, sums.as<uint64_t>() + 2 * blocks, sizeof(totals),
-                          hipMemcpyDeviceToHost, ctx->stream));
-  ctx->sync();
+                          hipMemcpyDeviceToHost, st));
+  sync();
   ch.V = totals[0];
   ch.I = 2 * totals[1];
   const uint64_t ib = use64 ? 8 : 4;
@@ -53,12 +59,12 @@ Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const
   a.colors = ch.colors;
   a.indices = ch.indices;
   const uint64_t threads = count * 64;
-  ctx->timed("compact_write", [&] {
+  ctx->timed("compact_write", st, [&] {
     hipLaunchKernelGGL(k_compact_write, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
-                       ctx->stream, a);
+                       st, a);
   });
   DPAH_HIP(hipGetLastError());
-  ctx->sync();
+  sync();
   return ch;
 }
 

@@ -106,14 +106,16 @@ struct dpah_context {
   hipEvent_t take_event();
   void resolve_timings();  // call after the stream is synchronised
   template <class F>
-  void timed(const char* name, F&& launch) {
+  void timed(const char* name, hipStream_t s, F&& launch) {
     if (!timing) { launch(); return; }
     dpah::PendingTiming t{name, take_event(), take_event()};
-    DPAH_HIP(hipEventRecord(t.start, stream));
+    DPAH_HIP(hipEventRecord(t.start, s));
     launch();
-    DPAH_HIP(hipEventRecord(t.stop, stream));
+    DPAH_HIP(hipEventRecord(t.stop, s));
     pending.push_back(t);
   }
+  template <class F>
+  void timed(const char* name, F&& launch) { timed(name, stream, launch); }
   void sync() {
     DPAH_HIP(hipStreamSynchronize(stream));
     resolve_timings();
@@ -228,6 +230,7 @@ dpah_result* run_ftle_aux(dpah_context* ctx, const dpah_field* field, const floa
 
 // Prunes, indexes and colours `count` slot rows; allocates the chunk's outputs.
 Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const uint32_t* steps,
-                   const uint8_t* special, uint64_t count, uint64_t vertex_base, int use64);
+                   const uint8_t* special, uint64_t count, uint64_t vertex_base, int use64,
+                   hipStream_t stream = nullptr);
 
 }  // namespace dpah

@@ -57,7 +57,11 @@ struct Request {
   // device state while running
   int slot = -1;
   DeviceBuffer pos, rem_d, it, status, special;
+  DeviceBuffer seeds_d, rem_in_d;      // staged inputs
   std::vector<RoundState> rounds;
+  bool set_up = false;
+  unsigned int left_h = 0;             // host staging of the device counters
+  unsigned long long steps_h = 0;
   dpah_result* result = nullptr;
 };
 
@@ -66,14 +70,14 @@ uint64_t ftle_points(const dpah_request& r) {
 }
 
 template <bool kRecord, int kIntegrator, bool kUnsteady>
-void launch_tick_t(dpah_context* ctx, const TickArgs& a) {
+void launch_tick_t(dpah_context* ctx, const TickArgs& a, hipStream_t s) {
   const int block = 256;
   uint64_t grid = (uint64_t)ctx->cu_count * ctx->tune.blocks_per_cu;
   const uint64_t need = (a.n + block - 1) / block;
   if (grid > need) grid = need;
-  ctx->timed("advect_tick", [&] {
+  ctx->timed("advect_tick", s, [&] {
     hipLaunchKernelGGL((k_advect_tick<kRecord, kIntegrator, kUnsteady>), dim3((unsigned)grid),
-                       dim3(block), 0, ctx->stream, a);
+                       dim3(block), 0, s, a);
   });
   DPAH_HIP(hipGetLastError());
 }
@@ -82,55 +86,68 @@ __global__ void k_reset_queue(unsigned long long* q) {
   if (threadIdx.x == 0) *q = 0ull;
 }
 
-void launch_tick(dpah_context* ctx, const TickArgs& a, int integrator, bool record) {
+void launch_tick(dpah_context* ctx, const TickArgs& a, int integrator, bool record,
+                 hipStream_t s) {
   if (a.n == 0) return;
-  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, a.queue);
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, a.queue);
   const bool unsteady = a.f.nt > 1;
   if (record) {
     if (integrator == kRk4) {
-      if (unsteady) launch_tick_t<true, kRk4, true>(ctx, a); else launch_tick_t<true, kRk4, false>(ctx, a);
+      if (unsteady) launch_tick_t<true, kRk4, true>(ctx, a, s); else launch_tick_t<true, kRk4, false>(ctx, a, s);
     } else {
-      if (unsteady) launch_tick_t<true, kDpaRk4, true>(ctx, a); else launch_tick_t<true, kDpaRk4, false>(ctx, a);
+      if (unsteady) launch_tick_t<true, kDpaRk4, true>(ctx, a, s); else launch_tick_t<true, kDpaRk4, false>(ctx, a, s);
     }
   } else {
     if (integrator == kRk4) {
-      if (unsteady) launch_tick_t<false, kRk4, true>(ctx, a); else launch_tick_t<false, kRk4, false>(ctx, a);
+      if (unsteady) launch_tick_t<false, kRk4, true>(ctx, a, s); else launch_tick_t<false, kRk4, false>(ctx, a, s);
     } else {
-      if (unsteady) launch_tick_t<false, kDpaRk4, true>(ctx, a); else launch_tick_t<false, kDpaRk4, false>(ctx, a);
+      if (unsteady) launch_tick_t<false, kDpaRk4, true>(ctx, a, s); else launch_tick_t<false, kDpaRk4, false>(ctx, a, s);
     }
   }
 }
 
 }  // namespace
 
+struct Launch {
+  bool busy = false;
+  std::vector<std::shared_ptr<Request>> reqs;
+  DeviceBuffer segs;
+  hipEvent_t done = nullptr;
+};
+
 struct dpah_server {
+  static constexpr int kStreams = 4;
   dpah_context* ctx = nullptr;
   dpah_server_options opt{};
   uint64_t budget = 0;
   uint64_t in_use = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<std::shared_ptr<Request>> pending;
-  std::vector<std::shared_ptr<Request>> active;
+  std::deque<std::shared_ptr<Request>> pending;   // submitted, not admitted
+  std::vector<std::shared_ptr<Request>> ready;    // admitted, waiting for a stream
   std::map<uint64_t, std::shared_ptr<Request>> all;
   uint64_t next_ticket = 1;
   bool stop = false;
   std::thread engine;
-  // per active slot: [0] active_left (u32 in a u64 cell), steps
+  // per admitted request (slot): particles not yet retired, particle-steps
   unsigned int* d_left = nullptr;
   unsigned long long* d_steps = nullptr;
   std::vector<int> free_slots;
+  hipStream_t streams[kStreams] = {};
+  unsigned long long* d_queues = nullptr;         // one queue head per stream
+  Launch launches[kStreams];
   dpah_server_stats stats{};
-  DeviceBuffer seg_buf;       // tick segment table, grown on demand
 
   void run();
   void run_batched();
   void run_per_request();
-  bool admit_locked(std::vector<std::shared_ptr<Request>>& admitted);
-  void setup(Request& r);
-  void tick();
-  void poll_and_finish();
-  void finish(Request& r);
+  bool admit_locked();
+  void setup(Request& r, hipStream_t s);
+  bool launch_ready();
+  bool poll_launches();
+  void finish(Request& r, hipStream_t s);
+  void release_request(const std::shared_ptr<Request>& r, int state, int err,
+                       const std::string& msg);
   void complete(const std::shared_ptr<Request>& r, int state, int err, const std::string& msg);
 };
 
@@ -142,11 +159,11 @@ uint64_t footprint(const Request& r) {
   const uint64_t n = r.count;
   uint64_t b = n * (12 + 8 + 4 + 1 + 1) + 64;
   if (r.spec.kind == DPAH_REQUEST_FTLE_AUX) return b + ftle_points(r.spec) * 4;
+  b += n * (12 + 8);   // staged seeds and remaining
   if (r.spec.params.record) {
     const uint64_t ib = r.spec.params.use_64_bit_indices ? 8 : 4;
     uint64_t verts = 0;
     for (uint64_t i = 0; i < n; ++i) verts += r.rem[i] + 2;
-    // slots rows are at most (max remaining + 2) per particle of a round
     const uint64_t ppr = r.spec.params.particles_per_round;
     uint64_t slots = 0, active = n;
     do {
@@ -181,9 +198,26 @@ void dpah_server::complete(const std::shared_ptr<Request>& r, int state, int err
   cv_done.notify_all();
 }
 
-bool dpah_server::admit_locked(std::vector<std::shared_ptr<Request>>& admitted) {
+// Frees a request's running state and its admission slot, then completes it.
+void dpah_server::release_request(const std::shared_ptr<Request>& r, int state, int err,
+                                  const std::string& msg) {
+  r->pos.reset(); r->rem_d.reset(); r->it.reset(); r->status.reset(); r->special.reset();
+  r->seeds_d.reset(); r->rem_in_d.reset();
+  r->rounds.clear();
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    in_use -= r->footprint;
+    if (r->slot >= 0) free_slots.push_back(r->slot);
+    r->slot = -1;
+  }
+  complete(r, state, err, msg);
+}
+
+// FIFO admission by memory footprint and admission slots (no overtaking).
+bool dpah_server::admit_locked() {
+  bool any = false;
   while (!pending.empty()) {
-    auto& r = pending.front();
+    auto r = pending.front();
     if (r->cancel) {
       r->state = kCancelled;
       r->t_done = now_s();
@@ -198,13 +232,15 @@ bool dpah_server::admit_locked(std::vector<std::shared_ptr<Request>>& admitted) 
     in_use += r->footprint;
     r->state = kRunning;
     r->t_admit = now_s();
-    admitted.push_back(r);
+    ready.push_back(r);
     pending.pop_front();
+    any = true;
   }
-  return !admitted.empty();
+  return any;
 }
 
-void dpah_server::setup(Request& r) {
+// Device state of a newly admitted request, written on stream s.
+void dpah_server::setup(Request& r, hipStream_t s) {
   const uint64_t n = r.count;
   const uint64_t n1 = std::max<uint64_t>(n, 1);
   r.pos = DeviceBuffer(ctx, 3 * n1 * sizeof(float));
@@ -214,7 +250,6 @@ void dpah_server::setup(Request& r) {
   r.special = DeviceBuffer(ctx, n1);
   const bool ftle = r.spec.kind == DPAH_REQUEST_FTLE_AUX;
   const bool record = !ftle && r.spec.params.record;
-  DeviceBuffer seeds, rem;
   TickInitArgs ia{};
   ia.count = n;
   ia.pos = r.pos.as<float>();
@@ -232,19 +267,21 @@ void dpah_server::setup(Request& r) {
     L.dl0 = r.spec.delta[0]; L.dl1 = r.spec.delta[1]; L.dl2 = r.spec.delta[2];
     ia.uniform_rem = r.spec.iterations;
   } else {
-    seeds = DeviceBuffer(ctx, 3 * n1 * sizeof(float));
-    rem = DeviceBuffer(ctx, n1 * sizeof(uint64_t));
+    // staged inputs live until the request finishes (another stream may reuse
+    // a released block before this stream has read it)
+    r.seeds_d = DeviceBuffer(ctx, 3 * n1 * sizeof(float));
+    r.rem_in_d = DeviceBuffer(ctx, n1 * sizeof(uint64_t));
     if (n) {
-      DPAH_HIP(hipMemcpyAsync(seeds.ptr, r.seeds.data(), 3 * n * sizeof(float),
-                              hipMemcpyHostToDevice, ctx->stream));
-      DPAH_HIP(hipMemcpyAsync(rem.ptr, r.rem.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice,
-                              ctx->stream));
+      DPAH_HIP(hipMemcpyAsync(r.seeds_d.ptr, r.seeds.data(), 3 * n * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+      DPAH_HIP(hipMemcpyAsync(r.rem_in_d.ptr, r.rem.data(), n * sizeof(uint64_t),
+                              hipMemcpyHostToDevice, s));
     }
     ia.seed_kind = kSeedArray;
-    ia.seeds = seeds.as<float>();
-    ia.rem_in = rem.as<uint64_t>();
+    ia.seeds = r.seeds_d.as<float>();
+    ia.rem_in = r.rem_in_d.as<uint64_t>();
   }
-  // rounds (tail first); each round has its own slot rows
+  // dpa's rounds (tail first); each round has its own slot rows
   const uint64_t ppr = ftle ? std::max<uint64_t>(n, 1) : r.spec.params.particles_per_round;
   uint64_t active = n;
   do {
@@ -270,75 +307,158 @@ void dpah_server::setup(Request& r) {
     a.special = ia.special + rs.first;
     a.slots = record ? rs.slots.as<float>() : nullptr;
     a.stride = rs.stride;
-    hipLaunchKernelGGL(k_tick_init, dim3((unsigned)((rs.count + 255) / 256)), dim3(256), 0,
-                       ctx->stream, a);
+    hipLaunchKernelGGL(k_tick_init, dim3((unsigned)((rs.count + 255) / 256)), dim3(256), 0, s, a);
   }
   DPAH_HIP(hipGetLastError());
-  const unsigned int left = (unsigned int)n;
-  DPAH_HIP(hipMemcpyAsync(d_left + r.slot, &left, sizeof(left), hipMemcpyHostToDevice,
-                          ctx->stream));
-  const unsigned long long zero = 0;
-  DPAH_HIP(hipMemcpyAsync(d_steps + r.slot, &zero, sizeof(zero), hipMemcpyHostToDevice,
-                          ctx->stream));
-  ctx->sync();   // seeds / rem staging buffers die here
+  r.left_h = (unsigned int)n;
+  DPAH_HIP(hipMemcpyAsync(d_left + r.slot, &r.left_h, sizeof(unsigned int), hipMemcpyHostToDevice,
+                          s));
+  r.steps_h = 0;
+  DPAH_HIP(hipMemcpyAsync(d_steps + r.slot, &r.steps_h, sizeof(unsigned long long),
+                          hipMemcpyHostToDevice, s));
+  r.set_up = true;
 }
 
-void dpah_server::tick() {
-  // group running requests by launch-uniform arguments
-  std::map<std::tuple<const dpah_field*, int, int>, std::vector<TickSeg>> groups;
-  for (auto& rp : active) {
-    Request& r = *rp;
-    const bool ftle = r.spec.kind == DPAH_REQUEST_FTLE_AUX;
-    const bool record = !ftle && r.spec.params.record;
-    const int integ = r.spec.params.integrator;
-    auto& segs = groups[{r.spec.field, integ, record ? 1 : 0}];
-    for (RoundState& rs : r.rounds) {
-      if (!rs.count) continue;
-      TickSeg s{};
-      s.count = rs.count;
-      s.pos = r.pos.as<float>() + 3 * rs.first;
-      s.rem = r.rem_d.as<uint64_t>() + rs.first;
-      s.it = r.it.as<uint32_t>() + rs.first;
-      s.status = r.status.as<uint8_t>() + rs.first;
-      s.special = r.special.as<uint8_t>() + rs.first;
-      s.slots = record ? rs.slots.as<float>() : nullptr;
-      s.stride = rs.stride;
-      s.dt = r.spec.params.step_size;
-      s.t0 = r.spec.params.start_time;
-      s.active_left = d_left + r.slot;
-      s.steps = d_steps + r.slot;
-      segs.push_back(s);
+// Starts one launch per free stream, each over the ready requests that share
+// field, integrator and record flag (everything admitted since the last
+// launch is coalesced into it).
+bool dpah_server::launch_ready() {
+  bool launched = false;
+  while (!ready.empty()) {
+    int sid = -1;
+    for (int i = 0; i < kStreams; ++i)
+      if (!launches[i].busy) { sid = i; break; }
+    if (sid < 0) break;
+    const Request& head = *ready.front();
+    const bool rec0 = head.spec.kind == DPAH_REQUEST_ADVECT && head.spec.params.record;
+    const auto key = std::make_tuple(head.spec.field, head.spec.params.integrator, rec0);
+    Launch& L = launches[sid];
+    L.reqs.clear();
+    std::vector<std::shared_ptr<Request>> rest;
+    for (auto& r : ready) {
+      const bool rec = r->spec.kind == DPAH_REQUEST_ADVECT && r->spec.params.record;
+      if (std::make_tuple(r->spec.field, r->spec.params.integrator, rec) == key)
+        L.reqs.push_back(r);
+      else
+        rest.push_back(r);
     }
-  }
-  for (auto& g : groups) {
-    auto& segs = g.second;
-    if (segs.empty()) continue;
+    ready.swap(rest);
+    hipStream_t s = streams[sid];
+    std::vector<TickSeg> segs;
+    std::vector<std::shared_ptr<Request>> live;
+    for (auto& rp : L.reqs) {
+      Request& r = *rp;
+      bool cancelled;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        cancelled = r.cancel;
+      }
+      if (cancelled) {
+        release_request(rp, kCancelled, DPAH_ERR_CANCELLED, "");
+        continue;
+      }
+      try {
+        if (!r.set_up) setup(r, s);
+      } catch (const Error& e) {
+        release_request(rp, kFailed, e.code, e.what());
+        continue;
+      }
+      live.push_back(rp);
+      for (RoundState& rs : r.rounds) {
+        if (!rs.count) continue;
+        TickSeg t{};
+        t.count = rs.count;
+        t.pos = r.pos.as<float>() + 3 * rs.first;
+        t.rem = r.rem_d.as<uint64_t>() + rs.first;
+        t.it = r.it.as<uint32_t>() + rs.first;
+        t.status = r.status.as<uint8_t>() + rs.first;
+        t.special = r.special.as<uint8_t>() + rs.first;
+        t.slots = rec0 ? rs.slots.as<float>() : nullptr;
+        t.stride = rs.stride;
+        t.dt = r.spec.params.step_size;
+        t.t0 = r.spec.params.start_time;
+        t.active_left = d_left + r.slot;
+        t.steps = d_steps + r.slot;
+        segs.push_back(t);
+      }
+    }
+    L.reqs.swap(live);
+    if (L.reqs.empty()) continue;
     uint64_t q = 0;
-    for (auto& s : segs) { s.qbase = q; q += s.count; }
-    const uint64_t need = segs.size() * sizeof(TickSeg);
-    if (seg_buf.bytes < need) seg_buf = DeviceBuffer(ctx, std::max<uint64_t>(need, 64 * sizeof(TickSeg)));
-    DPAH_HIP(hipMemcpyAsync(seg_buf.ptr, segs.data(), need, hipMemcpyHostToDevice, ctx->stream));
+    for (auto& t : segs) { t.qbase = q; q += t.count; }
+    L.segs = DeviceBuffer(ctx, std::max<size_t>(segs.size(), 1) * sizeof(TickSeg));
+    if (!segs.empty())
+      DPAH_HIP(hipMemcpyAsync(L.segs.ptr, segs.data(), segs.size() * sizeof(TickSeg),
+                              hipMemcpyHostToDevice, s));
     TickArgs a{};
-    a.f = std::get<0>(g.first)->view;
-    a.segs = seg_buf.as<TickSeg>();
+    a.f = std::get<0>(key)->view;
+    a.segs = L.segs.as<TickSeg>();
     a.nseg = (uint32_t)segs.size();
     a.n = q;
-    a.budget = std::max<uint32_t>(1, opt.steps_per_tick);
-    a.queue = ctx->d_counters + 3;
-    launch_tick(ctx, a, std::get<1>(g.first), std::get<2>(g.first) != 0);
-    ctx->sync();   // the table is rewritten by the next group
+    a.budget = opt.steps_per_tick ? opt.steps_per_tick : 0xffffffffu;
+    a.queue = d_queues + sid;
+    launch_tick(ctx, a, std::get<1>(key), rec0, s);
+    DPAH_HIP(hipEventRecord(L.done, s));
+    L.busy = true;
+    launched = true;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stats.ticks += 1;
+    }
   }
-  stats.ticks += 1;
+  return launched;
 }
 
-void dpah_server::finish(Request& r) {
+// Finishes the requests of every completed launch.
+bool dpah_server::poll_launches() {
+  bool progressed = false;
+  for (int sid = 0; sid < kStreams; ++sid) {
+    Launch& L = launches[sid];
+    if (!L.busy) continue;
+    const hipError_t q = hipEventQuery(L.done);
+    if (q == hipErrorNotReady) continue;
+    DPAH_HIP(q);
+    hipStream_t s = streams[sid];
+    for (auto& rp : L.reqs)
+      DPAH_HIP(hipMemcpyAsync(&rp->left_h, d_left + rp->slot, sizeof(unsigned int),
+                              hipMemcpyDeviceToHost, s));
+    DPAH_HIP(hipStreamSynchronize(s));
+    ctx->resolve_timings();
+    for (auto& rp : L.reqs) {
+      Request& r = *rp;
+      bool cancelled;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        cancelled = r.cancel;
+      }
+      if (cancelled) {
+        release_request(rp, kCancelled, DPAH_ERR_CANCELLED, "");
+      } else if (r.left_h == 0) {
+        try {
+          finish(r, s);
+          release_request(rp, kDone, DPAH_OK, "");
+        } catch (const Error& e) {
+          release_request(rp, kFailed, e.code, e.what());
+        }
+      } else {
+        ready.push_back(rp);   // step budget used up: continue in a later launch
+      }
+    }
+    L.reqs.clear();
+    L.segs.reset();
+    L.busy = false;
+    progressed = true;
+  }
+  return progressed;
+}
+
+void dpah_server::finish(Request& r, hipStream_t s) {
   const bool ftle = r.spec.kind == DPAH_REQUEST_FTLE_AUX;
   auto res = std::make_unique<dpah_result>();
   res->ctx = ctx;
   const uint64_t n = r.count;
-  unsigned long long steps = 0;
-  DPAH_HIP(hipMemcpyAsync(&steps, d_steps + r.slot, sizeof(steps), hipMemcpyDeviceToHost,
-                          ctx->stream));
+  DPAH_HIP(hipMemcpyAsync(&r.steps_h, d_steps + r.slot, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
   if (ftle) {
     const uint64_t P = ftle_points(r.spec);
     res->ftle_count = P;
@@ -351,8 +471,8 @@ void dpah_server::finish(Request& r) {
     a.T_abs = std::fabs(r.spec.params.step_size * (float)r.spec.iterations);
     a.out = res->d_ftle;
     if (P)
-      hipLaunchKernelGGL(k_ftle_aux_reduce, dim3((unsigned)((P + 255) / 256)), dim3(256), 0,
-                         ctx->stream, a);
+      hipLaunchKernelGGL(k_ftle_aux_reduce, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s,
+                         a);
     DPAH_HIP(hipGetLastError());
   } else {
     res->use64 = r.spec.params.use_64_bit_indices ? 1 : 0;
@@ -361,11 +481,11 @@ void dpah_server::finish(Request& r) {
     res->final_status.resize(n);
     if (n) {
       DPAH_HIP(hipMemcpyAsync(res->final_pos.data(), r.pos.ptr, 3 * n * sizeof(float),
-                              hipMemcpyDeviceToHost, ctx->stream));
+                              hipMemcpyDeviceToHost, s));
       DPAH_HIP(hipMemcpyAsync(res->final_rem.data(), r.rem_d.ptr, n * sizeof(uint64_t),
-                              hipMemcpyDeviceToHost, ctx->stream));
+                              hipMemcpyDeviceToHost, s));
       DPAH_HIP(hipMemcpyAsync(res->final_status.data(), r.status.ptr, n, hipMemcpyDeviceToHost,
-                              ctx->stream));
+                              s));
     }
     if (r.spec.params.record) {
       for (RoundState& rs : r.rounds) {
@@ -373,95 +493,54 @@ void dpah_server::finish(Request& r) {
         if (rs.count) {
           Chunk ch = compact_rows(ctx, rs.slots.as<float>(), rs.stride,
                                   r.it.as<uint32_t>() + rs.first,
-                                  r.special.as<uint8_t>() + rs.first, rs.count, 0, res->use64);
+                                  r.special.as<uint8_t>() + rs.first, rs.count, 0, res->use64, s);
           round.V = ch.V;
           round.I = ch.I;
           round.chunks.push_back(ch);
+          if (!res->use64 && round.V > 0xffffffffull)
+            throw Error(DPAH_ERR_INVALID, "round has more than 2^32-1 vertices");
         }
         res->rounds.push_back(std::move(round));
       }
     }
   }
-  ctx->sync();
-  res->total_steps = steps;
-  r.steps = steps;
+  DPAH_HIP(hipStreamSynchronize(s));
+  res->total_steps = r.steps_h;
+  r.steps = r.steps_h;
   r.result = res.release();
-  // release the running state
-  r.pos.reset(); r.rem_d.reset(); r.it.reset(); r.status.reset(); r.special.reset();
-  r.rounds.clear();
-}
-
-void dpah_server::poll_and_finish() {
-  if (active.empty()) return;
-  std::vector<unsigned int> left(opt.max_active);
-  DPAH_HIP(hipMemcpyAsync(left.data(), d_left, opt.max_active * sizeof(unsigned int),
-                          hipMemcpyDeviceToHost, ctx->stream));
-  ctx->sync();
-  std::vector<std::shared_ptr<Request>> still;
-  for (auto& rp : active) {
-    Request& r = *rp;
-    bool cancelled;
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      cancelled = r.cancel;
-    }
-    if (cancelled) {
-      r.pos.reset(); r.rem_d.reset(); r.it.reset(); r.status.reset(); r.special.reset();
-      r.rounds.clear();
-    } else if (left[r.slot] == 0) {
-      try {
-        finish(r);
-      } catch (const Error& e) {
-        r.rounds.clear();
-        complete(rp, kFailed, e.code, e.what());
-        std::lock_guard<std::mutex> lk(mu);
-        in_use -= r.footprint;
-        free_slots.push_back(r.slot);
-        continue;
-      }
-    } else {
-      still.push_back(rp);
-      continue;
-    }
-    complete(rp, cancelled ? kCancelled : kDone, cancelled ? DPAH_ERR_CANCELLED : DPAH_OK, "");
-    std::lock_guard<std::mutex> lk(mu);
-    in_use -= r.footprint;
-    free_slots.push_back(r.slot);
-  }
-  active.swap(still);
 }
 
 void dpah_server::run_batched() {
   while (true) {
-    std::vector<std::shared_ptr<Request>> admitted;
+    bool busy = false, free_stream = false;
+    for (auto& L : launches) {
+      busy |= L.busy;
+      free_stream |= !L.busy;
+    }
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
-      if (stop && pending.empty() && active.empty()) return;
-      if (stop) {   // cancel what is left
+      if (!busy && ready.empty())
+        cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+      else if (busy && !stop && (ready.empty() || !free_stream))
+        // launches in flight: poll their events every 50 us, or sooner when a
+        // request arrives and a stream is free to take it
+        cv_work.wait_for(lk, std::chrono::microseconds(50),
+                         [&] { return stop || (free_stream && !pending.empty()); });
+      if (stop) {
         for (auto& r : pending) r->cancel = true;
-        for (auto& r : active) r->cancel = true;
+        for (auto& r : ready) r->cancel = true;
+        for (auto& L : launches)
+          for (auto& r : L.reqs) r->cancel = true;
+        if (pending.empty() && ready.empty() && !busy) return;
       }
-      admit_locked(admitted);
+      admit_locked();
     }
     std::lock_guard<std::recursive_mutex> ctx_lk(ctx->mu);
     DPAH_HIP(hipSetDevice(ctx->device));
-    for (auto& r : admitted) {
-      try {
-        setup(*r);
-        active.push_back(r);
-      } catch (const Error& e) {
-        r->rounds.clear();
-        complete(r, kFailed, e.code, e.what());
-        std::lock_guard<std::mutex> lk(mu);
-        in_use -= r->footprint;
-        free_slots.push_back(r->slot);
-      }
-    }
-    if (!active.empty()) {
-      const double t = now_s();
-      tick();
-      poll_and_finish();
+    const double t = now_s();
+    const bool p = poll_launches();
+    const bool l = launch_ready();
+    if (p || l) {
       std::lock_guard<std::mutex> lk(mu);
       stats.busy_seconds += now_s() - t;
     }
@@ -532,7 +611,7 @@ DPAH_EXPORT int dpah_server_create(dpah_context* ctx, const dpah_server_options*
     s->ctx = ctx;
     if (options) s->opt = *options;
     if (!s->opt.max_active) s->opt.max_active = 256;
-    if (!s->opt.steps_per_tick) s->opt.steps_per_tick = 256;
+    // steps_per_tick 0: every launch runs its particles to completion
     s->budget = s->opt.memory_budget_bytes ? s->opt.memory_budget_bytes : ctx->memory_budget / 2;
     {
       std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -541,6 +620,12 @@ DPAH_EXPORT int dpah_server_create(dpah_context* ctx, const dpah_server_options*
       s->d_steps = static_cast<unsigned long long*>(
           ctx->alloc(s->opt.max_active * sizeof(unsigned long long)));
       DPAH_HIP(hipMemsetAsync(s->d_left, 0, s->opt.max_active * sizeof(unsigned int), ctx->stream));
+      s->d_queues = static_cast<unsigned long long*>(
+          ctx->alloc(dpah_server::kStreams * sizeof(unsigned long long)));
+      for (int i = 0; i < dpah_server::kStreams; ++i) {
+        DPAH_HIP(hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking));
+        DPAH_HIP(hipEventCreateWithFlags(&s->launches[i].done, hipEventDisableTiming));
+      }
       ctx->sync();
     }
     for (int i = (int)s->opt.max_active - 1; i >= 0; --i) s->free_slots.push_back(i);
@@ -715,7 +800,13 @@ DPAH_EXPORT int dpah_server_destroy(dpah_server* s) {
     for (auto& kv : s->all)
       if (kv.second->result) delete kv.second->result;
     s->all.clear();
-    s->seg_buf.reset();
+    for (int i = 0; i < dpah_server::kStreams; ++i) {
+      if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
+      s->launches[i].segs.reset();
+      if (s->launches[i].done) (void)hipEventDestroy(s->launches[i].done);
+      if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
+    }
+    s->ctx->release(s->d_queues, dpah_server::kStreams * sizeof(unsigned long long));
     s->ctx->release(s->d_left, s->opt.max_active * sizeof(unsigned int));
     s->ctx->release(s->d_steps, s->opt.max_active * sizeof(unsigned long long));
     delete s;
