@@ -175,7 +175,7 @@ def run_cfg2(args):
     dt = np.float32(0.25)
     n = len(seeds)
 
-    ctx = dpa_amd.Context(0, kernel_timing=True)
+    ctx = make_context(args)
     fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
     d_seeds = ctx.to_device(seeds)
     d_pos = ctx.alloc(seeds.nbytes)
@@ -206,6 +206,9 @@ def run_cfg2(args):
     value = args.steps * steps_per_pass / elapsed
     log(f"timed {args.steps} passes in {elapsed:.3f} s: {value:.4e} particle-steps/s; "
         f"kernel avg {avg_ms:.3f} ms over {launches} launches")
+    ol, oms = ctx.kernel_timing("order_queue")
+    if ol:
+        log(f"queue ordering avg {oms / ol:.3f} ms over {ol} launches")
     for p in (d_seeds, d_pos, d_rem, d_st):
         ctx.free(p)
 
@@ -245,6 +248,18 @@ def run_cfg2(args):
     return out
 
 
+def make_context(args):
+    """Context with kernel timing and any launch-shape overrides from the command line."""
+    import dpa_amd
+    ctx = dpa_amd.Context(0, kernel_timing=True)
+    over = {k: v for k, v in (("order", args.order), ("blocks_per_cu", args.blocks_per_cu),
+                              ("block", args.block)) if v is not None}
+    if over:
+        ctx.set_tuning(**over)
+    log(f"tuning {ctx.tuning()}")
+    return ctx
+
+
 def base_line(args, value, elapsed, dtype):
     return {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup,
@@ -258,7 +273,7 @@ def run_cfg1(args):
     import dpa_amd
     from dpa_amd import workloads as W
     c = W.cfg1()
-    ctx = dpa_amd.Context(0, kernel_timing=True)
+    ctx = make_context(args)
     fld = ctx.field(c["field"], c["offset"], c["spacing"])
 
     def one():
@@ -309,7 +324,7 @@ def run_cfg3(args, dpa_mode=False):
     import dpa_amd
     from dpa_amd import workloads as W
     c = W.cfg3()
-    ctx = dpa_amd.Context(0, kernel_timing=True)
+    ctx = make_context(args)
     fld = ctx.field(c["field"], c["offset"], c["spacing"])
     shape = c["lattice_shape"]
     points = int(np.prod(shape))
@@ -354,7 +369,7 @@ def run_cfg4(args):
     from dpa_amd import workloads as W
     field = W.vortex_field(256)
     jobs = W.cfg4_jobs()
-    ctx = dpa_amd.Context(0, kernel_timing=True)
+    ctx = make_context(args)
     fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
     for j in jobs:
         j["field"] = fld
@@ -407,7 +422,7 @@ def run_cfg5(args):
     import dpa_amd
     from dpa_amd import workloads as W
     base = W.vortex_field(256)
-    ctx = dpa_amd.Context(0, kernel_timing=True)
+    ctx = make_context(args)
     fields = []
     for k in range(4):
         shifted = np.roll(base, shift=(37 * k, 71 * k, 13 * k), axis=(0, 1, 2))
@@ -448,6 +463,10 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=100.0)
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--rates", default="50,100,200,500")
+    ap.add_argument("--order", type=int, choices=[0, 1], default=None,
+                    help="spatially ordered, XCD-split work queue (default: library's)")
+    ap.add_argument("--blocks-per-cu", type=int, default=None)
+    ap.add_argument("--block", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

@@ -63,10 +63,16 @@ class _Request(C.Structure):
                 ("delta", C.c_float * 3), ("iterations", C.c_uint64)]
 
 
+class _Tuning(C.Structure):
+    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("order", C.c_int),
+                ("parts", C.c_int)]
+
+
 class _RequestInfo(C.Structure):
     _fields_ = [("state", C.c_int), ("queue_seconds", C.c_double),
                 ("latency_seconds", C.c_double), ("particle_steps", C.c_uint64),
-                ("footprint_bytes", C.c_uint64)]
+                ("footprint_bytes", C.c_uint64), ("launch_seconds", C.c_double),
+                ("device_seconds", C.c_double)]
 
 
 class _ServerStats(C.Structure):
@@ -137,6 +143,8 @@ def load_library() -> C.CDLL:
         "dpah_result_ftle": (C.c_int, [vp, vp, _u64p]),
         "dpah_kernel_timing": (C.c_int, [vp, C.c_char_p, _u64p, C.POINTER(C.c_double)]),
         "dpah_kernel_timing_reset": (C.c_int, [vp]),
+        "dpah_get_tuning": (C.c_int, [vp, C.POINTER(_Tuning)]),
+        "dpah_set_tuning": (C.c_int, [vp, C.POINTER(_Tuning)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -329,6 +337,17 @@ class Context:
     def kernel_timing_reset(self) -> None:
         _check(self.lib.dpah_kernel_timing_reset(self.h))
 
+    def tuning(self) -> dict:
+        t = _Tuning()
+        _check(self.lib.dpah_get_tuning(self.h, C.byref(t)))
+        return dict(block=t.block, blocks_per_cu=t.blocks_per_cu, order=t.order, parts=t.parts)
+
+    def set_tuning(self, **kw) -> None:
+        cur = self.tuning()
+        cur.update(kw)
+        t = _Tuning(cur["block"], cur["blocks_per_cu"], cur["order"], cur["parts"])
+        _check(self.lib.dpah_set_tuning(self.h, C.byref(t)))
+
 
 class Field:
     def __init__(self, ctx: Context, h, shape, nt):
@@ -459,7 +478,8 @@ class Server:
         _check(self.lib.dpah_server_request_info(self.h, ticket, C.byref(i)))
         return dict(state=i.state, queue_seconds=i.queue_seconds,
                     latency_seconds=i.latency_seconds, particle_steps=i.particle_steps,
-                    footprint_bytes=i.footprint_bytes)
+                    footprint_bytes=i.footprint_bytes, launch_seconds=i.launch_seconds,
+                    device_seconds=i.device_seconds)
 
     def cancel(self, ticket: int) -> None:
         _check(self.lib.dpah_server_cancel(self.h, ticket))

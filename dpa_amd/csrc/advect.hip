@@ -1,5 +1,9 @@
-// advect.hip -- launchers of the K_advect kernels (advect_kernels.hpp).
+// advect.hip -- launchers of the K_advect kernels (advect_kernels.hpp) and of
+// the queue ordering that precedes a persistent launch (order_kernels.hpp).
+#include <algorithm>
+
 #include "advect_kernels.hpp"
+#include "order_kernels.hpp"
 #include "runtime.hpp"
 
 namespace dpah {
@@ -40,8 +44,57 @@ void dispatch_integrator(dpah_context* ctx, const AdvectArgs& a, int integrator,
   }
 }
 
+// Clears the step counter and the queue heads (a kernel, not a memset, so the
+// sequence stays valid inside a captured graph).
 __global__ void k_reset_counters(unsigned long long* c) {
-  if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
+  const unsigned t = threadIdx.x;
+  if (t == 1 || (t >= (unsigned)kQueueHeads && t < (unsigned)(kQueueHeads + kMaxParts)))
+    c[t] = 0ull;
+}
+
+// Device scratch of one ordering; released when the launch is enqueued, which
+// is safe because the caching allocator hands blocks out again only in the
+// context stream's order.
+struct QueueOrder {
+  DeviceBuffer key, count, order;
+};
+
+void set_parts(AdvectArgs& a, uint32_t parts) {
+  a.parts = parts;
+  for (uint32_t p = 0; p <= parts; ++p) a.part_lo[p] = a.n / parts * p + std::min<uint64_t>(p, a.n % parts);
+}
+
+// Particles worth ordering: enough to fill the persistent grid several times,
+// and few enough for 32-bit positions.
+constexpr uint64_t kOrderMinParticles = 1ull << 15;
+
+void order_queue(dpah_context* ctx, AdvectArgs& a, QueueOrder& q) {
+  const int64_t D = std::max({(int64_t)a.f.n0, (int64_t)a.f.n1, (int64_t)a.f.n2});
+  uint32_t shift = 2;
+  while (((D + (1ll << shift) - 1) >> shift) > (1ll << kOrderMaxBits)) ++shift;
+  uint32_t bits = 1;
+  while ((1ll << (bits + shift)) < D) ++bits;
+  OrderArgs o{};
+  o.shift = shift;
+  o.bits = bits;
+  o.keys = (1u << (3 * bits)) + 1;
+  q.key = DeviceBuffer(ctx, a.n * sizeof(uint32_t));
+  q.order = DeviceBuffer(ctx, a.n * sizeof(uint32_t));
+  q.count = DeviceBuffer(ctx, (uint64_t)o.keys * sizeof(uint32_t));
+  o.key = q.key.as<uint32_t>();
+  o.order = q.order.as<uint32_t>();
+  o.count = q.count.as<uint32_t>();
+  const unsigned clear_grid = std::min<unsigned>((o.keys + 255) / 256, 1024u);
+  const unsigned grid = (unsigned)((a.n + 255) / 256);
+  ctx->timed("order_queue", [&] {
+    hipLaunchKernelGGL(k_order_clear, dim3(clear_grid), dim3(256), 0, ctx->stream, o);
+    hipLaunchKernelGGL(k_order_key, dim3(grid), dim3(256), 0, ctx->stream, a, o);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kOrderScanThreads), 0, ctx->stream, o);
+    hipLaunchKernelGGL(k_order_scatter, dim3(grid), dim3(256), 0, ctx->stream, a.n, o);
+  });
+  DPAH_HIP(hipGetLastError());
+  a.order = o.order;
+  set_parts(a, (uint32_t)std::max(1, std::min(ctx->tune.parts, kMaxParts)));
 }
 
 }  // namespace
@@ -66,12 +119,16 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   a.n = L.n;
   a.t0 = L.t0;
   a.coef = make_coef(L.dt);
-  a.queue = ctx->d_counters;
+  a.queue = ctx->d_counters + kQueueHeads;
   a.total_steps = ctx->d_counters + 1;
-  // queue head and step counter start at zero for every launch (a kernel, not
-  // a memset, so the sequence stays valid inside a captured graph)
+  a.order = nullptr;
+  set_parts(a, 1);
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
   DPAH_HIP(hipGetLastError());
+  QueueOrder q;
+  if (L.persistent && ctx->tune.order && ctx->caching && L.nseg == 0 &&
+      L.n >= kOrderMinParticles && L.n < (1ull << 32))
+    order_queue(ctx, a, q);
   const bool unsteady = L.f->nt > 1;
   if (L.slots) dispatch_integrator<true>(ctx, a, L.integrator, unsteady, L.persistent);
   else         dispatch_integrator<false>(ctx, a, L.integrator, unsteady, L.persistent);

@@ -59,8 +59,11 @@ struct AdvectArgs {
   uint64_t        n;
   float           t0;
   StepCoef        coef;
-  unsigned long long* queue;        // [0]: next particle to hand out (persistent form)
+  unsigned long long* queue;        // [parts]: next position to hand out in each part
   unsigned long long* total_steps;  // one atomic add per wave at exit
+  const uint32_t* order;            // position -> queue ordinal, or null (identity)
+  uint32_t        parts;            // queue parts (one per XCD), 1..8
+  uint64_t        part_lo[9];       // part p covers positions [part_lo[p], part_lo[p+1])
 };
 
 __device__ __forceinline__ bool is_sentinel(float a, float b, float c) {
@@ -225,7 +228,11 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
   add_wave_steps(a.total_steps, mine);
 }
 
-// Persistent form with the on-device work queue.
+// Persistent form with the on-device work queue.  The queue positions
+// [0, n) are split into `parts` contiguous parts, each with its own head;
+// workgroups are dispatched to the XCDs round-robin, so workgroup b starts on
+// part b % parts (its XCD's share of a spatially ordered queue) and moves on to
+// the following parts once its own is drained.
 template <bool kRecord, int kIntegrator, bool kUnsteady>
 __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   const uint32_t lane = threadIdx.x & 63;
@@ -234,27 +241,34 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   L.row = nullptr;
   L.pid = ~0ull;
   L.rem = 0; L.it = 0; L.p0 = L.p1 = L.p2 = 0.0f; L.special = false;
-  bool open = true;       // wave-uniform: the queue still has particles
+  uint32_t part = blockIdx.x % a.parts;   // wave-uniform
+  uint32_t parts_left = a.parts;          // parts this wave has not found drained
+  bool open = true;                       // wave-uniform: some part may have particles
   uint64_t mine = 0;
   while (true) {
     const bool idle = (L.pid == ~0ull);
     if (open) {
       const uint64_t need = __ballot(idle);
       if (need) {
+        const uint64_t lo = a.part_lo[part], hi = a.part_lo[part + 1];
         const uint32_t cnt = (uint32_t)__popcll(need);
         const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
         unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)cnt);
+        if (lane == leader) base = atomicAdd(a.queue + part, (unsigned long long)cnt);
         base = __shfl(base, (int)leader, 64);
         if (idle) {
-          const uint64_t cand = base + lane_rank(need);
-          if (cand < a.n) begin_particle<kRecord>(a, L, cand);
+          const uint64_t pos = lo + base + lane_rank(need);
+          if (pos < hi) begin_particle<kRecord>(a, L, a.order ? (uint64_t)a.order[pos] : pos);
         }
-        if (base + cnt >= a.n) open = false;
+        if (lo + base + cnt >= hi) {
+          part = part + 1 == a.parts ? 0 : part + 1;
+          if (--parts_left == 0) open = false;
+        }
       }
     }
     const bool busy = (L.pid != ~0ull);
-    if (!__any(busy)) break;
+    // a wave whose lanes all found a drained part keeps drawing from the next
+    if (!__any(busy) && !open) break;
     if (busy) {
       const uint8_t st = step_once<kRecord, kIntegrator, kUnsteady>(a, L);
       if (st != kActive) {

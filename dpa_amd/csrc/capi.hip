@@ -143,6 +143,11 @@ DPAH_EXPORT int dpah_create(int device, const dpah_options* options, dpah_contex
       hipDeviceProp_t prop;
       DPAH_HIP(hipGetDeviceProperties(&prop, device));
       ctx->cu_count = prop.multiProcessorCount;
+      int xcc = 1;
+      if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, device) != hipSuccess ||
+          xcc < 1)
+        xcc = 1;
+      ctx->tune.parts = std::min(xcc, dpah::kMaxParts);
       size_t free_b = 0, total_b = 0;
       DPAH_HIP(hipMemGetInfo(&free_b, &total_b));
       ctx->memory_budget = (options && options->memory_budget_bytes)
@@ -151,9 +156,12 @@ DPAH_EXPORT int dpah_create(int device, const dpah_options* options, dpah_contex
       ctx->slot_budget = (options && options->slot_budget_bytes) ? options->slot_budget_bytes
                                                                  : (4ull << 30);
       ctx->timing = options && options->kernel_timing;
+      ctx->arch = prop.gcnArchName;
+      if (options && options->autotune_cache_path)
+        ctx->autotune_path = options->autotune_cache_path;
       DPAH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-      ctx->d_counters = static_cast<unsigned long long*>(ctx->alloc(8 * sizeof(unsigned long long)));
-      DPAH_HIP(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+      ctx->d_counters = static_cast<unsigned long long*>(ctx->alloc(dpah::kCounterCount * sizeof(unsigned long long)));
+      DPAH_HIP(hipMemsetAsync(ctx->d_counters, 0, dpah::kCounterCount * sizeof(unsigned long long), ctx->stream));
       ctx->sync();
     } catch (...) {
       delete ctx;
@@ -171,7 +179,7 @@ DPAH_EXPORT int dpah_destroy(dpah_context* ctx) {
     ctx->resolve_timings();
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto& a : ctx->user_allocs) ctx->release(a.first, a.second);
-    if (ctx->d_counters) ctx->release(ctx->d_counters, 8 * sizeof(unsigned long long));
+    if (ctx->d_counters) ctx->release(ctx->d_counters, dpah::kCounterCount * sizeof(unsigned long long));
     ctx->trim_cache();
     for (auto& kv : ctx->block_size) (void)hipFree(kv.first);   // blocks never released
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -581,5 +589,32 @@ DPAH_EXPORT int dpah_kernel_timing_reset(dpah_context* ctx) {
     DPAH_HIP(hipSetDevice(ctx->device));
     ctx->sync();
     ctx->timings.clear();
+  });
+}
+
+DPAH_EXPORT int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && out, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    out->block = ctx->tune.block;
+    out->blocks_per_cu = ctx->tune.blocks_per_cu;
+    out->order = ctx->tune.order;
+    out->parts = ctx->tune.parts;
+  });
+}
+
+DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && t, "null argument");
+    DPAH_REQUIRE(t->block >= 64 && t->block <= 256 && t->block % 64 == 0,
+                 "block must be 64, 128, 192 or 256");
+    DPAH_REQUIRE(t->blocks_per_cu >= 1 && t->blocks_per_cu <= 32, "blocks_per_cu must be 1..32");
+    DPAH_REQUIRE(t->order == 0 || t->order == 1, "order must be 0 or 1");
+    DPAH_REQUIRE(t->parts >= 1 && t->parts <= dpah::kMaxParts, "parts must be 1..8");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->tune.block = t->block;
+    ctx->tune.blocks_per_cu = t->blocks_per_cu;
+    ctx->tune.order = t->order;
+    ctx->tune.parts = t->parts;
   });
 }

@@ -79,14 +79,17 @@ __device__ __forceinline__ V3 node(const FieldView& f, const float* base, int64_
 // A particle moves a fraction of a cell per step, so most steps sample the
 // same cell again and need no memory access at all; only lanes that cross
 // into a new cell reload.  The values are the grid's, so results are
-// unchanged.  c0 < 0 marks an empty cache.
+// unchanged.  c0 < 0 marks an empty cache; clearing sets every coordinate, so
+// the cell test never compares an indeterminate value.
 struct CellCache {
   int32_t c0, c1, c2, ct;
   V3 L0, L1, L2, L3, L4, L5, L6, L7;   // corners of slice ct (the only slice if steady)
   V3 M0, M1, M2, M3, M4, M5, M6, M7;   // corners of slice ct + 1 (time-variant fields)
 };
 
-__device__ __forceinline__ void cache_clear(CellCache& cc) { cc.c0 = -1; }
+__device__ __forceinline__ void cache_clear(CellCache& cc) {
+  cc.c0 = -1; cc.c1 = -1; cc.c2 = -1; cc.ct = -1;
+}
 
 // Trilinear sample of one time slice, corner order and lerp cascade of
 // regular_grid.hpp:59-74: list index 4a+2b+c, axis 2 first, then 1, then 0.

@@ -72,7 +72,15 @@ struct PendingTiming {
 struct Tuning {
   int block = 256;          // threads per block of the persistent advect kernel
   int blocks_per_cu = 8;    // persistent grid = CUs * blocks_per_cu
+  int order = 1;            // 1: spatially ordered queue split across the XCDs
+  int parts = 8;            // queue parts (XCDs on the device)
 };
+
+// Device counters of a context: [1] particle-steps, [2] FTLE misses,
+// [kQueueHeads, kQueueHeads + kMaxParts) the persistent queue's part heads.
+constexpr int kCounterCount = 16;
+constexpr int kQueueHeads = 8;
+constexpr int kMaxParts = 8;             // AdvectArgs::part_lo holds kMaxParts + 1
 
 }  // namespace dpah
 
@@ -93,12 +101,14 @@ struct dpah_context {
   std::map<void*, uint64_t> block_size;       // every live or cached block
   uint64_t cached_bytes = 0;
   void trim_cache();
-  unsigned long long* d_counters = nullptr;   // [0] queue head, [1] steps, [2..7] spare
+  unsigned long long* d_counters = nullptr;   // kCounterCount counters (see Tuning)
   bool timing = false;
   std::map<std::string, dpah::TimedKernel> timings;
   std::vector<dpah::PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
   dpah::Tuning tune;
+  std::string arch;              // gcnArchName
+  std::string autotune_path;     // tuning cache file, empty: none
   std::recursive_mutex mu;
 
   void* alloc(uint64_t bytes);

@@ -53,6 +53,7 @@ struct Request {
   std::string message;
   bool cancel = false;
   double t_submit = 0, t_admit = 0, t_done = 0;
+  double t_launch = 0, t_device = 0;   // first launch enqueued, advection seen done
   uint64_t steps = 0;
   // device state while running
   int slot = -1;
@@ -364,6 +365,7 @@ bool dpah_server::launch_ready() {
         continue;
       }
       live.push_back(rp);
+      if (r.t_launch == 0) r.t_launch = now_s();
       for (RoundState& rs : r.rounds) {
         if (!rs.count) continue;
         TickSeg t{};
@@ -424,8 +426,10 @@ bool dpah_server::poll_launches() {
                               hipMemcpyDeviceToHost, s));
     DPAH_HIP(hipStreamSynchronize(s));
     ctx->resolve_timings();
+    const double t_seen = now_s();
     for (auto& rp : L.reqs) {
       Request& r = *rp;
+      r.t_device = t_seen;
       bool cancelled;
       {
         std::lock_guard<std::mutex> lk(mu);
@@ -565,6 +569,7 @@ void dpah_server::run_per_request() {
       }
       r->state = kRunning;
       r->t_admit = now_s();
+      r->t_launch = r->t_admit;
     }
     const double t = now_s();
     try {
@@ -587,6 +592,7 @@ void dpah_server::run_per_request() {
                                &r->spec.params, false);
       }
       r->steps = r->result->total_steps;
+      r->t_device = now_s();
       complete(r, kDone, DPAH_OK, "");
     } catch (const Error& e) {
       complete(r, kFailed, e.code, e.what());
@@ -742,6 +748,8 @@ DPAH_EXPORT int dpah_server_request_info(dpah_server* s, uint64_t ticket, dpah_r
     info->latency_seconds = r.t_done > 0 ? r.t_done - r.t_submit : 0.0;
     info->particle_steps = r.steps;
     info->footprint_bytes = r.footprint;
+    info->launch_seconds = r.t_launch > 0 ? r.t_launch - r.t_submit : 0.0;
+    info->device_seconds = r.t_device > 0 ? r.t_device - r.t_submit : 0.0;
   });
 }
 

@@ -46,6 +46,47 @@ __device__ __forceinline__ uint32_t find_seg(const TickArgs& a, uint64_t q) {
   return lo;
 }
 
+// Per-lane tallies of one segment's request counters (particle-steps and
+// retired particles), flushed when the lane moves to another segment and,
+// grouped by segment across the wave, when the kernel ends: a few atomics per
+// wave instead of two per particle on the same two addresses.
+struct Tally {
+  uint32_t seg;              // ~0u: nothing held
+  uint32_t retired;
+  uint64_t steps;
+};
+
+__device__ __forceinline__ void tally_flush_lane(const TickArgs& a, Tally& t) {
+  if (t.seg == ~0u) return;
+  const TickSeg& s = a.segs[t.seg];
+  if (t.steps) atomicAdd(s.steps, (unsigned long long)t.steps);
+  if (t.retired) atomicSub(s.active_left, t.retired);
+  t.seg = ~0u; t.steps = 0; t.retired = 0;
+}
+
+__device__ __forceinline__ void tally_flush_wave(const TickArgs& a, Tally& t) {
+  const uint32_t lane = threadIdx.x & 63;
+  while (true) {
+    const uint64_t held = __ballot(t.seg != ~0u);
+    if (!held) break;
+    const int leader = __ffsll((unsigned long long)held) - 1;
+    const uint32_t seg = __shfl(t.seg, leader, 64);
+    const bool mine = t.seg == seg;
+    uint64_t st = mine ? t.steps : 0;
+    uint32_t rt = mine ? t.retired : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+      st += __shfl_xor(st, off, 64);
+      rt += __shfl_xor(rt, off, 64);
+    }
+    if ((int)lane == leader) {
+      const TickSeg& s = a.segs[seg];
+      if (st) atomicAdd(s.steps, (unsigned long long)st);
+      if (rt) atomicSub(s.active_left, rt);
+    }
+    if (mine) { t.seg = ~0u; t.steps = 0; t.retired = 0; }
+  }
+}
+
 template <bool kRecord, int kIntegrator, bool kUnsteady>
 static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
   const uint32_t lane = threadIdx.x & 63;
@@ -58,6 +99,7 @@ static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
   StepCoef coef = make_coef(0.0f);
   float t0 = 0.0f;
   uint32_t it_begin = 0;
+  Tally tally{~0u, 0u, 0ull};
   bool open = true;
   while (true) {
     const bool idle = (L.pid == ~0ull);
@@ -96,7 +138,11 @@ static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
       }
     }
     const bool busy = (L.pid != ~0ull);
-    if (!__any(busy)) break;
+    // lanes that drew only retired particles draw again while the queue is open
+    if (!__any(busy) && !open) {
+      tally_flush_wave(a, tally);
+      break;
+    }
     if (busy) {
       const uint8_t st = step_lane<kRecord, kIntegrator, kUnsteady>(a.f, coef, t0, L);
       if (st == kActive) --left;              // budget >= 1 (checked on the host)
@@ -106,14 +152,18 @@ static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
         s.pos[3 * i] = L.p0; s.pos[3 * i + 1] = L.p1; s.pos[3 * i + 2] = L.p2;
         s.rem[i] = L.rem;
         s.it[i] = L.it;
-        atomicAdd(s.steps, (unsigned long long)(L.it - it_begin));
+        if (tally.seg != seg) {
+          tally_flush_lane(a, tally);
+          tally.seg = seg;
+        }
+        tally.steps += L.it - it_begin;
         if (st != kActive) {
           if constexpr (kRecord) {
             put3(L.row + 3 * ((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
             s.special[i] = L.special ? 1 : 0;
           }
           s.status[i] = st;
-          atomicSub(s.active_left, 1u);
+          tally.retired += 1;
         } else if constexpr (kRecord) {
           s.special[i] = L.special ? 1 : 0;
         }

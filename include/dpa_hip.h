@@ -255,6 +255,8 @@ typedef struct dpah_request_info {
   double   latency_seconds;      /* submit -> result ready (on device)    */
   uint64_t particle_steps;
   uint64_t footprint_bytes;
+  double   launch_seconds;       /* submit -> first kernel launch enqueued */
+  double   device_seconds;       /* submit -> last advection kernel seen done */
 } dpah_request_info;
 
 typedef struct dpah_server_stats {
@@ -286,6 +288,18 @@ int dpah_result_ftle(const dpah_result* r, float* out, uint64_t* count);
  * kernel ("advect_persistent", "advect_simple", "compact_write", ...). */
 int dpah_kernel_timing(dpah_context* ctx, const char* kernel, uint64_t* launches, double* total_ms);
 int dpah_kernel_timing_reset(dpah_context* ctx);
+
+/* Launch shape of the persistent advect kernel and the order of its work
+ * queue.  No counterpart in dpa, whose CPU loop has no launch shape;
+ * dpah_autotune_endpoints picks these by timing a sample of a workload. */
+typedef struct dpah_tuning {
+  int block;           /* threads per workgroup: 64, 128, 192 or 256          */
+  int blocks_per_cu;   /* persistent grid = CUs x blocks_per_cu, 1..32         */
+  int order;           /* 1: queue in brick Morton order, split across XCDs    */
+  int parts;           /* queue parts, 1..8 (the XCD count by default)         */
+} dpah_tuning;
+int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
+int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* tuning);
 
 #ifdef __cplusplus
 }
