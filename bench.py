@@ -144,17 +144,24 @@ def serve_trace(ctx, fields, trace, mode, steps_per_tick=0):
         steps += results[i]["particle_steps"]
     lat = np.array(lat)
     span = t_end - t0
+
+    def p50(key):
+        return float(np.percentile([results[i][key] for i in range(len(trace))], 50) * 1e3)
+
     return {"requests": len(trace), "particle_steps": int(steps),
             "throughput": steps / span, "makespan_s": span,
             "p50_ms": float(np.percentile(lat, 50) * 1e3),
-            "p99_ms": float(np.percentile(lat, 99) * 1e3)}
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "p50_admitted_ms": p50("queue_seconds"), "p50_launched_ms": p50("launch_seconds"),
+            "p50_device_done_ms": p50("device_seconds")}
 
 
 def serving_leg(ctx, fields, trace, steps_per_tick=0):
     import dpa_amd
     b = serve_trace(ctx, fields, trace, dpa_amd.SERVE_BATCHED, steps_per_tick)
     log(f"served batched: {b['throughput']:.3e} steps/s p50 {b['p50_ms']:.1f} ms "
-        f"p99 {b['p99_ms']:.1f} ms")
+        f"p99 {b['p99_ms']:.1f} ms (p50 admitted {b['p50_admitted_ms']:.1f}, launched "
+        f"{b['p50_launched_ms']:.1f}, device done {b['p50_device_done_ms']:.1f} ms)")
     p = serve_trace(ctx, fields, trace, dpa_amd.SERVE_PER_REQUEST)
     log(f"per-request:    {p['throughput']:.3e} steps/s p50 {p['p50_ms']:.1f} ms "
         f"p99 {p['p99_ms']:.1f} ms")
@@ -253,7 +260,7 @@ def make_context(args):
     import dpa_amd
     ctx = dpa_amd.Context(0, kernel_timing=True)
     over = {k: v for k, v in (("order", args.order), ("blocks_per_cu", args.blocks_per_cu),
-                              ("block", args.block)) if v is not None}
+                              ("block", args.block), ("refill", args.refill)) if v is not None}
     if over:
         ctx.set_tuning(**over)
     log(f"tuning {ctx.tuning()}")
@@ -467,6 +474,7 @@ def main() -> int:
                     help="spatially ordered, XCD-split work queue (default: library's)")
     ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
+    ap.add_argument("--refill", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

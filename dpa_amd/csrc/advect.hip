@@ -80,7 +80,8 @@ void order_queue(dpah_context* ctx, AdvectArgs& a, QueueOrder& q) {
   o.keys = (1u << (3 * bits)) + 1;
   q.key = DeviceBuffer(ctx, a.n * sizeof(uint32_t));
   q.order = DeviceBuffer(ctx, a.n * sizeof(uint32_t));
-  q.count = DeviceBuffer(ctx, (uint64_t)o.keys * sizeof(uint32_t));
+  const uint32_t tiles = (o.keys + kOrderTile - 1) / kOrderTile;   // <= 129 for 6 bits
+  q.count = DeviceBuffer(ctx, ((uint64_t)o.keys + tiles) * sizeof(uint32_t));
   o.key = q.key.as<uint32_t>();
   o.order = q.order.as<uint32_t>();
   o.count = q.count.as<uint32_t>();
@@ -89,7 +90,10 @@ void order_queue(dpah_context* ctx, AdvectArgs& a, QueueOrder& q) {
   ctx->timed("order_queue", [&] {
     hipLaunchKernelGGL(k_order_clear, dim3(clear_grid), dim3(256), 0, ctx->stream, o);
     hipLaunchKernelGGL(k_order_key, dim3(grid), dim3(256), 0, ctx->stream, a, o);
-    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kOrderScanThreads), 0, ctx->stream, o);
+    uint32_t* sums = o.count + o.keys;
+    hipLaunchKernelGGL(k_order_tile_sums, dim3(tiles), dim3(256), 0, ctx->stream, o, sums);
+    hipLaunchKernelGGL(k_order_scan_sums, dim3(1), dim3(256), 0, ctx->stream, sums, tiles);
+    hipLaunchKernelGGL(k_order_scan_apply, dim3(tiles), dim3(256), 0, ctx->stream, o, sums);
     hipLaunchKernelGGL(k_order_scatter, dim3(grid), dim3(256), 0, ctx->stream, a.n, o);
   });
   DPAH_HIP(hipGetLastError());
@@ -123,6 +127,7 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   a.total_steps = ctx->d_counters + 1;
   a.order = nullptr;
   set_parts(a, 1);
+  a.refill = (uint32_t)ctx->tune.refill;
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;

@@ -64,6 +64,7 @@ struct AdvectArgs {
   const uint32_t* order;            // position -> queue ordinal, or null (identity)
   uint32_t        parts;            // queue parts (one per XCD), 1..8
   uint64_t        part_lo[9];       // part p covers positions [part_lo[p], part_lo[p+1])
+  uint32_t        refill;           // a wave draws new particles once this many lanes idle
 };
 
 __device__ __forceinline__ bool is_sentinel(float a, float b, float c) {
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
     const bool idle = (L.pid == ~0ull);
     if (open) {
       const uint64_t need = __ballot(idle);
-      if (need) {
+      if (need && (uint32_t)__popcll(need) >= a.refill) {
         const uint64_t lo = a.part_lo[part], hi = a.part_lo[part + 1];
         const uint32_t cnt = (uint32_t)__popcll(need);
         const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
@@ -258,7 +259,8 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
         base = __shfl(base, (int)leader, 64);
         if (idle) {
           const uint64_t pos = lo + base + lane_rank(need);
-          if (pos < hi) begin_particle<kRecord>(a, L, a.order ? (uint64_t)a.order[pos] : pos);
+          const uint64_t q = pos < hi ? (a.order ? (uint64_t)a.order[pos] : pos) : a.n;
+          if (q < a.n) begin_particle<kRecord>(a, L, q);   // q < n also guards the order table
         }
         if (lo + base + cnt >= hi) {
           part = part + 1 == a.parts ? 0 : part + 1;

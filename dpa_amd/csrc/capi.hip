@@ -70,6 +70,7 @@ void* dpah_context::alloc(uint64_t bytes) {
     throw Error(DPAH_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(want) + "): " +
                                     hipGetErrorString(e));
   block_size[p] = want;
+  if (!caching) direct.insert(p);
   bytes_in_use += want;
   return p;
 }
@@ -80,12 +81,16 @@ void dpah_context::release(void* p, uint64_t /*bytes*/) {
   if (it == block_size.end()) return;
   const uint64_t sz = it->second;
   bytes_in_use -= sz;
-  if (caching) {
+  auto d = direct.find(p);
+  if (caching && d == direct.end()) {
     cached.emplace(sz, p);
     cached_bytes += sz;
   } else {
+    // blocks taken while caching was off go back to the runtime whenever they
+    // are released, so the per-request form never feeds the cache
     (void)hipFree(p);
     block_size.erase(it);
+    if (d != direct.end()) direct.erase(d);
   }
 }
 
@@ -600,6 +605,7 @@ DPAH_EXPORT int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out) {
     out->blocks_per_cu = ctx->tune.blocks_per_cu;
     out->order = ctx->tune.order;
     out->parts = ctx->tune.parts;
+    out->refill = ctx->tune.refill;
   });
 }
 
@@ -611,10 +617,12 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(t->blocks_per_cu >= 1 && t->blocks_per_cu <= 32, "blocks_per_cu must be 1..32");
     DPAH_REQUIRE(t->order == 0 || t->order == 1, "order must be 0 or 1");
     DPAH_REQUIRE(t->parts >= 1 && t->parts <= dpah::kMaxParts, "parts must be 1..8");
+    DPAH_REQUIRE(t->refill >= 1 && t->refill <= 64, "refill must be 1..64");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     ctx->tune.block = t->block;
     ctx->tune.blocks_per_cu = t->blocks_per_cu;
     ctx->tune.order = t->order;
     ctx->tune.parts = t->parts;
+    ctx->tune.refill = t->refill;
   });
 }

@@ -2,7 +2,7 @@
 //
 // The queue hands particles out in the Morton order of the brick (a cube of
 // 2^shift cells per axis) their start position lies in: a counting sort of
-// brick keys (key + histogram, one-block scan, scatter).  The persistent
+// brick keys (key + histogram, three-pass scan, scatter).  The persistent
 // kernel splits the ordered queue into one contiguous part per XCD and a
 // workgroup drains its own XCD's part before it takes from the others.  The
 // 64 lanes of a wave then start in a few neighbouring bricks, and the particles
@@ -16,7 +16,6 @@
 namespace dpah {
 
 constexpr uint32_t kOrderMaxBits = 6;      // brick coordinate bits per axis (2^18 keys)
-constexpr uint32_t kOrderScanThreads = 1024;
 
 __device__ __forceinline__ uint32_t spread_bits3(uint32_t v) {   // bit i -> bit 3i (10 bits)
   v &= 0x3ffu;
@@ -67,28 +66,68 @@ static __global__ __launch_bounds__(256) void k_order_key(AdvectArgs a, OrderArg
   atomicAdd(o.count + k, 1u);
 }
 
-// Exclusive scan of the histogram by one block: each thread sums a run of
-// keys, the run totals are scanned in LDS, then each thread writes its run.
-static __global__ __launch_bounds__(kOrderScanThreads) void k_order_scan(OrderArgs o) {
-  __shared__ uint32_t part[kOrderScanThreads];
+// Exclusive scan of the histogram in three passes: tile totals, a one-block
+// scan of the totals, then each tile scanned with its offset.
+constexpr uint32_t kOrderScanItems = 8;
+constexpr uint32_t kOrderTile = 256 * kOrderScanItems;
+
+// Exclusive scan of one value per thread over a 256-thread block; returns the
+// block total through `total`.
+__device__ __forceinline__ uint32_t block_exclusive_scan_u32(uint32_t v, uint32_t& total) {
+  __shared__ uint32_t sh[256];
   const uint32_t t = threadIdx.x;
-  const uint32_t per = (o.keys + kOrderScanThreads - 1) / kOrderScanThreads;
-  const uint32_t lo = min(o.keys, t * per), hi = min(o.keys, lo + per);
-  uint32_t s = 0;
-  for (uint32_t k = lo; k < hi; ++k) s += o.count[k];
-  part[t] = s;
+  sh[t] = v;
   __syncthreads();
-  for (uint32_t off = 1; off < kOrderScanThreads; off <<= 1) {
-    const uint32_t v = t >= off ? part[t - off] : 0u;
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t x = t >= off ? sh[t - off] : 0u;
     __syncthreads();
-    part[t] += v;
+    sh[t] += x;
     __syncthreads();
   }
-  uint32_t run = part[t] - s;
-  for (uint32_t k = lo; k < hi; ++k) {
-    const uint32_t c = o.count[k];
-    o.count[k] = run;
-    run += c;
+  total = sh[255];
+  const uint32_t incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+static __global__ __launch_bounds__(256) void k_order_tile_sums(OrderArgs o, uint32_t* sums) {
+  const uint32_t base = blockIdx.x * kOrderTile + threadIdx.x * kOrderScanItems;
+  uint32_t s = 0;
+  for (uint32_t i = 0; i < kOrderScanItems; ++i)
+    if (base + i < o.keys) s += o.count[base + i];
+  uint32_t total;
+  block_exclusive_scan_u32(s, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One block over the tile totals (tiles <= 256 * kOrderScanItems).
+static __global__ __launch_bounds__(256) void k_order_scan_sums(uint32_t* sums, uint32_t tiles) {
+  const uint32_t base = threadIdx.x * kOrderScanItems;
+  uint32_t v[kOrderScanItems], s = 0;
+  for (uint32_t i = 0; i < kOrderScanItems; ++i) {
+    v[i] = base + i < tiles ? sums[base + i] : 0u;
+    s += v[i];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan_u32(s, total);
+  for (uint32_t i = 0; i < kOrderScanItems; ++i) {
+    if (base + i < tiles) sums[base + i] = run;
+    run += v[i];
+  }
+}
+
+static __global__ __launch_bounds__(256) void k_order_scan_apply(OrderArgs o, const uint32_t* sums) {
+  const uint32_t base = blockIdx.x * kOrderTile + threadIdx.x * kOrderScanItems;
+  uint32_t v[kOrderScanItems], s = 0;
+  for (uint32_t i = 0; i < kOrderScanItems; ++i) {
+    v[i] = base + i < o.keys ? o.count[base + i] : 0u;
+    s += v[i];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan_u32(s, total) + sums[blockIdx.x];
+  for (uint32_t i = 0; i < kOrderScanItems; ++i) {
+    if (base + i < o.keys) o.count[base + i] = run;
+    run += v[i];
   }
 }
 
@@ -98,7 +137,7 @@ static __global__ __launch_bounds__(256) void k_order_scatter(uint64_t n, OrderA
   const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const uint32_t slot = atomicAdd(o.count + o.key[q], 1u);
-  o.order[slot] = (uint32_t)q;
+  if (slot < n) o.order[slot] = (uint32_t)q;   // always true for a correct scan
 }
 
 }  // namespace dpah

@@ -6,6 +6,7 @@
 
 #include <atomic>
 #include <map>
+#include <set>
 #include <new>
 #include <mutex>
 #include <stdexcept>
@@ -74,6 +75,7 @@ struct Tuning {
   int blocks_per_cu = 8;    // persistent grid = CUs * blocks_per_cu
   int order = 1;            // 1: spatially ordered queue split across the XCDs
   int parts = 8;            // queue parts (XCDs on the device)
+  int refill = 64;          // idle lanes that make a wave draw new particles (1..64)
 };
 
 // Device counters of a context: [1] particle-steps, [2] FTLE misses,
@@ -99,6 +101,7 @@ struct dpah_context {
   bool caching = true;
   std::multimap<uint64_t, void*> cached;      // size -> block
   std::map<void*, uint64_t> block_size;       // every live or cached block
+  std::set<void*> direct;                     // live blocks taken while caching was off
   uint64_t cached_bytes = 0;
   void trim_cache();
   unsigned long long* d_counters = nullptr;   // kCounterCount counters (see Tuning)

@@ -72,25 +72,18 @@ uint64_t ftle_points(const dpah_request& r) {
 
 template <bool kRecord, int kIntegrator, bool kUnsteady>
 void launch_tick_t(dpah_context* ctx, const TickArgs& a, hipStream_t s) {
-  const int block = 256;
-  uint64_t grid = (uint64_t)ctx->cu_count * ctx->tune.blocks_per_cu;
-  const uint64_t need = (a.n + block - 1) / block;
-  if (grid > need) grid = need;
+  const uint64_t grid = (a.n + 255) / 256;
+  DPAH_REQUIRE(grid < (1ull << 31), "too many particles in one launch");
   ctx->timed("advect_tick", s, [&] {
     hipLaunchKernelGGL((k_advect_tick<kRecord, kIntegrator, kUnsteady>), dim3((unsigned)grid),
-                       dim3(block), 0, s, a);
+                       dim3(256), 0, s, a);
   });
   DPAH_HIP(hipGetLastError());
-}
-
-__global__ void k_reset_queue(unsigned long long* q) {
-  if (threadIdx.x == 0) *q = 0ull;
 }
 
 void launch_tick(dpah_context* ctx, const TickArgs& a, int integrator, bool record,
                  hipStream_t s) {
   if (a.n == 0) return;
-  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, a.queue);
   const bool unsteady = a.f.nt > 1;
   if (record) {
     if (integrator == kRk4) {
@@ -135,7 +128,6 @@ struct dpah_server {
   unsigned long long* d_steps = nullptr;
   std::vector<int> free_slots;
   hipStream_t streams[kStreams] = {};
-  unsigned long long* d_queues = nullptr;         // one queue head per stream
   Launch launches[kStreams];
   dpah_server_stats stats{};
 
@@ -386,8 +378,8 @@ bool dpah_server::launch_ready() {
     }
     L.reqs.swap(live);
     if (L.reqs.empty()) continue;
-    uint64_t q = 0;
-    for (auto& t : segs) { t.qbase = q; q += t.count; }
+    uint64_t q = 0;   // segments start at multiples of 64 (wave-uniform in the kernel)
+    for (auto& t : segs) { t.qbase = q; q += (t.count + 63) / 64 * 64; }
     L.segs = DeviceBuffer(ctx, std::max<size_t>(segs.size(), 1) * sizeof(TickSeg));
     if (!segs.empty())
       DPAH_HIP(hipMemcpyAsync(L.segs.ptr, segs.data(), segs.size() * sizeof(TickSeg),
@@ -398,7 +390,6 @@ bool dpah_server::launch_ready() {
     a.nseg = (uint32_t)segs.size();
     a.n = q;
     a.budget = opt.steps_per_tick ? opt.steps_per_tick : 0xffffffffu;
-    a.queue = d_queues + sid;
     launch_tick(ctx, a, std::get<1>(key), rec0, s);
     DPAH_HIP(hipEventRecord(L.done, s));
     L.busy = true;
@@ -626,8 +617,6 @@ DPAH_EXPORT int dpah_server_create(dpah_context* ctx, const dpah_server_options*
       s->d_steps = static_cast<unsigned long long*>(
           ctx->alloc(s->opt.max_active * sizeof(unsigned long long)));
       DPAH_HIP(hipMemsetAsync(s->d_left, 0, s->opt.max_active * sizeof(unsigned int), ctx->stream));
-      s->d_queues = static_cast<unsigned long long*>(
-          ctx->alloc(dpah_server::kStreams * sizeof(unsigned long long)));
       for (int i = 0; i < dpah_server::kStreams; ++i) {
         DPAH_HIP(hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking));
         DPAH_HIP(hipEventCreateWithFlags(&s->launches[i].done, hipEventDisableTiming));
@@ -814,7 +803,6 @@ DPAH_EXPORT int dpah_server_destroy(dpah_server* s) {
       if (s->launches[i].done) (void)hipEventDestroy(s->launches[i].done);
       if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
     }
-    s->ctx->release(s->d_queues, dpah_server::kStreams * sizeof(unsigned long long));
     s->ctx->release(s->d_left, s->opt.max_active * sizeof(unsigned int));
     s->ctx->release(s->d_steps, s->opt.max_active * sizeof(unsigned long long));
     delete s;

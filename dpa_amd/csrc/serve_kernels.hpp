@@ -3,8 +3,8 @@
 // Continuous batching: every admitted request owns per-particle state in HBM
 // (position, remaining, iteration, status, record slots).  One tick launch
 // advances the particles of ALL admitted requests that share a field and an
-// integrator by at most `budget` steps each, through the same on-device queue
-// as the persistent kernel; requests join and leave between ticks.  A particle
+// integrator by at most `budget` steps each; requests join and leave between
+// ticks.  A particle
 // advanced in several ticks goes through exactly the operations it would go
 // through in one launch, so results do not depend on the mix.
 #pragma once
@@ -14,7 +14,7 @@
 namespace dpah {
 
 struct TickSeg {
-  uint64_t qbase;           // first queue ordinal of this segment in the tick
+  uint64_t qbase;           // first ordinal of this segment in the tick, a multiple of 64
   uint64_t count;           // particles
   float* pos;               // [count][3] state (seed before the first tick)
   uint64_t* rem;            // [count]
@@ -32,9 +32,8 @@ struct TickArgs {
   FieldView f;
   const TickSeg* segs;
   uint32_t nseg;
-  uint64_t n;               // total ordinals
+  uint64_t n;               // total ordinals (every segment padded to a multiple of 64)
   uint32_t budget;          // steps per particle per tick
-  unsigned long long* queue;
 };
 
 __device__ __forceinline__ uint32_t find_seg(const TickArgs& a, uint64_t q) {
@@ -47,22 +46,13 @@ __device__ __forceinline__ uint32_t find_seg(const TickArgs& a, uint64_t q) {
 }
 
 // Per-lane tallies of one segment's request counters (particle-steps and
-// retired particles), flushed when the lane moves to another segment and,
-// grouped by segment across the wave, when the kernel ends: a few atomics per
-// wave instead of two per particle on the same two addresses.
+// retired particles), summed per segment across the wave when the kernel ends:
+// a few atomics per wave instead of two per particle on the same two addresses.
 struct Tally {
   uint32_t seg;              // ~0u: nothing held
   uint32_t retired;
   uint64_t steps;
 };
-
-__device__ __forceinline__ void tally_flush_lane(const TickArgs& a, Tally& t) {
-  if (t.seg == ~0u) return;
-  const TickSeg& s = a.segs[t.seg];
-  if (t.steps) atomicAdd(s.steps, (unsigned long long)t.steps);
-  if (t.retired) atomicSub(s.active_left, t.retired);
-  t.seg = ~0u; t.steps = 0; t.retired = 0;
-}
 
 __device__ __forceinline__ void tally_flush_wave(const TickArgs& a, Tally& t) {
   const uint32_t lane = threadIdx.x & 63;
@@ -87,90 +77,56 @@ __device__ __forceinline__ void tally_flush_wave(const TickArgs& a, Tally& t) {
   }
 }
 
+// One thread per ordinal of the launch: the particle runs until it retires or
+// has taken `budget` steps, then its state goes back to HBM.  The hardware
+// dispatches workgroups in ordinal order, so the 64 lanes of a wave hold 64
+// consecutive particles of one request (spatially coherent for lattice seeds)
+// until all of them are done.  Segments start at multiples of 64, so a wave
+// never straddles two: the segment is wave-uniform and its pointers live in
+// scalar registers, which keeps occupancy at the per-request kernel's level.
 template <bool kRecord, int kIntegrator, bool kUnsteady>
 static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
-  const uint32_t lane = threadIdx.x & 63;
-  Lane L;
-  cache_clear(L.cc);
-  L.row = nullptr;
-  L.pid = ~0ull;
-  L.rem = 0; L.it = 0; L.p0 = L.p1 = L.p2 = 0.0f; L.special = false;
-  uint32_t seg = 0, left = 0;
-  StepCoef coef = make_coef(0.0f);
-  float t0 = 0.0f;
-  uint32_t it_begin = 0;
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Tally tally{~0u, 0u, 0ull};
-  bool open = true;
-  while (true) {
-    const bool idle = (L.pid == ~0ull);
-    if (open) {
-      const uint64_t need = __ballot(idle);
-      if (need) {
-        const uint32_t cnt = (uint32_t)__popcll(need);
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)cnt);
-        base = __shfl(base, (int)leader, 64);
-        if (idle) {
-          const uint64_t q = base + lane_rank(need);
-          if (q < a.n) {
-            seg = find_seg(a, q);
-            const TickSeg& s = a.segs[seg];
-            const uint64_t i = q - s.qbase;
-            if (s.status[i] == kActive) {
-              L.pid = i;
-              cache_clear(L.cc);
-              L.p0 = s.pos[3 * i]; L.p1 = s.pos[3 * i + 1]; L.p2 = s.pos[3 * i + 2];
-              L.rem = s.rem[i];
-              L.it = s.it[i];
-              if constexpr (kRecord) {
-                L.row = s.slots + 3 * i * s.stride;
-                L.special = s.special[i] != 0;
-              }
-              coef = make_coef(s.dt);
-              t0 = s.t0;
-              left = a.budget;
-              it_begin = L.it;
-            }
-          }
-        }
-        if (base + cnt >= a.n) open = false;
+  if (q < a.n) {   // a.n is a multiple of 64: whole waves pass or fail together
+    const uint32_t seg = __builtin_amdgcn_readfirstlane(find_seg(a, q));
+    const TickSeg& s = a.segs[seg];
+    const uint64_t i = q - s.qbase;
+    if (i < s.count && s.status[i] == kActive) {
+      Lane L;
+      cache_clear(L.cc);
+      L.pid = i;
+      L.p0 = s.pos[3 * i]; L.p1 = s.pos[3 * i + 1]; L.p2 = s.pos[3 * i + 2];
+      L.rem = s.rem[i];
+      L.it = s.it[i];
+      L.row = nullptr;
+      L.special = false;
+      if constexpr (kRecord) {
+        L.row = s.slots + 3 * i * s.stride;
+        L.special = s.special[i] != 0;
       }
-    }
-    const bool busy = (L.pid != ~0ull);
-    // lanes that drew only retired particles draw again while the queue is open
-    if (!__any(busy) && !open) {
-      tally_flush_wave(a, tally);
-      break;
-    }
-    if (busy) {
-      const uint8_t st = step_lane<kRecord, kIntegrator, kUnsteady>(a.f, coef, t0, L);
-      if (st == kActive) --left;              // budget >= 1 (checked on the host)
-      if (st != kActive || left == 0) {
-        const TickSeg& s = a.segs[seg];
-        const uint64_t i = L.pid;
-        s.pos[3 * i] = L.p0; s.pos[3 * i + 1] = L.p1; s.pos[3 * i + 2] = L.p2;
-        s.rem[i] = L.rem;
-        s.it[i] = L.it;
-        if (tally.seg != seg) {
-          tally_flush_lane(a, tally);
-          tally.seg = seg;
-        }
-        tally.steps += L.it - it_begin;
-        if (st != kActive) {
-          if constexpr (kRecord) {
-            put3(L.row + 3 * ((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
-            s.special[i] = L.special ? 1 : 0;
-          }
-          s.status[i] = st;
-          tally.retired += 1;
-        } else if constexpr (kRecord) {
-          s.special[i] = L.special ? 1 : 0;
-        }
-        L.pid = ~0ull;
+      const StepCoef coef = make_coef(s.dt);
+      // budget >= 1 (checked on the host); the stop index wraps modulo 2^32,
+      // and remaining < 2^32 - 2 retires the particle before a wrap matters
+      const uint32_t stop = L.it + a.budget;
+      uint8_t st;
+      do {
+        st = step_lane<kRecord, kIntegrator, kUnsteady>(a.f, coef, s.t0, L);
+      } while (st == kActive && L.it != stop);
+      s.pos[3 * i] = L.p0; s.pos[3 * i + 1] = L.p1; s.pos[3 * i + 2] = L.p2;
+      s.rem[i] = L.rem;
+      tally.steps = L.it - (stop - a.budget);
+      s.it[i] = L.it;
+      tally.seg = seg;
+      if (st != kActive) {
+        if constexpr (kRecord) put3(L.row + 3 * ((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
+        s.status[i] = st;
+        tally.retired = 1;
       }
+      if constexpr (kRecord) s.special[i] = L.special ? 1 : 0;
     }
   }
+  tally_flush_wave(a, tally);   // every lane of the wave takes part
 }
 
 // State of a new request: positions from seeds (or auxiliary lattice points),

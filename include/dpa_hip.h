@@ -297,6 +297,9 @@ typedef struct dpah_tuning {
   int blocks_per_cu;   /* persistent grid = CUs x blocks_per_cu, 1..32         */
   int order;           /* 1: queue in brick Morton order, split across XCDs    */
   int parts;           /* queue parts, 1..8 (the XCD count by default)         */
+  int refill;          /* idle lanes (1..64) that make a wave draw new particles:
+                          1 keeps every lane busy, 64 keeps a wave's particles
+                          together until all have retired                     */
 } dpah_tuning;
 int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
 int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* tuning);
