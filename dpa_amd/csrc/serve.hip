@@ -7,8 +7,9 @@
 // then the ticket becomes ready).  Each tick launches one kernel per
 // (field, integrator, record) group that advances every admitted particle by
 // up to `steps_per_tick` steps.  The per-request mode instead runs each request
-// alone, in arrival order, with allocation, upload, one non-persistent launch,
-// compaction and release per request: the comparator for batching.
+// alone, in arrival order, through the library's single-request path (the
+// dpah_advect / dpah_ftle_aux_device code on the context's one stream): the
+// comparator for batching.
 #include <algorithm>
 #include <cmath>
 #include <chrono>
@@ -566,21 +567,16 @@ void dpah_server::run_per_request() {
     try {
       std::lock_guard<std::recursive_mutex> ctx_lk(ctx->mu);
       DPAH_HIP(hipSetDevice(ctx->device));
-      // the per-request form allocates and frees through the runtime every time
-      struct Uncached {
-        dpah_context* c;
-        bool was;
-        explicit Uncached(dpah_context* x) : c(x), was(x->caching) { c->caching = false; }
-        ~Uncached() { c->caching = was; }
-      } uncached(ctx);
+      // exactly what a caller gets from dpah_advect / dpah_ftle_aux_device on
+      // this context: the same kernels, ordering and allocator
       if (r->spec.kind == DPAH_REQUEST_FTLE_AUX) {
         r->result = run_ftle_aux(ctx, r->spec.field, r->spec.origin, r->spec.lattice_spacing,
                                  r->spec.lattice_shape, r->spec.delta, r->spec.iterations,
                                  r->spec.params.step_size, r->spec.params.start_time,
-                                 r->spec.params.integrator, false);
+                                 r->spec.params.integrator, true);
       } else {
         r->result = run_advect(ctx, r->spec.field, r->seeds.data(), r->rem.data(), 0, r->count,
-                               &r->spec.params, false);
+                               &r->spec.params, true);
       }
       r->steps = r->result->total_steps;
       r->t_device = now_s();

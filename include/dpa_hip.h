@@ -205,11 +205,13 @@ int dpah_ftle_aux_device(dpah_context* ctx, const dpah_field* field, const float
  * No counterpart in dpa (a batch pipeline): a request loop on one GPU that
  * keeps fields resident, admits advection and FTLE queries by memory and
  * batches them continuously.  One engine thread owns the device.  In
- * DPAH_SERVE_BATCHED mode every tick advances the particles of all admitted
- * requests by up to steps_per_tick steps in one launch per (field, integrator,
- * record) group; requests join and leave between ticks.  DPAH_SERVE_PER_REQUEST
- * runs each request alone in arrival order (allocate, upload, one
- * non-persistent launch, compact, release): the comparator.
+ * DPAH_SERVE_BATCHED mode admitted requests are launched as soon as one of
+ * four streams is free; the requests waiting at that moment that share
+ * (field, integrator, record) go into ONE launch, which advances each particle
+ * by up to steps_per_tick steps (0: to retirement) and puts unfinished
+ * requests back in line; results are compacted on the launch's stream.
+ * DPAH_SERVE_PER_REQUEST runs each request alone in arrival order through the
+ * single-request path (dpah_advect / dpah_ftle_aux_device): the comparator.
  * A request's result is the same alone and in any mix, bit for bit. */
 #define DPAH_REQUEST_ADVECT   0   /* dpah_advect arguments                    */
 #define DPAH_REQUEST_FTLE_AUX 1   /* dpah_ftle_aux_device arguments           */
@@ -228,7 +230,7 @@ typedef struct dpah_server dpah_server;
 typedef struct dpah_server_options {
   uint64_t memory_budget_bytes;  /* device bytes for admitted requests; 0: half the context's */
   uint32_t max_active;           /* requests admitted at once; 0: 256                        */
-  uint32_t steps_per_tick;       /* per particle per tick; 0: 256                            */
+  uint32_t steps_per_tick;       /* per particle per launch; 0: run to retirement            */
   int      mode;                 /* DPAH_SERVE_*                                              */
 } dpah_server_options;
 
