@@ -260,7 +260,8 @@ def make_context(args):
     import dpa_amd
     ctx = dpa_amd.Context(0, kernel_timing=True)
     over = {k: v for k, v in (("order", args.order), ("blocks_per_cu", args.blocks_per_cu),
-                              ("block", args.block), ("refill", args.refill)) if v is not None}
+                              ("block", args.block), ("refill", args.refill),
+                              ("cells", args.cells)) if v is not None}
     if over:
         ctx.set_tuning(**over)
     log(f"tuning {ctx.tuning()}")
@@ -475,6 +476,7 @@ def main() -> int:
     ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--refill", type=int, default=None)
+    ap.add_argument("--cells", type=int, choices=[0, 1, 2], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

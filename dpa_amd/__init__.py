@@ -65,7 +65,7 @@ class _Request(C.Structure):
 
 class _Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("order", C.c_int),
-                ("parts", C.c_int), ("refill", C.c_int)]
+                ("parts", C.c_int), ("refill", C.c_int), ("cells", C.c_int)]
 
 
 class _RequestInfo(C.Structure):
@@ -341,12 +341,13 @@ class Context:
         t = _Tuning()
         _check(self.lib.dpah_get_tuning(self.h, C.byref(t)))
         return dict(block=t.block, blocks_per_cu=t.blocks_per_cu, order=t.order, parts=t.parts,
-                    refill=t.refill)
+                    refill=t.refill, cells=t.cells)
 
     def set_tuning(self, **kw) -> None:
         cur = self.tuning()
         cur.update(kw)
-        t = _Tuning(cur["block"], cur["blocks_per_cu"], cur["order"], cur["parts"], cur["refill"])
+        t = _Tuning(cur["block"], cur["blocks_per_cu"], cur["order"], cur["parts"], cur["refill"],
+                    cur["cells"])
         _check(self.lib.dpah_set_tuning(self.h, C.byref(t)))
 
 

@@ -103,9 +103,14 @@ void order_queue(dpah_context* ctx, AdvectArgs& a, QueueOrder& q) {
 
 }  // namespace
 
+bool use_cells(const dpah_context* ctx, bool scattered) {
+  return ctx->tune.cells == 2 || (ctx->tune.cells == 1 && scattered);
+}
+
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   AdvectArgs a{};
   a.f = *L.f;
+  if (!use_cells(ctx, L.seed_kind == kSeedArray)) a.f.cells = nullptr;
   a.segs = L.segs;
   a.nseg = L.nseg;
   a.seed_kind = L.seed_kind;

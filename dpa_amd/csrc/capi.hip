@@ -247,6 +247,50 @@ DPAH_EXPORT int dpah_copy_to_host(dpah_context* ctx, void* dst, const void* src,
 
 // -------------------------------------------------------------- fields ----
 
+// One thread per cell record: gathers the cell's eight corners from the node
+// array and writes them, in corner order 4a+2b+c, as one 128-B record.
+static __global__ __launch_bounds__(256) void k_build_cells(FieldView v, float4* out,
+                                                            uint64_t records) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= records) return;
+  const uint64_t m2 = (uint64_t)v.n2 - 1, m1 = (uint64_t)v.n1 - 1, m0 = (uint64_t)v.n0 - 1;
+  const uint64_t c2 = r % m2, r1 = r / m2;
+  const uint64_t c1 = r1 % m1, r0 = r1 / m1;
+  const uint64_t c0 = r0 % m0, it = r0 / m0;
+  const uint64_t n2 = (uint64_t)v.n2, n12 = (uint64_t)v.n1 * v.n2;
+  const float* s = v.data + 3 * it * (uint64_t)v.slice;
+  const uint64_t b = c0 * n12 + c1 * n2 + c2;
+  const uint64_t off[8] = {b, b + 1, b + n2, b + n2 + 1, b + n12, b + n12 + 1, b + n12 + n2,
+                           b + n12 + n2 + 1};
+  float c[32];
+  for (int k = 0; k < 8; ++k) {
+    const float* p = s + 3 * off[k];
+    c[3 * k] = p[0]; c[3 * k + 1] = p[1]; c[3 * k + 2] = p[2];
+  }
+  for (int k = 24; k < 32; ++k) c[k] = 0.0f;
+  float4* o = out + 8 * r;
+  for (int k = 0; k < 8; ++k) o[k] = make_float4(c[4 * k], c[4 * k + 1], c[4 * k + 2], c[4 * k + 3]);
+}
+
+// Builds the cell layout of a field when the context asks for it and the copy
+// (128 B per cell and time slice) fits a tenth of the memory budget.
+static void build_cells(dpah_context* ctx, dpah_field* f) {
+  FieldView& v = f->view;
+  const uint64_t records = (uint64_t)(v.n0 - 1) * (v.n1 - 1) * (v.n2 - 1) * (uint64_t)v.nt;
+  const uint64_t bytes = records * 128;
+  if (ctx->tune.cells == 0 || bytes > ctx->memory_budget / 10) return;
+  f->d_cells = static_cast<float4*>(ctx->alloc(bytes));
+  f->cell_bytes = bytes;
+  const uint64_t grid = (records + 255) / 256;
+  DPAH_REQUIRE(grid < (1ull << 31), "field too large for the cell layout");
+  hipLaunchKernelGGL(k_build_cells, dim3((unsigned)grid), dim3(256), 0, ctx->stream, v,
+                     f->d_cells, records);
+  DPAH_HIP(hipGetLastError());
+  ctx->sync();
+  v.cells = f->d_cells;
+  v.cell_slice = (int64_t)(records / (uint64_t)v.nt);
+}
+
 static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shape[3], uint64_t nt,
                        const float offset[3], const float spacing[3], float t_off, float t_sp,
                        dpah_field** out) {
@@ -282,6 +326,16 @@ static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shap
   v.nt = (int32_t)nt;
   v.o0 = offset[0]; v.o1 = offset[1]; v.o2 = offset[2]; v.to = t_off;
   v.h0 = spacing[0]; v.h1 = spacing[1]; v.h2 = spacing[2]; v.th = nt > 1 ? t_sp : 1.0f;
+  v.cells = nullptr;
+  v.cell_slice = 0;
+  try {
+    build_cells(ctx, f);
+  } catch (...) {
+    if (f->d_cells) ctx->release(f->d_cells, f->cell_bytes);
+    ctx->release(f->d_data, f->bytes);
+    delete f;
+    throw;
+  }
   *out = f;
 }
 
@@ -309,6 +363,7 @@ DPAH_EXPORT int dpah_field_release(dpah_field* field) {
     DPAH_HIP(hipSetDevice(ctx->device));
     DPAH_HIP(hipStreamSynchronize(ctx->stream));
     ctx->release(field->d_data, field->bytes);
+    if (field->d_cells) ctx->release(field->d_cells, field->cell_bytes);
     delete field;
   });
 }
@@ -606,6 +661,7 @@ DPAH_EXPORT int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out) {
     out->order = ctx->tune.order;
     out->parts = ctx->tune.parts;
     out->refill = ctx->tune.refill;
+    out->cells = ctx->tune.cells;
   });
 }
 
@@ -618,11 +674,13 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(t->order == 0 || t->order == 1, "order must be 0 or 1");
     DPAH_REQUIRE(t->parts >= 1 && t->parts <= dpah::kMaxParts, "parts must be 1..8");
     DPAH_REQUIRE(t->refill >= 1 && t->refill <= 64, "refill must be 1..64");
+    DPAH_REQUIRE(t->cells >= 0 && t->cells <= 2, "cells must be 0, 1 or 2");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     ctx->tune.block = t->block;
     ctx->tune.blocks_per_cu = t->blocks_per_cu;
     ctx->tune.order = t->order;
     ctx->tune.parts = t->parts;
     ctx->tune.refill = t->refill;
+    ctx->tune.cells = t->cells;
   });
 }

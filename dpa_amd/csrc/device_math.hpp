@@ -21,6 +21,12 @@ struct FieldView {
   int32_t n0, n1, n2, nt;
   float o0, o1, o2, to;
   float h0, h1, h2, th;
+  // Optional cell layout: record ((it*(n0-1) + c0)*(n1-1) + c1)*(n2-1) + c2 holds
+  // the eight corners of cell (c0, c1, c2) of slice it as 24 floats in corner
+  // order 4a+2b+c, padded to 128 B, so a reload reads one aligned line instead
+  // of four rows.  The values are copies of the nodes; null when absent.
+  const float4* cells;
+  int64_t cell_slice;  // records per time slice
 };
 
 enum Status : uint8_t { kActive = 0, kCompleted = 1, kExited = 2, kZero = 3 };
@@ -75,6 +81,20 @@ __device__ __forceinline__ V3 node(const FieldView& f, const float* base, int64_
   return V3{p[0], p[1], p[2]};
 }
 
+// Record index of cell (c0, c1, c2) of time slice it in the cell layout.
+__device__ __forceinline__ int64_t cell_record(const FieldView& f, int32_t it, int32_t c0,
+                                               int32_t c1, int32_t c2) {
+  return (((int64_t)it * (f.n0 - 1) + c0) * (f.n1 - 1) + c1) * (int64_t)(f.n2 - 1) + c2;
+}
+
+// Eight corners from one cell record: six 16-B loads of one 128-B line.
+__device__ __forceinline__ void load_cell(const float4* rec, V3& L0, V3& L1, V3& L2, V3& L3,
+                                          V3& L4, V3& L5, V3& L6, V3& L7) {
+  const float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3], e = rec[4], g = rec[5];
+  L0 = V3{a.x, a.y, a.z}; L1 = V3{a.w, b.x, b.y}; L2 = V3{b.z, b.w, c.x}; L3 = V3{c.y, c.z, c.w};
+  L4 = V3{d.x, d.y, d.z}; L5 = V3{d.w, e.x, e.y}; L6 = V3{e.z, e.w, g.x}; L7 = V3{g.y, g.z, g.w};
+}
+
 // The eight corners of the cell a particle last sampled, kept in registers.
 // A particle moves a fraction of a cell per step, so most steps sample the
 // same cell again and need no memory access at all; only lanes that cross
@@ -97,16 +117,21 @@ __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
                                         const Axis& a0, const Axis& a1, const Axis& a2,
                                         CellCache& cc) {
   if (a0.c != cc.c0 || a1.c != cc.c1 || a2.c != cc.c2) {
-    const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
-    const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
-    cc.L0 = node(f, slice, b);
-    cc.L1 = node(f, slice, b + 1);
-    cc.L2 = node(f, slice, b + n2);
-    cc.L3 = node(f, slice, b + n2 + 1);
-    cc.L4 = node(f, slice, b + n12);
-    cc.L5 = node(f, slice, b + n12 + 1);
-    cc.L6 = node(f, slice, b + n12 + n2);
-    cc.L7 = node(f, slice, b + n12 + n2 + 1);
+    if (f.cells) {   // field-uniform branch
+      const float4* rec = f.cells + 8 * cell_record(f, 0, a0.c, a1.c, a2.c);
+      load_cell(rec, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
+    } else {
+      const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
+      const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
+      cc.L0 = node(f, slice, b);
+      cc.L1 = node(f, slice, b + 1);
+      cc.L2 = node(f, slice, b + n2);
+      cc.L3 = node(f, slice, b + n2 + 1);
+      cc.L4 = node(f, slice, b + n12);
+      cc.L5 = node(f, slice, b + n12 + 1);
+      cc.L6 = node(f, slice, b + n12 + n2);
+      cc.L7 = node(f, slice, b + n12 + n2 + 1);
+    }
     cc.c0 = a0.c; cc.c1 = a1.c; cc.c2 = a2.c;
   }
   const float w2 = a2.w, m2 = 1.0f - w2;
@@ -125,19 +150,26 @@ __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
 __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, const Axis& a1,
                                          const Axis& a2, const Axis& at, CellCache& cc) {
   if (a0.c != cc.c0 || a1.c != cc.c1 || a2.c != cc.c2 || at.c != cc.ct) {
-    const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
-    const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
-    const float* s0 = f.data + 3 * (int64_t)at.c * f.slice;
-    const float* s1 = s0 + 3 * f.slice;
-    cc.L0 = node(f, s0, b);            cc.M0 = node(f, s1, b);
-    cc.L1 = node(f, s0, b + 1);        cc.M1 = node(f, s1, b + 1);
-    cc.L2 = node(f, s0, b + n2);       cc.M2 = node(f, s1, b + n2);
-    cc.L3 = node(f, s0, b + n2 + 1);   cc.M3 = node(f, s1, b + n2 + 1);
-    cc.L4 = node(f, s0, b + n12);      cc.M4 = node(f, s1, b + n12);
-    cc.L5 = node(f, s0, b + n12 + 1);  cc.M5 = node(f, s1, b + n12 + 1);
-    cc.L6 = node(f, s0, b + n12 + n2); cc.M6 = node(f, s1, b + n12 + n2);
-    cc.L7 = node(f, s0, b + n12 + n2 + 1);
-    cc.M7 = node(f, s1, b + n12 + n2 + 1);
+    if (f.cells) {   // field-uniform branch
+      const float4* r0 = f.cells + 8 * cell_record(f, at.c, a0.c, a1.c, a2.c);
+      const float4* r1 = r0 + 8 * f.cell_slice;
+      load_cell(r0, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
+      load_cell(r1, cc.M0, cc.M1, cc.M2, cc.M3, cc.M4, cc.M5, cc.M6, cc.M7);
+    } else {
+      const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
+      const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
+      const float* s0 = f.data + 3 * (int64_t)at.c * f.slice;
+      const float* s1 = s0 + 3 * f.slice;
+      cc.L0 = node(f, s0, b);            cc.M0 = node(f, s1, b);
+      cc.L1 = node(f, s0, b + 1);        cc.M1 = node(f, s1, b + 1);
+      cc.L2 = node(f, s0, b + n2);       cc.M2 = node(f, s1, b + n2);
+      cc.L3 = node(f, s0, b + n2 + 1);   cc.M3 = node(f, s1, b + n2 + 1);
+      cc.L4 = node(f, s0, b + n12);      cc.M4 = node(f, s1, b + n12);
+      cc.L5 = node(f, s0, b + n12 + 1);  cc.M5 = node(f, s1, b + n12 + 1);
+      cc.L6 = node(f, s0, b + n12 + n2); cc.M6 = node(f, s1, b + n12 + n2);
+      cc.L7 = node(f, s0, b + n12 + n2 + 1);
+      cc.M7 = node(f, s1, b + n12 + n2 + 1);
+    }
     cc.c0 = a0.c; cc.c1 = a1.c; cc.c2 = a2.c; cc.ct = at.c;
   }
   const float wt = at.w, mt = 1.0f - wt;

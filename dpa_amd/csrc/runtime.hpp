@@ -76,6 +76,10 @@ struct Tuning {
   int order = 1;            // 1: spatially ordered queue split across the XCDs
   int parts = 8;            // queue parts (XCDs on the device)
   int refill = 64;          // idle lanes that make a wave draw new particles (1..64)
+  // 0: node layout only; 1: fields also get a cell layout when it fits, read by
+  // launches of seed arrays (scattered particles), while lattice seeds, whose
+  // neighbouring lanes share node rows, read the nodes; 2: cells for every launch
+  int cells = 1;
 };
 
 // Device counters of a context: [1] particle-steps, [2] FTLE misses,
@@ -139,6 +143,8 @@ struct dpah_field {
   dpah_context* ctx = nullptr;
   float* d_data = nullptr;
   uint64_t bytes = 0;
+  float4* d_cells = nullptr;      // cell layout (FieldView::cells), or null
+  uint64_t cell_bytes = 0;
   dpah::FieldView view{};
   uint64_t shape[3] = {0, 0, 0};
   uint64_t time_steps = 1;
@@ -223,6 +229,8 @@ struct AdvectLaunch {
   bool persistent;
 };
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
+// Whether a launch reads the cell layout (when its field has one).
+bool use_cells(const dpah_context* ctx, bool scattered);
 
 // One rank's advection stage (dpah_advect); persistent=false uses one thread
 // per particle (the per-request form).

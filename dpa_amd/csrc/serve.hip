@@ -387,6 +387,9 @@ bool dpah_server::launch_ready() {
                               hipMemcpyHostToDevice, s));
     TickArgs a{};
     a.f = std::get<0>(key)->view;
+    bool scattered = true;   // FTLE requests seed lattices
+    for (auto& rp : L.reqs) scattered &= rp->spec.kind == DPAH_REQUEST_ADVECT;
+    if (!use_cells(ctx, scattered)) a.f.cells = nullptr;
     a.segs = L.segs.as<TickSeg>();
     a.nseg = (uint32_t)segs.size();
     a.n = q;

@@ -302,6 +302,12 @@ typedef struct dpah_tuning {
   int refill;          /* idle lanes (1..64) that make a wave draw new particles:
                           1 keeps every lane busy, 64 keeps a wave's particles
                           together until all have retired                     */
+  int cells;           /* 0: node layout only.  1: fields created afterwards also
+                          keep a cell layout (each cell's 8 corners in one 128-B
+                          record, 10.7x the node bytes) when it fits a tenth of
+                          the memory budget; launches of seed arrays read it,
+                          lattice-seeded launches (FTLE) read the nodes.
+                          2: every launch reads the cells when present.        */
 } dpah_tuning;
 int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
 int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* tuning);

@@ -1,0 +1,99 @@
+"""Both field layouts against the oracle.
+
+A field keeps its nodes and, when the context's tuning asks for it and it
+fits, a cell layout (each cell's eight corners in one 128-B record).  The
+kernels read corners from whichever the launch is given; the values are the
+same copies, so every result must equal the oracle's bit for bit either way,
+on steady and time-variant fields, with both integrators, for recorded
+curves and for the auxiliary-neighbour FTLE.
+"""
+import sys
+from contextlib import contextmanager
+
+import numpy as np
+import pytest
+
+import dpa_amd
+from dpa_amd import workloads as W
+from oracle import py_oracle as po
+from test_gpu_parity import bits, compare_advect
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+OFFSET, SPACING = (0.5, -1.0, 0.25), (0.75, 1.0, 0.5)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@contextmanager
+def tuned(ctx, **kw):
+    before = ctx.tuning()
+    ctx.set_tuning(**kw)
+    try:
+        yield
+    finally:
+        ctx.set_tuning(**before)
+
+
+def make_field(ctx, data, unsteady):
+    if unsteady:
+        return ctx.field_unsteady(data, OFFSET, SPACING, 0.5, 0.5)
+    return ctx.field(data, OFFSET, SPACING)
+
+
+@pytest.mark.parametrize("cells", [0, 1, 2])
+@pytest.mark.parametrize("unsteady", [False, True])
+@pytest.mark.parametrize("integrator", [po.DPA_RK4, po.RK4])
+def test_layout_matches_oracle(ctx, cells, unsteady, integrator):
+    say(f"case layout cells={cells} unsteady={unsteady} integrator={integrator}")
+    if unsteady:
+        data = W.unsteady_vortex_field(24, time_steps=5, count=6)
+        ofld = po.Field(data, OFFSET, SPACING, 0.5, 0.5)
+    else:
+        data = W.vortex_field(40, count=12)
+        ofld = po.Field(data, OFFSET, SPACING)
+    n = data.shape[-2]
+    lo = np.array(OFFSET, f32)
+    hi = lo + np.array(SPACING, f32) * (n - 1)
+    seeds = W.uniform_seeds(3000, lo, hi, seed_offset=900 + cells)
+    rem = np.full(len(seeds), 120, np.uint64)
+    dt, t0 = (f32(0.01) if unsteady else f32(0.2)), f32(0.75)
+    with tuned(ctx, cells=cells):
+        fld = make_field(ctx, data, unsteady)
+        try:
+            res = ctx.advect(fld, seeds, remaining=rem, particles_per_round=1000, step_size=dt,
+                             start_time=t0, integrator=integrator)
+            try:
+                ora = po.run_rounds(ofld, seeds, rem, 1000, dt, t0, integrator)
+                compare_advect(res, ora, len(seeds))
+                assert ora["total_steps"] > 50000
+            finally:
+                res.free()
+        finally:
+            fld.release()
+
+
+@pytest.mark.parametrize("unsteady", [False, True])
+def test_ftle_aux_same_on_both_layouts(ctx, unsteady):
+    say(f"case ftle aux layouts unsteady={unsteady}")
+    data = (W.unsteady_vortex_field(24, time_steps=5, count=6) if unsteady
+            else W.vortex_field(40, count=12))
+    shape = (6, 9, 11)
+    out = []
+    for cells in (0, 2):
+        with tuned(ctx, cells=cells):
+            fld = make_field(ctx, data, unsteady)
+            d = ctx.alloc(4 * int(np.prod(shape)))
+            try:
+                steps = ctx.ftle_aux_device(fld, (3.0, 2.0, 4.0), (1.5, 1.0, 0.75), shape,
+                                            (0.25, 0.25, 0.25), 60, f32(0.02 if unsteady else 0.2),
+                                            d, start_time=f32(0.6))
+                out.append((ctx.to_host(d, np.empty(int(np.prod(shape)), f32)), steps))
+            finally:
+                ctx.free(d)
+                fld.release()
+    assert out[0][1] == out[1][1] and out[0][1] > 0
+    assert np.array_equal(bits(out[0][0]), bits(out[1][0]))
