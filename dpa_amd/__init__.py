@@ -83,7 +83,7 @@ class _ServerStats(C.Structure):
 
 
 REQUEST_ADVECT, REQUEST_FTLE_AUX = 0, 1
-SERVE_BATCHED, SERVE_PER_REQUEST = 0, 1
+SERVE_BATCHED, SERVE_PER_REQUEST, SERVE_SERIAL = 0, 1, 2
 TICKET_PENDING, TICKET_RUNNING, TICKET_DONE, TICKET_FAILED, TICKET_CANCELLED = range(5)
 
 _lib = None
@@ -407,8 +407,10 @@ class Result:
 
 
 class Server:
-    """A dpah_server: continuous batching (mode SERVE_BATCHED) or the
-    per-request comparator (SERVE_PER_REQUEST)."""
+    """A dpah_server: continuous batching (mode SERVE_BATCHED) or one of the
+    comparators that run requests one at a time (SERVE_PER_REQUEST: allocate,
+    launch, download and free per request; SERVE_SERIAL: the library's
+    single-request path)."""
 
     def __init__(self, ctx: Context, mode: int = SERVE_BATCHED, memory_budget: int = 0,
                  max_active: int = 0, steps_per_tick: int = 0):

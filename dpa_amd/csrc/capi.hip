@@ -124,6 +124,45 @@ void dpah_context::resolve_timings() {
   pending.swap(still);
 }
 
+void dpah::materialize_host(dpah_result* r, hipStream_t s) {
+  dpah_context* ctx = r->ctx;
+  const uint64_t ib = r->use64 ? 8 : 4;
+  for (auto& round : r->rounds)
+    for (auto& c : round.chunks) {
+      if (!c.vertices) continue;
+      c.host_vertices.resize(3 * c.V);
+      c.host_colors.resize(c.V);
+      c.host_indices.resize(c.I * ib);
+      if (c.V) {
+        DPAH_HIP(hipMemcpyAsync(c.host_vertices.data(), c.vertices, 12 * c.V,
+                                hipMemcpyDeviceToHost, s));
+        DPAH_HIP(hipMemcpyAsync(c.host_colors.data(), c.colors, 4 * c.V, hipMemcpyDeviceToHost, s));
+      }
+      if (c.I)
+        DPAH_HIP(hipMemcpyAsync(c.host_indices.data(), c.indices, c.I * ib, hipMemcpyDeviceToHost,
+                                s));
+    }
+  if (r->d_ftle) {
+    r->host_ftle.resize(r->ftle_count);
+    if (r->ftle_count)
+      DPAH_HIP(hipMemcpyAsync(r->host_ftle.data(), r->d_ftle, r->ftle_count * sizeof(float),
+                              hipMemcpyDeviceToHost, s));
+  }
+  DPAH_HIP(hipStreamSynchronize(s));
+  for (auto& round : r->rounds)
+    for (auto& c : round.chunks)
+      if (c.vertices) {
+        ctx->release(c.vertices, c.bytes);
+        c.vertices = nullptr;
+        c.colors = nullptr;
+        c.indices = nullptr;
+      }
+  if (r->d_ftle) {
+    ctx->release(r->d_ftle, r->ftle_count * sizeof(float));
+    r->d_ftle = nullptr;
+  }
+}
+
 dpah_result::~dpah_result() {
   if (d_ftle) ctx->release(d_ftle, ftle_count * sizeof(float));
   for (auto& r : rounds)
@@ -546,6 +585,15 @@ DPAH_EXPORT int dpah_result_copy_round(const dpah_result* r, uint64_t round, flo
     const uint64_t ib = r->use64 ? 8 : 4;
     uint64_t v0 = 0, i0 = 0;
     for (const Chunk& c : r->rounds[round].chunks) {
+      if (!c.vertices) {   // materialised on the host
+        if (vertices && c.V) std::memcpy(vertices + 3 * v0, c.host_vertices.data(), 12 * c.V);
+        if (colors && c.V) std::memcpy(colors + v0, c.host_colors.data(), 4 * c.V);
+        if (indices && c.I)
+          std::memcpy(static_cast<char*>(indices) + i0 * ib, c.host_indices.data(), c.I * ib);
+        v0 += c.V;
+        i0 += c.I;
+        continue;
+      }
       if (vertices && c.V)
         DPAH_HIP(hipMemcpyAsync(vertices + 3 * v0, c.vertices, 3 * c.V * sizeof(float),
                                 hipMemcpyDeviceToHost, ctx->stream));

@@ -153,11 +153,13 @@ struct dpah_field {
 namespace dpah {
 
 struct Chunk {
-  float* vertices = nullptr;
+  float* vertices = nullptr;   // device arrays (one block of `bytes`), or null once on the host
   float* colors = nullptr;
   void* indices = nullptr;
   uint64_t V = 0, I = 0;   // vertices, index entries
   uint64_t bytes = 0;
+  std::vector<float> host_vertices, host_colors;   // after materialize_host
+  std::vector<uint8_t> host_indices;
 };
 
 struct Round {
@@ -177,6 +179,7 @@ struct dpah_result {
   uint64_t total_steps = 0;
   float* d_ftle = nullptr;        // FTLE requests: device volume
   uint64_t ftle_count = 0;
+  std::vector<float> host_ftle;   // FTLE volume after materialize_host
   ~dpah_result();
 };
 
@@ -231,6 +234,10 @@ struct AdvectLaunch {
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
 // Whether a launch reads the cell layout (when its field has one).
 bool use_cells(const dpah_context* ctx, bool scattered);
+
+// Copies a result's device arrays to host memory and releases them (the
+// per-request serving form returns results this way).
+void materialize_host(dpah_result* r, hipStream_t stream);
 
 // One rank's advection stage (dpah_advect); persistent=false uses one thread
 // per particle (the per-request form).

@@ -6,10 +6,11 @@
 // their particle state, and retires finished ones (compaction / FTLE reduce,
 // then the ticket becomes ready).  Each tick launches one kernel per
 // (field, integrator, record) group that advances every admitted particle by
-// up to `steps_per_tick` steps.  The per-request mode instead runs each request
-// alone, in arrival order, through the library's single-request path (the
-// dpah_advect / dpah_ftle_aux_device code on the context's one stream): the
-// comparator for batching.
+// up to `steps_per_tick` steps.  Two comparators run each request alone, in
+// arrival order: the per-request mode allocates, uploads, launches one thread
+// per particle, compacts, downloads and frees for every request; the serial
+// mode goes through the library's single-request path (dpah_advect /
+// dpah_ftle_aux_device on the context's one stream).
 #include <algorithm>
 #include <cmath>
 #include <chrono>
@@ -570,17 +571,27 @@ void dpah_server::run_per_request() {
     try {
       std::lock_guard<std::recursive_mutex> ctx_lk(ctx->mu);
       DPAH_HIP(hipSetDevice(ctx->device));
-      // exactly what a caller gets from dpah_advect / dpah_ftle_aux_device on
-      // this context: the same kernels, ordering and allocator
+      // DPAH_SERVE_SERIAL: exactly what a caller gets from dpah_advect /
+      // dpah_ftle_aux_device on this context (same kernels, ordering, caching
+      // allocator).  DPAH_SERVE_PER_REQUEST: hipMalloc, upload, one thread per
+      // particle, compaction, download and hipFree for every request.
+      const bool naive = opt.mode == DPAH_SERVE_PER_REQUEST;
+      struct CachingOff {
+        dpah_context* c;
+        bool was;
+        CachingOff(dpah_context* x, bool on) : c(x), was(x->caching) { if (on) c->caching = false; }
+        ~CachingOff() { c->caching = was; }
+      } caching_off(ctx, naive);
       if (r->spec.kind == DPAH_REQUEST_FTLE_AUX) {
         r->result = run_ftle_aux(ctx, r->spec.field, r->spec.origin, r->spec.lattice_spacing,
                                  r->spec.lattice_shape, r->spec.delta, r->spec.iterations,
                                  r->spec.params.step_size, r->spec.params.start_time,
-                                 r->spec.params.integrator, true);
+                                 r->spec.params.integrator, !naive);
       } else {
         r->result = run_advect(ctx, r->spec.field, r->seeds.data(), r->rem.data(), 0, r->count,
-                               &r->spec.params, true);
+                               &r->spec.params, !naive);
       }
+      if (naive) materialize_host(r->result, ctx->stream);
       r->steps = r->result->total_steps;
       r->t_device = now_s();
       complete(r, kDone, DPAH_OK, "");
@@ -593,7 +604,7 @@ void dpah_server::run_per_request() {
 }
 
 void dpah_server::run() {
-  if (opt.mode == DPAH_SERVE_PER_REQUEST) run_per_request();
+  if (opt.mode == DPAH_SERVE_PER_REQUEST || opt.mode == DPAH_SERVE_SERIAL) run_per_request();
   else run_batched();
 }
 
@@ -606,6 +617,9 @@ DPAH_EXPORT int dpah_server_create(dpah_context* ctx, const dpah_server_options*
     auto s = std::make_unique<dpah_server>();
     s->ctx = ctx;
     if (options) s->opt = *options;
+    DPAH_REQUIRE(s->opt.mode == DPAH_SERVE_BATCHED || s->opt.mode == DPAH_SERVE_PER_REQUEST ||
+                     s->opt.mode == DPAH_SERVE_SERIAL,
+                 "unknown serving mode");
     if (!s->opt.max_active) s->opt.max_active = 256;
     // steps_per_tick 0: every launch runs its particles to completion
     s->budget = s->opt.memory_budget_bytes ? s->opt.memory_budget_bytes : ctx->memory_budget / 2;
@@ -675,7 +689,7 @@ DPAH_EXPORT int dpah_server_submit(dpah_server* s, const dpah_request* req, uint
     }
     r->footprint = footprint(*r);
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->opt.mode != DPAH_SERVE_PER_REQUEST && r->footprint > s->budget) {
+    if (s->opt.mode == DPAH_SERVE_BATCHED && r->footprint > s->budget) {
       s->stats.rejected += 1;
       throw Error(DPAH_ERR_ADMISSION, "request needs " + std::to_string(r->footprint) +
                                           " bytes, more than the server budget " +
@@ -812,7 +826,9 @@ DPAH_EXPORT int dpah_result_ftle(const dpah_result* r, float* out, uint64_t* cou
   return guarded([&] {
     DPAH_REQUIRE(r && count, "null argument");
     *count = r->ftle_count;
-    if (out && r->ftle_count) {
+    if (out && r->ftle_count && !r->d_ftle) {
+      std::memcpy(out, r->host_ftle.data(), r->ftle_count * sizeof(float));
+    } else if (out && r->ftle_count) {
       std::lock_guard<std::recursive_mutex> lk(r->ctx->mu);
       DPAH_HIP(hipSetDevice(r->ctx->device));
       DPAH_HIP(hipMemcpyAsync(out, r->d_ftle, r->ftle_count * sizeof(float), hipMemcpyDeviceToHost,

@@ -210,14 +210,18 @@ int dpah_ftle_aux_device(dpah_context* ctx, const dpah_field* field, const float
  * (field, integrator, record) go into ONE launch, which advances each particle
  * by up to steps_per_tick steps (0: to retirement) and puts unfinished
  * requests back in line; results are compacted on the launch's stream.
- * DPAH_SERVE_PER_REQUEST runs each request alone in arrival order through the
- * single-request path (dpah_advect / dpah_ftle_aux_device): the comparator.
- * A request's result is the same alone and in any mix, bit for bit. */
+ * Two comparators run each request alone in arrival order:
+ * DPAH_SERVE_PER_REQUEST allocates (hipMalloc), uploads, launches one thread
+ * per particle, compacts, downloads the result to host memory and frees
+ * (hipFree) per request; DPAH_SERVE_SERIAL goes through the single-request
+ * path (dpah_advect / dpah_ftle_aux_device, caching allocator, persistent
+ * kernel).  A request's result is the same in every mode and mix, bit for bit. */
 #define DPAH_REQUEST_ADVECT   0   /* dpah_advect arguments                    */
 #define DPAH_REQUEST_FTLE_AUX 1   /* dpah_ftle_aux_device arguments           */
 
 #define DPAH_SERVE_BATCHED     0
 #define DPAH_SERVE_PER_REQUEST 1
+#define DPAH_SERVE_SERIAL      2
 
 #define DPAH_TICKET_PENDING   0
 #define DPAH_TICKET_RUNNING   1

@@ -98,7 +98,8 @@ def alone(ctx, r):
 
 
 @pytest.mark.parametrize("mode,tick", [(dpa_amd.SERVE_BATCHED, 16), (dpa_amd.SERVE_BATCHED, 1000),
-                                       (dpa_amd.SERVE_PER_REQUEST, 0)])
+                                       (dpa_amd.SERVE_BATCHED, 0), (dpa_amd.SERVE_PER_REQUEST, 0),
+                                       (dpa_amd.SERVE_SERIAL, 0)])
 def test_mixed_requests_match_alone(ctx, fields, mode, tick):
     say(f"case serve mode={mode} steps_per_tick={tick}")
     reqs = make_requests(fields)
