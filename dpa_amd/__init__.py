@@ -65,7 +65,8 @@ class _Request(C.Structure):
 
 class _Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("order", C.c_int),
-                ("parts", C.c_int), ("refill", C.c_int), ("cells", C.c_int)]
+                ("parts", C.c_int), ("refill", C.c_int), ("cells", C.c_int),
+                ("fastdiv", C.c_int)]
 
 
 class _RequestInfo(C.Structure):
@@ -145,6 +146,10 @@ def load_library() -> C.CDLL:
         "dpah_kernel_timing_reset": (C.c_int, [vp]),
         "dpah_get_tuning": (C.c_int, [vp, C.POINTER(_Tuning)]),
         "dpah_set_tuning": (C.c_int, [vp, C.POINTER(_Tuning)]),
+        "dpah_verify_division": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, _u64p]),
+        "dpah_autotune_endpoints": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, C.c_float,
+                                              C.c_float, C.c_int, C.POINTER(_Tuning),
+                                              C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -183,9 +188,10 @@ class Context:
     """A dpah_context on one GPU."""
 
     def __init__(self, device: int = 0, memory_budget: int = 0, slot_budget: int = 0,
-                 kernel_timing: bool = False):
+                 kernel_timing: bool = False, autotune_cache: str | None = None):
         self.lib = load_library()
-        opt = _Options(memory_budget, slot_budget, 1 if kernel_timing else 0, None)
+        self._cache_path = None if autotune_cache is None else str(autotune_cache).encode()
+        opt = _Options(memory_budget, slot_budget, 1 if kernel_timing else 0, self._cache_path)
         h = C.c_void_p()
         _check(self.lib.dpah_create(device, C.byref(opt), C.byref(h)))
         self.h = h
@@ -337,17 +343,38 @@ class Context:
     def kernel_timing_reset(self) -> None:
         _check(self.lib.dpah_kernel_timing_reset(self.h))
 
+    def verify_division(self, h: float, r: float, bound: float) -> int:
+        """Floats in [-bound, bound] where the reciprocal division differs from x / h."""
+        n = C.c_uint64()
+        _check(self.lib.dpah_verify_division(self.h, float(h), float(r), float(bound),
+                                             C.byref(n)))
+        return n.value
+
+    def autotune_endpoints(self, field: "Field", d_seeds: int, count: int, uniform_remaining: int,
+                           step_size: float, start_time: float = 0.0,
+                           integrator: int = INTEGRATOR_DPA_RK4) -> tuple[dict, float]:
+        """Times candidate tunings on a device-resident seed sample; the fastest
+        applies to later launches of the same kind.  Returns (tuning, ms)."""
+        t = _Tuning()
+        ms = C.c_double()
+        _check(self.lib.dpah_autotune_endpoints(self.h, field.h, C.c_void_p(d_seeds), int(count),
+                                                int(uniform_remaining), float(step_size),
+                                                float(start_time), int(integrator), C.byref(t),
+                                                C.byref(ms)))
+        return (dict(block=t.block, blocks_per_cu=t.blocks_per_cu, order=t.order, parts=t.parts,
+                     refill=t.refill, cells=t.cells, fastdiv=t.fastdiv), ms.value)
+
     def tuning(self) -> dict:
         t = _Tuning()
         _check(self.lib.dpah_get_tuning(self.h, C.byref(t)))
         return dict(block=t.block, blocks_per_cu=t.blocks_per_cu, order=t.order, parts=t.parts,
-                    refill=t.refill, cells=t.cells)
+                    refill=t.refill, cells=t.cells, fastdiv=t.fastdiv)
 
     def set_tuning(self, **kw) -> None:
         cur = self.tuning()
         cur.update(kw)
         t = _Tuning(cur["block"], cur["blocks_per_cu"], cur["order"], cur["parts"], cur["refill"],
-                    cur["cells"])
+                    cur["cells"], cur["fastdiv"])
         _check(self.lib.dpah_set_tuning(self.h, C.byref(t)))
 
 

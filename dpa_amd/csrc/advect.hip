@@ -11,11 +11,11 @@ namespace dpah {
 namespace {
 
 template <bool kRecord, int kIntegrator, bool kUnsteady>
-void launch_t(dpah_context* ctx, const AdvectArgs& a, bool persistent) {
+void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool persistent) {
   if (a.n == 0) return;
   if (persistent) {
-    const int block = ctx->tune.block;
-    uint64_t grid = (uint64_t)ctx->cu_count * ctx->tune.blocks_per_cu;
+    const int block = T.block;
+    uint64_t grid = (uint64_t)ctx->cu_count * T.blocks_per_cu;
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
     ctx->timed("advect_persistent", [&] {
@@ -33,14 +33,14 @@ void launch_t(dpah_context* ctx, const AdvectArgs& a, bool persistent) {
 }
 
 template <bool kRecord>
-void dispatch_integrator(dpah_context* ctx, const AdvectArgs& a, int integrator, bool unsteady,
-                         bool persistent) {
+void dispatch_integrator(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, int integrator,
+                         bool unsteady, bool persistent) {
   if (integrator == kRk4) {
-    if (unsteady) launch_t<kRecord, kRk4, true>(ctx, a, persistent);
-    else          launch_t<kRecord, kRk4, false>(ctx, a, persistent);
+    if (unsteady) launch_t<kRecord, kRk4, true>(ctx, T, a, persistent);
+    else          launch_t<kRecord, kRk4, false>(ctx, T, a, persistent);
   } else {
-    if (unsteady) launch_t<kRecord, kDpaRk4, true>(ctx, a, persistent);
-    else          launch_t<kRecord, kDpaRk4, false>(ctx, a, persistent);
+    if (unsteady) launch_t<kRecord, kDpaRk4, true>(ctx, T, a, persistent);
+    else          launch_t<kRecord, kDpaRk4, false>(ctx, T, a, persistent);
   }
 }
 
@@ -68,7 +68,7 @@ void set_parts(AdvectArgs& a, uint32_t parts) {
 // and few enough for 32-bit positions.
 constexpr uint64_t kOrderMinParticles = 1ull << 15;
 
-void order_queue(dpah_context* ctx, AdvectArgs& a, QueueOrder& q) {
+void order_queue(dpah_context* ctx, const Tuning& T, AdvectArgs& a, QueueOrder& q) {
   const int64_t D = std::max({(int64_t)a.f.n0, (int64_t)a.f.n1, (int64_t)a.f.n2});
   uint32_t shift = 2;
   while (((D + (1ll << shift) - 1) >> shift) > (1ll << kOrderMaxBits)) ++shift;
@@ -98,19 +98,33 @@ void order_queue(dpah_context* ctx, AdvectArgs& a, QueueOrder& q) {
   });
   DPAH_HIP(hipGetLastError());
   a.order = o.order;
-  set_parts(a, (uint32_t)std::max(1, std::min(ctx->tune.parts, kMaxParts)));
+  set_parts(a, (uint32_t)std::max(1, std::min(T.parts, kMaxParts)));
 }
 
 }  // namespace
 
-bool use_cells(const dpah_context* ctx, bool scattered) {
-  return ctx->tune.cells == 2 || (ctx->tune.cells == 1 && scattered);
+bool use_cells(const Tuning& T, bool scattered) {
+  return T.cells == 2 || (T.cells == 1 && scattered);
+}
+
+std::string launch_key(const FieldView& f, int integrator, int seed_kind, bool record) {
+  return std::to_string(f.n0) + "x" + std::to_string(f.n1) + "x" + std::to_string(f.n2) + "x" +
+         std::to_string(f.nt) + ":i" + std::to_string(integrator) + ":s" +
+         std::to_string(seed_kind) + ":r" + std::to_string(record ? 1 : 0);
+}
+
+const Tuning& tuning_for(const dpah_context* ctx, const std::string& key) {
+  auto it = ctx->tuned.find(key);
+  return it == ctx->tuned.end() ? ctx->tune : it->second;
 }
 
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
+  const Tuning& T =
+      tuning_for(ctx, launch_key(*L.f, L.integrator, L.seed_kind, L.slots != nullptr));
   AdvectArgs a{};
   a.f = *L.f;
-  if (!use_cells(ctx, L.seed_kind == kSeedArray)) a.f.cells = nullptr;
+  if (!use_cells(T, L.seed_kind == kSeedArray)) a.f.cells = nullptr;
+  if (!T.fastdiv) a.f.fastdiv = 0;
   a.segs = L.segs;
   a.nseg = L.nseg;
   a.seed_kind = L.seed_kind;
@@ -132,16 +146,16 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   a.total_steps = ctx->d_counters + 1;
   a.order = nullptr;
   set_parts(a, 1);
-  a.refill = (uint32_t)ctx->tune.refill;
+  a.refill = (uint32_t)(T.refill ? T.refill : (L.seed_kind == kSeedArray ? 16 : 64));
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;
-  if (L.persistent && ctx->tune.order && ctx->caching && L.nseg == 0 &&
-      L.n >= kOrderMinParticles && L.n < (1ull << 32))
-    order_queue(ctx, a, q);
+  if (L.persistent && T.order && ctx->caching && L.nseg == 0 && L.n >= kOrderMinParticles &&
+      L.n < (1ull << 32))
+    order_queue(ctx, T, a, q);
   const bool unsteady = L.f->nt > 1;
-  if (L.slots) dispatch_integrator<true>(ctx, a, L.integrator, unsteady, L.persistent);
-  else         dispatch_integrator<false>(ctx, a, L.integrator, unsteady, L.persistent);
+  if (L.slots) dispatch_integrator<true>(ctx, T, a, L.integrator, unsteady, L.persistent);
+  else         dispatch_integrator<false>(ctx, T, a, L.integrator, unsteady, L.persistent);
 }
 
 }  // namespace dpah

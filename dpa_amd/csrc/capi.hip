@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -203,6 +204,7 @@ DPAH_EXPORT int dpah_create(int device, const dpah_options* options, dpah_contex
       ctx->arch = prop.gcnArchName;
       if (options && options->autotune_cache_path)
         ctx->autotune_path = options->autotune_cache_path;
+      load_autotune_cache(ctx);
       DPAH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
       ctx->d_counters = static_cast<unsigned long long*>(ctx->alloc(dpah::kCounterCount * sizeof(unsigned long long)));
       DPAH_HIP(hipMemsetAsync(ctx->d_counters, 0, dpah::kCounterCount * sizeof(unsigned long long), ctx->stream));
@@ -311,6 +313,51 @@ static __global__ __launch_bounds__(256) void k_build_cells(FieldView v, float4*
   for (int k = 0; k < 8; ++k) o[k] = make_float4(c[4 * k], c[4 * k + 1], c[4 * k + 2], c[4 * k + 3]);
 }
 
+// Mismatches of div_by(x, h, r) against x / h over every float x in [-B, B].
+static uint64_t verify_division(dpah_context* ctx, float h, float r, float B) {
+  uint32_t hi;
+  std::memcpy(&hi, &B, 4);
+  unsigned long long* bad = ctx->d_counters + 3;
+  DPAH_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), ctx->stream));
+  const unsigned grid = (unsigned)ctx->cu_count * 16;
+  hipLaunchKernelGGL(verify_division_kernel, dim3(grid), dim3(256), 0, ctx->stream, h, r, hi, bad);
+  DPAH_HIP(hipGetLastError());
+  unsigned long long n = 0;
+  DPAH_HIP(hipMemcpyAsync(&n, bad, sizeof(n), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  return n;
+}
+
+// Reciprocal spacings and, per axis, whether the reciprocal division matches
+// the correctly rounded one over the axis's whole range (checked once per
+// (spacing, range) on a context).
+static void set_division(dpah_context* ctx, FieldView& v) {
+  const float h[4] = {v.h0, v.h1, v.h2, v.th};
+  const int32_t n[4] = {v.n0, v.n1, v.n2, v.nt};
+  float r[4];
+  v.fastdiv = 0;
+  for (int a = 0; a < 4; ++a) {
+    r[a] = 1.0f / h[a];
+    if (n[a] < 2 || !ctx->tune.fastdiv) continue;
+    const float B = std::nextafter(2.0f * (float)(n[a] - 1) * h[a], INFINITY);
+    if (!(B < INFINITY)) continue;
+    uint32_t hb, bb;
+    std::memcpy(&hb, &h[a], 4);
+    std::memcpy(&bb, &B, 4);
+    const uint64_t key = ((uint64_t)hb << 32) | bb;
+    auto it = ctx->division_checked.find(key);
+    bool ok;
+    if (it != ctx->division_checked.end()) {
+      ok = it->second;
+    } else {
+      ok = verify_division(ctx, h[a], r[a], B) == 0;
+      ctx->division_checked[key] = ok;
+    }
+    if (ok) v.fastdiv |= 1u << a;
+  }
+  v.r0 = r[0]; v.r1 = r[1]; v.r2 = r[2]; v.rt = r[3];
+}
+
 // Builds the cell layout of a field when the context asks for it and the copy
 // (128 B per cell and time slice) fits a tenth of the memory budget.
 static void build_cells(dpah_context* ctx, dpah_field* f) {
@@ -368,6 +415,7 @@ static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shap
   v.cells = nullptr;
   v.cell_slice = 0;
   try {
+    set_division(ctx, v);
     build_cells(ctx, f);
   } catch (...) {
     if (f->d_cells) ctx->release(f->d_cells, f->cell_bytes);
@@ -710,6 +758,7 @@ DPAH_EXPORT int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out) {
     out->parts = ctx->tune.parts;
     out->refill = ctx->tune.refill;
     out->cells = ctx->tune.cells;
+    out->fastdiv = ctx->tune.fastdiv;
   });
 }
 
@@ -721,8 +770,9 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(t->blocks_per_cu >= 1 && t->blocks_per_cu <= 32, "blocks_per_cu must be 1..32");
     DPAH_REQUIRE(t->order == 0 || t->order == 1, "order must be 0 or 1");
     DPAH_REQUIRE(t->parts >= 1 && t->parts <= dpah::kMaxParts, "parts must be 1..8");
-    DPAH_REQUIRE(t->refill >= 1 && t->refill <= 64, "refill must be 1..64");
+    DPAH_REQUIRE(t->refill >= 0 && t->refill <= 64, "refill must be 0 (auto) or 1..64");
     DPAH_REQUIRE(t->cells >= 0 && t->cells <= 2, "cells must be 0, 1 or 2");
+    DPAH_REQUIRE(t->fastdiv == 0 || t->fastdiv == 1, "fastdiv must be 0 or 1");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     ctx->tune.block = t->block;
     ctx->tune.blocks_per_cu = t->blocks_per_cu;
@@ -730,5 +780,18 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     ctx->tune.parts = t->parts;
     ctx->tune.refill = t->refill;
     ctx->tune.cells = t->cells;
+    ctx->tune.fastdiv = t->fastdiv;
+    ctx->tuned.clear();   // explicit settings replace autotuned ones
+  });
+}
+
+DPAH_EXPORT int dpah_verify_division(dpah_context* ctx, float h, float r, float bound,
+                                     uint64_t* mismatches) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && mismatches, "null argument");
+    DPAH_REQUIRE(h > 0.0f && bound >= 0.0f && bound < INFINITY, "need h > 0 and a finite bound");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    *mismatches = verify_division(ctx, h, r, bound);
   });
 }

@@ -27,6 +27,11 @@ struct FieldView {
   // of four rows.  The values are copies of the nodes; null when absent.
   const float4* cells;
   int64_t cell_slice;  // records per time slice
+  // Reciprocals of the spacings and, per axis (bit 0..2, bit 3 = time), whether
+  // div_by() with that reciprocal was verified at field creation to equal the
+  // correctly rounded quotient for every float in the axis's range.
+  float r0, r1, r2, rt;
+  uint32_t fastdiv;
 };
 
 enum Status : uint8_t { kActive = 0, kCompleted = 1, kExited = 2, kZero = 3 };
@@ -53,14 +58,51 @@ __device__ __forceinline__ float exact_fmod(float d, float h, float s) {
   return r;
 }
 
+// d / h from r = fl(1/h): q0 = fl(d*r), the residual d - q0*h is exact in one
+// fmaf, and one more fmaf corrects q0 -- three instructions where a correctly
+// rounded division takes about twelve.  Below |d| = 2^-90 the residual could
+// underflow and lose exactness, so zeros, tiny values and NaN divide as usual.
+// It is used only on axes where the field's creation checked it, bit for bit,
+// against d / h for every float d in [-B, B], B = 2 (n-1) h
+// (verify_division_kernel); outside that range the quotient only decides
+// "outside the grid", and both forms agree on that (same sign, magnitude
+// beyond n-1).
+__device__ __forceinline__ float div_by(float d, float h, float r) {
+  if (!(__builtin_fabsf(d) >= 0x1p-90f)) return d / h;
+  const float q0 = d * r;
+  const float e = __builtin_fmaf(-q0, h, d);
+  return __builtin_fmaf(e, r, q0);
+}
+
+__device__ __forceinline__ float quotient(float d, float h, float r, bool fast) {
+  return fast ? div_by(d, h, r) : d / h;
+}
+
+// Counts the floats x in [-B, B] (bit patterns 0..hi of |x|, both signs) where
+// div_by(x, h, r) and x / h differ in any bit.  Grid-stride over 2^32-bounded
+// indices; one atomic per mismatch.
+static __global__ __launch_bounds__(256) void verify_division_kernel(float h, float r, uint32_t hi,
+                                                                     unsigned long long* bad) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= hi; i += stride) {
+    const float x = __uint_as_float((uint32_t)i);
+    const float qp = x / h, qn = (-x) / h;
+    const bool ok = __float_as_uint(div_by(x, h, r)) == __float_as_uint(qp) &&
+                    __float_as_uint(div_by(-x, h, r)) == __float_as_uint(qn);
+    if (!ok) atomicAdd(bad, 1ull);
+  }
+}
+
 // regular_grid::contains for one axis (regular_grid.hpp:41-50) and the weight /
 // start index of regular_grid::interpolate (52-75).  Returns false outside.
-__device__ __forceinline__ bool locate(float x, float o, float h, int32_t n, Axis& a) {
+// `fast` is field-uniform (a scalar branch).
+__device__ __forceinline__ bool locate(float x, float o, float h, float rh, bool fast, int32_t n,
+                                       Axis& a) {
   const float d = x - o;
-  const float s = floorf(d / h);
+  const float s = floorf(quotient(d, h, rh, fast));
   if (!(s >= 0.0f) || s >= (float)(n - 1)) return false;
   const float r = exact_fmod(d, h, s);
-  a.w = r / h;
+  a.w = quotient(r, h, rh, fast);   // r in [0, h) or -0: inside the verified range
   a.c = (int32_t)s;
   return true;
 }
@@ -192,12 +234,12 @@ template <bool kUnsteady>
 __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, float p2, float t,
                                        V3& v, CellCache& cc) {
   Axis a0, a1, a2;
-  if (!locate(p0, f.o0, f.h0, f.n0, a0)) return false;
-  if (!locate(p1, f.o1, f.h1, f.n1, a1)) return false;
-  if (!locate(p2, f.o2, f.h2, f.n2, a2)) return false;
+  if (!locate(p0, f.o0, f.h0, f.r0, f.fastdiv & 1u, f.n0, a0)) return false;
+  if (!locate(p1, f.o1, f.h1, f.r1, f.fastdiv & 2u, f.n1, a1)) return false;
+  if (!locate(p2, f.o2, f.h2, f.r2, f.fastdiv & 4u, f.n2, a2)) return false;
   if constexpr (kUnsteady) {
     Axis at;
-    if (!locate(t, f.to, f.th, f.nt, at)) return false;
+    if (!locate(t, f.to, f.th, f.rt, f.fastdiv & 8u, f.nt, at)) return false;
     v = quadlinear(f, a0, a1, a2, at, cc);
   } else {
     (void)t;

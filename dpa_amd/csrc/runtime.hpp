@@ -75,14 +75,19 @@ struct Tuning {
   int blocks_per_cu = 8;    // persistent grid = CUs * blocks_per_cu
   int order = 1;            // 1: spatially ordered queue split across the XCDs
   int parts = 8;            // queue parts (XCDs on the device)
-  int refill = 64;          // idle lanes that make a wave draw new particles (1..64)
+  // idle lanes that make a wave draw new particles (1..64); 0: 16 for seed
+  // arrays (keeps lanes busy) and 64 for lattice seeds (keeps a wave's
+  // neighbouring particles together)
+  int refill = 0;
   // 0: node layout only; 1: fields also get a cell layout when it fits, read by
   // launches of seed arrays (scattered particles), while lattice seeds, whose
   // neighbouring lanes share node rows, read the nodes; 2: cells for every launch
   int cells = 1;
+  int fastdiv = 1;          // reciprocal division on axes where it was verified exact
 };
 
 // Device counters of a context: [1] particle-steps, [2] FTLE misses,
+// [3] division-check mismatches,
 // [kQueueHeads, kQueueHeads + kMaxParts) the persistent queue's part heads.
 constexpr int kCounterCount = 16;
 constexpr int kQueueHeads = 8;
@@ -114,8 +119,10 @@ struct dpah_context {
   std::vector<dpah::PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
   dpah::Tuning tune;
+  std::map<std::string, dpah::Tuning> tuned;   // autotuned launch key -> tuning
   std::string arch;              // gcnArchName
   std::string autotune_path;     // tuning cache file, empty: none
+  std::map<uint64_t, bool> division_checked;   // (spacing bits, range bits) -> exact
   std::recursive_mutex mu;
 
   void* alloc(uint64_t bytes);
@@ -233,7 +240,11 @@ struct AdvectLaunch {
 };
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
 // Whether a launch reads the cell layout (when its field has one).
-bool use_cells(const dpah_context* ctx, bool scattered);
+bool use_cells(const Tuning& T, bool scattered);
+// Autotuning key of a launch: field shape, integrator, seed kind, record flag.
+std::string launch_key(const FieldView& f, int integrator, int seed_kind, bool record);
+// Loads the context's autotune cache file (entries of this device) into ctx->tuned.
+void load_autotune_cache(dpah_context* ctx);
 
 // Copies a result's device arrays to host memory and releases them (the
 // per-request serving form returns results this way).

@@ -390,7 +390,8 @@ bool dpah_server::launch_ready() {
     a.f = std::get<0>(key)->view;
     bool scattered = true;   // FTLE requests seed lattices
     for (auto& rp : L.reqs) scattered &= rp->spec.kind == DPAH_REQUEST_ADVECT;
-    if (!use_cells(ctx, scattered)) a.f.cells = nullptr;
+    if (!use_cells(ctx->tune, scattered)) a.f.cells = nullptr;
+    if (!ctx->tune.fastdiv) a.f.fastdiv = 0;
     a.segs = L.segs.as<TickSeg>();
     a.nseg = (uint32_t)segs.size();
     a.n = q;

@@ -305,16 +305,40 @@ typedef struct dpah_tuning {
   int parts;           /* queue parts, 1..8 (the XCD count by default)         */
   int refill;          /* idle lanes (1..64) that make a wave draw new particles:
                           1 keeps every lane busy, 64 keeps a wave's particles
-                          together until all have retired                     */
+                          together until all have retired; 0: 16 for seed
+                          arrays, 64 for lattice seeds                         */
   int cells;           /* 0: node layout only.  1: fields created afterwards also
                           keep a cell layout (each cell's 8 corners in one 128-B
                           record, 10.7x the node bytes) when it fits a tenth of
                           the memory budget; launches of seed arrays read it,
                           lattice-seeded launches (FTLE) read the nodes.
                           2: every launch reads the cells when present.        */
+  int fastdiv;         /* 1: divisions by a grid spacing use its reciprocal with
+                          one correction on axes where field creation verified
+                          it bit for bit against the correctly rounded quotient
+                          over the axis's range; 0: always divide              */
 } dpah_tuning;
 int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
+/* Sets the context's tuning and drops autotuned entries. */
 int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* tuning);
+
+/* Times candidate tunings (refill 4/16/32/64 x 6/8 workgroups per CU x queue
+ * order on/off x cell/node layout) on a sample of an endpoint workload --
+ * seeds: count x 3 device floats, as dpah_advect_endpoints_device takes them --
+ * and keeps the fastest for launches with the same key (field shape,
+ * integrator, seed array, endpoints).  With dpah_options.autotune_cache_path
+ * set, the choice is written there and read back by later contexts on the same
+ * kind of device.  No counterpart in dpa (its CPU loop has no launch shape). */
+int dpah_autotune_endpoints(dpah_context* ctx, const dpah_field* field, const float* seeds,
+                            uint64_t count, uint64_t uniform_remaining, float step_size,
+                            float start_time, int integrator, dpah_tuning* best,
+                            double* best_ms);
+
+/* The check behind dpah_tuning.fastdiv, exposed for diagnostics: the number of
+ * floats x in [-bound, bound] whose reciprocal quotient (x*r corrected once by
+ * the exact residual) differs in any bit from the correctly rounded x / h.
+ * Replaces the divisions of regular_grid.hpp:45-46, 63-64 on verified axes. */
+int dpah_verify_division(dpah_context* ctx, float h, float r, float bound, uint64_t* mismatches);
 
 #ifdef __cplusplus
 }
