@@ -127,26 +127,20 @@ void dpah_context::resolve_timings() {
 
 void dpah::materialize_host(dpah_result* r, hipStream_t s) {
   dpah_context* ctx = r->ctx;
-  const uint64_t ib = r->use64 ? 8 : 4;
   for (auto& round : r->rounds)
     for (auto& c : round.chunks) {
       if (!c.vertices) continue;
-      c.host_vertices.resize(3 * c.V);
-      c.host_colors.resize(c.V);
-      c.host_indices.resize(c.I * ib);
-      if (c.V) {
-        DPAH_HIP(hipMemcpyAsync(c.host_vertices.data(), c.vertices, 12 * c.V,
-                                hipMemcpyDeviceToHost, s));
-        DPAH_HIP(hipMemcpyAsync(c.host_colors.data(), c.colors, 4 * c.V, hipMemcpyDeviceToHost, s));
-      }
-      if (c.I)
-        DPAH_HIP(hipMemcpyAsync(c.host_indices.data(), c.indices, c.I * ib, hipMemcpyDeviceToHost,
-                                s));
+      // one copy of the block: vertices | colours | indices, c.bytes in all
+      c.colors_off = 12 * c.V;
+      c.indices_off = 16 * c.V;
+      c.host.reset(new uint8_t[c.bytes], std::default_delete<uint8_t[]>());
+      DPAH_HIP(hipMemcpyAsync(c.host.get(), c.vertices, c.bytes, hipMemcpyDeviceToHost, s));
     }
   if (r->d_ftle) {
-    r->host_ftle.resize(r->ftle_count);
+    r->host_ftle.reset(new uint8_t[std::max<uint64_t>(r->ftle_count, 1) * sizeof(float)],
+                       std::default_delete<uint8_t[]>());
     if (r->ftle_count)
-      DPAH_HIP(hipMemcpyAsync(r->host_ftle.data(), r->d_ftle, r->ftle_count * sizeof(float),
+      DPAH_HIP(hipMemcpyAsync(r->host_ftle.get(), r->d_ftle, r->ftle_count * sizeof(float),
                               hipMemcpyDeviceToHost, s));
   }
   DPAH_HIP(hipStreamSynchronize(s));
@@ -634,10 +628,11 @@ DPAH_EXPORT int dpah_result_copy_round(const dpah_result* r, uint64_t round, flo
     uint64_t v0 = 0, i0 = 0;
     for (const Chunk& c : r->rounds[round].chunks) {
       if (!c.vertices) {   // materialised on the host
-        if (vertices && c.V) std::memcpy(vertices + 3 * v0, c.host_vertices.data(), 12 * c.V);
-        if (colors && c.V) std::memcpy(colors + v0, c.host_colors.data(), 4 * c.V);
+        const uint8_t* h = c.host.get();
+        if (vertices && c.V) std::memcpy(vertices + 3 * v0, h, 12 * c.V);
+        if (colors && c.V) std::memcpy(colors + v0, h + c.colors_off, 4 * c.V);
         if (indices && c.I)
-          std::memcpy(static_cast<char*>(indices) + i0 * ib, c.host_indices.data(), c.I * ib);
+          std::memcpy(static_cast<char*>(indices) + i0 * ib, h + c.indices_off, c.I * ib);
         v0 += c.V;
         i0 += c.I;
         continue;

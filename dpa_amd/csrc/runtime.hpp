@@ -6,6 +6,7 @@
 
 #include <atomic>
 #include <map>
+#include <memory>
 #include <set>
 #include <new>
 #include <mutex>
@@ -165,8 +166,9 @@ struct Chunk {
   void* indices = nullptr;
   uint64_t V = 0, I = 0;   // vertices, index entries
   uint64_t bytes = 0;
-  std::vector<float> host_vertices, host_colors;   // after materialize_host
-  std::vector<uint8_t> host_indices;
+  // after materialize_host: the device block's bytes, uninitialised host memory
+  std::shared_ptr<uint8_t> host;
+  uint64_t colors_off = 0, indices_off = 0;   // offsets within the block
 };
 
 struct Round {
@@ -186,7 +188,7 @@ struct dpah_result {
   uint64_t total_steps = 0;
   float* d_ftle = nullptr;        // FTLE requests: device volume
   uint64_t ftle_count = 0;
-  std::vector<float> host_ftle;   // FTLE volume after materialize_host
+  std::shared_ptr<uint8_t> host_ftle;   // FTLE volume after materialize_host
   ~dpah_result();
 };
 

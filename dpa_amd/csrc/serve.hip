@@ -828,7 +828,7 @@ DPAH_EXPORT int dpah_result_ftle(const dpah_result* r, float* out, uint64_t* cou
     DPAH_REQUIRE(r && count, "null argument");
     *count = r->ftle_count;
     if (out && r->ftle_count && !r->d_ftle) {
-      std::memcpy(out, r->host_ftle.data(), r->ftle_count * sizeof(float));
+      std::memcpy(out, r->host_ftle.get(), r->ftle_count * sizeof(float));
     } else if (out && r->ftle_count) {
       std::lock_guard<std::recursive_mutex> lk(r->ctx->mu);
       DPAH_HIP(hipSetDevice(r->ctx->device));
