@@ -236,6 +236,26 @@ def run_cfg2(args):
     ol, oms = ctx.kernel_timing("order_queue")
     if ol:
         log(f"queue ordering avg {oms / ol:.3f} ms over {ol} launches")
+    # the same passes replayed from a hipGraph captured once
+    graph = None
+    if args.graph:
+        with ctx.pass_endpoints(fld, d_seeds, n, d_pos, uniform_remaining=args.iterations,
+                                step_size=dt, d_rem_out=d_rem, d_status_out=d_st) as gp:
+            gp.launch()
+            if gp.total_steps() != steps_per_pass:
+                raise RuntimeError("graph replay took a different number of steps")
+            ctx.synchronize()
+            tg = time.perf_counter()
+            for _ in range(args.steps):
+                gp.launch()
+            ctx.synchronize()
+            g_elapsed = time.perf_counter() - tg
+            if gp.total_steps() != steps_per_pass:
+                raise RuntimeError("graph replay took a different number of steps")
+        graph = {"particle_steps_per_s": args.steps * steps_per_pass / g_elapsed,
+                 "ms_per_pass": g_elapsed / args.steps * 1e3}
+        log(f"graph replays: {graph['particle_steps_per_s']:.4e} particle-steps/s "
+            f"({graph['ms_per_pass']:.3f} ms per pass)")
     for p in (d_seeds, d_pos, d_rem, d_st):
         ctx.free(p)
     ctx_tuning = ctx.tuning()
@@ -274,6 +294,7 @@ def run_cfg2(args):
     out["roofline"] = rf
     out["cpu_baseline"] = cpu
     out["serving"] = serving
+    out["graph_replay"] = graph
     return out
 
 
@@ -501,6 +522,8 @@ def main() -> int:
     ap.add_argument("--refill", type=int, default=None)
     ap.add_argument("--cells", type=int, choices=[0, 1, 2], default=None)
     ap.add_argument("--fastdiv", type=int, choices=[0, 1], default=None)
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="skip the hipGraph replay leg")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the launch-shape autotuner before the timed passes")
     args = ap.parse_args()

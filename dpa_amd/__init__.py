@@ -147,6 +147,12 @@ def load_library() -> C.CDLL:
         "dpah_get_tuning": (C.c_int, [vp, C.POINTER(_Tuning)]),
         "dpah_set_tuning": (C.c_int, [vp, C.POINTER(_Tuning)]),
         "dpah_verify_division": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, _u64p]),
+        "dpah_pass_create_endpoints": (C.c_int, [vp, vp, vp, vp, C.c_uint64, C.c_uint64,
+                                                 C.c_float, C.c_float, C.c_int, vp, vp, vp,
+                                                 C.POINTER(vp)]),
+        "dpah_pass_launch": (C.c_int, [vp]),
+        "dpah_pass_total_steps": (C.c_int, [vp, _u64p]),
+        "dpah_pass_destroy": (C.c_int, [vp]),
         "dpah_autotune_endpoints": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, C.c_float,
                                               C.c_float, C.c_int, C.POINTER(_Tuning),
                                               C.POINTER(C.c_double)]),
@@ -224,6 +230,11 @@ class Context:
         p = self.alloc(max(arr.nbytes, 1))
         _check(self.lib.dpah_copy_to_device(self.h, C.c_void_p(p), _ptr(arr), arr.nbytes))
         return p
+
+    def memset_zero(self, ptr: int, nbytes: int) -> None:
+        """Zeroes device memory (through a host copy: test and setup use only)."""
+        z = np.zeros(nbytes, np.uint8)
+        _check(self.lib.dpah_copy_to_device(self.h, C.c_void_p(ptr), _ptr(z), nbytes))
 
     def to_host(self, ptr: int, out: np.ndarray) -> np.ndarray:
         _check(self.lib.dpah_copy_to_host(self.h, _ptr(out), C.c_void_p(ptr), out.nbytes))
@@ -350,6 +361,20 @@ class Context:
                                              C.byref(n)))
         return n.value
 
+    def pass_endpoints(self, field: "Field", d_seeds: int, count: int, d_pos_out: int,
+                       d_remaining: int | None = None, uniform_remaining: int = 0,
+                       step_size: float = 0.1, start_time: float = 0.0,
+                       integrator: int = INTEGRATOR_DPA_RK4, d_rem_out: int | None = None,
+                       d_status_out: int | None = None) -> "Pass":
+        """An endpoint pass captured into a hipGraph (see Pass)."""
+        h = C.c_void_p()
+        _check(self.lib.dpah_pass_create_endpoints(
+            self.h, field.h, C.c_void_p(d_seeds), C.c_void_p(d_remaining) if d_remaining else None,
+            int(uniform_remaining), int(count), float(step_size), float(start_time),
+            int(integrator), C.c_void_p(d_pos_out), C.c_void_p(d_rem_out) if d_rem_out else None,
+            C.c_void_p(d_status_out) if d_status_out else None, C.byref(h)))
+        return Pass(self, h)
+
     def autotune_endpoints(self, field: "Field", d_seeds: int, count: int, uniform_remaining: int,
                            step_size: float, start_time: float = 0.0,
                            integrator: int = INTEGRATOR_DPA_RK4) -> tuple[dict, float]:
@@ -376,6 +401,32 @@ class Context:
         t = _Tuning(cur["block"], cur["blocks_per_cu"], cur["order"], cur["parts"], cur["refill"],
                     cur["cells"], cur["fastdiv"])
         _check(self.lib.dpah_set_tuning(self.h, C.byref(t)))
+
+
+class Pass:
+    """A captured endpoint pass: launch() replays it on the context's stream."""
+
+    def __init__(self, ctx: "Context", h):
+        self.ctx, self.h = ctx, h
+
+    def launch(self) -> None:
+        _check(self.ctx.lib.dpah_pass_launch(self.h))
+
+    def total_steps(self) -> int:
+        s = C.c_uint64()
+        _check(self.ctx.lib.dpah_pass_total_steps(self.h, C.byref(s)))
+        return s.value
+
+    def destroy(self) -> None:
+        if self.h:
+            _check(self.ctx.lib.dpah_pass_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
 
 
 class Field:

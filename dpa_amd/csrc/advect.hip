@@ -64,27 +64,48 @@ void set_parts(AdvectArgs& a, uint32_t parts) {
   for (uint32_t p = 0; p <= parts; ++p) a.part_lo[p] = a.n / parts * p + std::min<uint64_t>(p, a.n % parts);
 }
 
-// Particles worth ordering: enough to fill the persistent grid several times,
-// and few enough for 32-bit positions.
-constexpr uint64_t kOrderMinParticles = 1ull << 15;
+}  // namespace
 
-void order_queue(dpah_context* ctx, const Tuning& T, AdvectArgs& a, QueueOrder& q) {
-  const int64_t D = std::max({(int64_t)a.f.n0, (int64_t)a.f.n1, (int64_t)a.f.n2});
-  uint32_t shift = 2;
+// Brick size and key count of a field's queue ordering: bricks of 2^shift
+// cells per axis, at most 2^kOrderMaxBits per axis; one more key for seeds
+// outside the grid; tiles of the histogram scan.
+void order_shape(const FieldView& f, uint32_t& shift, uint32_t& bits, uint32_t& keys,
+                 uint32_t& tiles) {
+  const int64_t D = std::max({(int64_t)f.n0, (int64_t)f.n1, (int64_t)f.n2});
+  shift = 2;
   while (((D + (1ll << shift) - 1) >> shift) > (1ll << kOrderMaxBits)) ++shift;
-  uint32_t bits = 1;
+  bits = 1;
   while ((1ll << (bits + shift)) < D) ++bits;
+  keys = (1u << (3 * bits)) + 1;
+  tiles = (keys + kOrderTile - 1) / kOrderTile;   // <= 129 for 6 bits
+}
+
+bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch& L) {
+  return L.persistent && T.order && (ctx->caching || L.order_scratch) && L.nseg == 0 &&
+         L.n >= kOrderMinParticles && L.n < (1ull << 32);
+}
+
+namespace {
+
+void order_queue(dpah_context* ctx, const Tuning& T, AdvectArgs& a, QueueOrder& q,
+                 const OrderScratch* keep) {
   OrderArgs o{};
-  o.shift = shift;
-  o.bits = bits;
-  o.keys = (1u << (3 * bits)) + 1;
-  q.key = DeviceBuffer(ctx, a.n * sizeof(uint32_t));
-  q.order = DeviceBuffer(ctx, a.n * sizeof(uint32_t));
-  const uint32_t tiles = (o.keys + kOrderTile - 1) / kOrderTile;   // <= 129 for 6 bits
-  q.count = DeviceBuffer(ctx, ((uint64_t)o.keys + tiles) * sizeof(uint32_t));
-  o.key = q.key.as<uint32_t>();
-  o.order = q.order.as<uint32_t>();
-  o.count = q.count.as<uint32_t>();
+  uint32_t tiles = 0;
+  order_shape(a.f, o.shift, o.bits, o.keys, tiles);
+  if (keep) {   // caller-owned buffers (a captured pass replays with the same ones)
+    DPAH_REQUIRE(keep->n_cap >= a.n && keep->count_cap >= (uint64_t)o.keys + tiles,
+                 "order scratch too small");
+    o.key = keep->key;
+    o.order = keep->order;
+    o.count = keep->count;
+  } else {
+    q.key = DeviceBuffer(ctx, a.n * sizeof(uint32_t));
+    q.order = DeviceBuffer(ctx, a.n * sizeof(uint32_t));
+    q.count = DeviceBuffer(ctx, ((uint64_t)o.keys + tiles) * sizeof(uint32_t));
+    o.key = q.key.as<uint32_t>();
+    o.order = q.order.as<uint32_t>();
+    o.count = q.count.as<uint32_t>();
+  }
   const unsigned clear_grid = std::min<unsigned>((o.keys + 255) / 256, 1024u);
   const unsigned grid = (unsigned)((a.n + 255) / 256);
   ctx->timed("order_queue", [&] {
@@ -150,9 +171,7 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;
-  if (L.persistent && T.order && ctx->caching && L.nseg == 0 && L.n >= kOrderMinParticles &&
-      L.n < (1ull << 32))
-    order_queue(ctx, T, a, q);
+  if (ordered_launch(ctx, T, L)) order_queue(ctx, T, a, q, L.order_scratch);
   const bool unsteady = L.f->nt > 1;
   if (L.slots) dispatch_integrator<true>(ctx, T, a, L.integrator, unsteady, L.persistent);
   else         dispatch_integrator<false>(ctx, T, a, L.integrator, unsteady, L.persistent);

@@ -217,6 +217,19 @@ struct DeviceBuffer {
   template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Caller-owned scratch of the queue ordering (key, order: n_cap entries;
+// count: count_cap entries).
+struct OrderScratch {
+  uint32_t* key = nullptr;
+  uint32_t* order = nullptr;
+  uint32_t* count = nullptr;
+  uint64_t n_cap = 0, count_cap = 0;
+};
+
+// Particles worth ordering: enough to fill the persistent grid several times,
+// and few enough for 32-bit positions.
+constexpr uint64_t kOrderMinParticles = 1ull << 15;
+
 // Launchers (advect.hip / compact.hip).
 struct AdvectLaunch {
   const FieldView* f;
@@ -239,8 +252,16 @@ struct AdvectLaunch {
   float t0;
   int integrator;
   bool persistent;
+  const OrderScratch* order_scratch = nullptr;   // null: temporaries from the allocator
 };
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
+// Brick shape of the queue ordering of a field (order_kernels.hpp).
+void order_shape(const FieldView& f, uint32_t& shift, uint32_t& bits, uint32_t& keys,
+                 uint32_t& tiles);
+// Whether a launch orders its queue (and so needs order scratch).
+bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch& L);
+// The tuning a launch of this key uses (an autotuned entry or the context's).
+const Tuning& tuning_for(const dpah_context* ctx, const std::string& key);
 // Whether a launch reads the cell layout (when its field has one).
 bool use_cells(const Tuning& T, bool scattered);
 // Autotuning key of a launch: field shape, integrator, seed kind, record flag.

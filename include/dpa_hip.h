@@ -289,6 +289,23 @@ int dpah_server_destroy(dpah_server* server);
 /* FTLE volume of a DPAH_REQUEST_FTLE_AUX result (out may be NULL: count only). */
 int dpah_result_ftle(const dpah_result* r, float* out, uint64_t* count);
 
+/* A repeated endpoint pass captured into a hipGraph: the arguments of
+ * dpah_advect_endpoints_device (device pointers, fixed for the pass's life),
+ * captured once (counter reset, queue ordering, persistent advect kernel) and
+ * replayed by dpah_pass_launch on the context's stream with no host launch
+ * work and no allocation.  Each replay recomputes every output from the seeds.
+ * dpah_pass_total_steps waits and returns the last replay's particle-steps.
+ * No counterpart in dpa (one CPU pass per run). */
+typedef struct dpah_pass dpah_pass;
+int dpah_pass_create_endpoints(dpah_context* ctx, const dpah_field* field, const float* seeds,
+                               const uint64_t* remaining, uint64_t uniform_remaining,
+                               uint64_t count, float step_size, float start_time, int integrator,
+                               float* positions_out, uint64_t* remaining_out, uint8_t* status_out,
+                               dpah_pass** out);
+int dpah_pass_launch(dpah_pass* pass);
+int dpah_pass_total_steps(dpah_pass* pass, uint64_t* steps);
+int dpah_pass_destroy(dpah_pass* pass);
+
 /* Kernel timing recorded with HIP events on the context's stream (enabled by
  * dpah_options.kernel_timing): launches and summed milliseconds of the named
  * kernel ("advect_persistent", "advect_simple", "compact_write", ...). */

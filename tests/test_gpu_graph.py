@@ -1,0 +1,61 @@
+"""Endpoint passes captured into a hipGraph.
+
+Every replay of the captured pass must reproduce the eager call's outputs bit
+for bit (and the oracle's), with the program's own eager work and
+synchronisations between replays, for a pass that orders its queue (>= 2^15
+seeds) and one that does not.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import dpa_amd
+from dpa_amd import workloads as W
+from oracle import py_oracle as po
+from test_gpu_parity import bits
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n", [50000, 3000])
+def test_graph_replays_match_eager_and_oracle(ctx, n):
+    say(f"case graph pass n={n}")
+    field = W.vortex_field(64, count=24)
+    fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
+    seeds = W.uniform_seeds(n, (0, 0, 0), (63, 63, 63), seed_offset=321)
+    dt, iters = f32(0.25), 120
+    ora = po.advect_endpoints(po.Field(field, (0, 0, 0), (1, 1, 1)), seeds, iters, dt)
+    d_s = ctx.to_device(seeds)
+    d_p, d_r, d_t = ctx.alloc(12 * n), ctx.alloc(8 * n), ctx.alloc(n)
+    e_p, e_r, e_t = ctx.alloc(12 * n), ctx.alloc(8 * n), ctx.alloc(n)
+    try:
+        with ctx.pass_endpoints(fld, d_s, n, d_p, uniform_remaining=iters, step_size=dt,
+                                d_rem_out=d_r, d_status_out=d_t) as gp:
+            for rep in range(4):
+                # eager work between replays, on other buffers
+                steps_eager = ctx.advect_endpoints_device(fld, d_s, n, e_p, uniform_remaining=iters,
+                                                          step_size=dt, d_rem_out=e_r,
+                                                          d_status_out=e_t)
+                eager = (ctx.to_host(e_p, np.empty((n, 3), f32)),
+                         ctx.to_host(e_r, np.empty(n, np.uint64)), ctx.to_host(e_t, np.empty(n, np.uint8)))
+                ctx.memset_zero(d_p, 12 * n)     # the replay must rewrite every output
+                gp.launch()
+                steps = gp.total_steps()
+                got = (ctx.to_host(d_p, np.empty((n, 3), f32)),
+                       ctx.to_host(d_r, np.empty(n, np.uint64)), ctx.to_host(d_t, np.empty(n, np.uint8)))
+                assert steps == steps_eager == ora["total_steps"], (rep, steps, steps_eager)
+                assert np.array_equal(bits(got[0]), bits(eager[0]))
+                assert np.array_equal(bits(got[0]), bits(ora["position"]))
+                assert np.array_equal(got[1], ora["remaining"]) and np.array_equal(got[2], ora["status"])
+                assert np.array_equal(got[1], eager[1]) and np.array_equal(got[2], eager[2])
+    finally:
+        for p in (d_s, d_p, d_r, d_t, e_p, e_r, e_t):
+            ctx.free(p)
+        fld.release()
