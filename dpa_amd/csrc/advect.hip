@@ -80,9 +80,19 @@ void order_shape(const FieldView& f, uint32_t& shift, uint32_t& bits, uint32_t& 
   tiles = (keys + kOrderTile - 1) / kOrderTile;   // <= 129 for 6 bits
 }
 
+// Seed arrays and plain lattices.  A lattice in C order gives a wave 64 points
+// along one row; bricks give it a 4x4x4 block (dpa-mode FTLE on 512x512x128:
+// 16.8 ms advection ordered, 39.7 ms not, for 4 ms of sorting).  The auxiliary
+// lattice already gives a wave the six neighbours of ~11 adjacent points, and
+// it has six times the particles: sorting it cost 50 ms against a 66 ms
+// advection that it did not speed up.  Endpoint launches only: a recorded
+// launch writes one slot row per particle, and with the queue in seed order a
+// wave's rows are neighbours in memory (2^18 cfg2 seeds x 1000 recorded
+// steps: 4.0 ms in seed order, 5.1 ms ordered).
 bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch& L) {
   return L.persistent && T.order && (ctx->caching || L.order_scratch) && L.nseg == 0 &&
-         L.n >= kOrderMinParticles && L.n < (1ull << 32);
+         L.slots == nullptr && L.seed_kind != kSeedAux && L.n >= kOrderMinParticles &&
+         L.n < (1ull << 32);
 }
 
 namespace {

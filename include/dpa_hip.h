@@ -318,7 +318,10 @@ int dpah_kernel_timing_reset(dpah_context* ctx);
 typedef struct dpah_tuning {
   int block;           /* threads per workgroup: 64, 128, 192 or 256          */
   int blocks_per_cu;   /* persistent grid = CUs x blocks_per_cu, 1..32         */
-  int order;           /* 1: queue in brick Morton order, split across XCDs    */
+  int order;           /* 1: endpoint launches of seed arrays and lattices take
+                          their queue in brick Morton order, split across the
+                          XCDs (auxiliary FTLE lattices are already compact;
+                          recorded launches write rows in seed order)          */
   int parts;           /* queue parts, 1..8 (the XCD count by default)         */
   int refill;          /* idle lanes (1..64) that make a wave draw new particles:
                           1 keeps every lane busy, 64 keeps a wave's particles

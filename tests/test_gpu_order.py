@@ -78,8 +78,10 @@ def test_ordered_endpoints_match_oracle(ctx, vortex, tuning):
 
 
 def test_ordered_recorded_rounds_match_oracle(ctx, vortex):
+    # recorded launches keep seed order even with ordering on (their slot rows
+    # are written in seed order); the rounds must still match the oracle
     field, fld = vortex
-    say("case ordered recorded rounds")
+    say("case recorded rounds with ordering on")
     seeds = seeds_with_strays(40000, 43)
     dt, iters = f32(0.25), 40
     rem = np.full(len(seeds), iters, np.uint64)
@@ -102,3 +104,21 @@ def test_set_tuning_rejects_bad_shapes(ctx):
                 dict(parts=9), dict(parts=0)):
         with pytest.raises(dpa_amd.DpahError):
             ctx.set_tuning(**bad)
+
+
+def test_ordered_lattice_ftle_same_as_unordered(ctx, vortex):
+    # dpa-mode FTLE seeds a plain lattice (64^3 = 262144 points, ordered when
+    # ordering is on); the volume and the step count must not depend on it
+    field, fld = vortex
+    say("case lattice ftle ordered vs unordered")
+    out = []
+    before = ctx.tuning()
+    try:
+        for order in (1, 0):
+            ctx.set_tuning(order=order)
+            vol, steps = ctx.ftle_lattice(fld, (1, 1, 1), 60, f32(0.25))
+            out.append((vol, steps))
+    finally:
+        ctx.set_tuning(**before)
+    assert out[0][1] == out[1][1] and out[0][1] > 1000000
+    assert np.array_equal(bits(out[0][0]), bits(out[1][0]))
