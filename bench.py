@@ -64,28 +64,44 @@ def host_threads() -> int:
     return max(1, min(16, len(os.sched_getaffinity(0))))
 
 
-def cpu_baseline_endpoints(field, offset, spacing, seeds, iters, dt, target_s: float,
-                           what: str):
-    """Oracle (CPU port of dpa's loop) on a bounded sample of the same workload,
-    OpenMP schedule(dynamic,1) over the host's cores."""
-    from oracle import py_oracle as po
-    threads = host_threads()
-    fld = po.Field(field, offset, spacing)
+def _timed_sample(fn, fld, seeds, iters, dt, threads, target_s):
+    """Runs fn on the first seeds, sized from a 2048-seed probe so that the
+    timed run takes about target_s seconds."""
     probe = seeds[: 2048]
     t = time.perf_counter()
-    r = po.advect_endpoints(fld, probe, iters, dt, threads=threads)
+    r = fn(fld, probe, iters, dt, threads=threads)
     el = time.perf_counter() - t
     per_seed = max(r["total_steps"] / len(probe), 1e-9)
     rate = r["total_steps"] / max(el, 1e-9)
     count = int(min(len(seeds), max(2048, rate * target_s / per_seed)))
-    sample = seeds[: count]
     t = time.perf_counter()
-    r = po.advect_endpoints(fld, sample, iters, dt, threads=threads)
+    r = fn(fld, seeds[: count], iters, dt, threads=threads)
     el = time.perf_counter() - t
-    return {"value": r["total_steps"] / el, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
+    return r["total_steps"] / el, count, r["total_steps"], el
+
+
+def cpu_baseline_endpoints(field, offset, spacing, seeds, iters, dt, target_s: float,
+                           what: str):
+    """CPU restatements of dpa's loop on bounded samples of the same workload,
+    OpenMP schedule(dynamic,1) over the host's cores: the dpa-structured form
+    (a heap-allocated corner list and a recursive corner walk per
+    interpolation, as dpa's regular_grid::interpolate) is the baseline; the
+    plain form and a one-thread run are reported beside it."""
+    from oracle import py_oracle as po
+    threads = host_threads()
+    fld = po.Field(field, offset, spacing)
+    v, count, steps, el = _timed_sample(po.advect_endpoints_dpa_structured, fld, seeds, iters, dt,
+                                        threads, target_s * 0.5)
+    vp, cp, _, _ = _timed_sample(po.advect_endpoints, fld, seeds, iters, dt, threads,
+                                 target_s * 0.25)
+    v1, c1, _, _ = _timed_sample(po.advect_endpoints_dpa_structured, fld, seeds, iters, dt, 1,
+                                 target_s * 0.25)
+    return {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"first {count} of the {what} seeds, all {iters} iterations, same field "
-                      f"({r['total_steps']} particle-steps in {el:.2f} s; host {cpu_model()})"}
+                      f"({steps} particle-steps in {el:.2f} s; host {cpu_model()}); "
+                      "dpa-structured interpolation (heap corner list, recursive corner walk)",
+            "plain_port": {"value": vp, "cores": threads, "seeds": cp},
+            "one_thread": {"value": v1, "cores": 1, "seeds": c1}}
 
 
 def roofline(steps_per_launch, avg_ms, bytes_per_step=BYTES_PER_STEP, traffic=None):
@@ -277,10 +293,11 @@ def run_cfg2(args):
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (oracle on a bounded sample)")
+        log("cpu baseline (CPU restatements on bounded samples)")
         cpu = cpu_baseline_endpoints(field, (0, 0, 0), (1, 1, 1), seeds, args.iterations, dt,
                                      args.cpu_seconds, "2^20")
-        log(f"cpu: {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads")
+        log(f"cpu: dpa-structured {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads, "
+            f"plain {cpu['plain_port']['value']:.4e}, one thread {cpu['one_thread']['value']:.4e}")
 
     rf = roofline(steps_per_pass, avg_ms, traffic=read_traffic())
     rf["kernel"] = "k_advect_persistent"

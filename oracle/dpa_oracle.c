@@ -417,6 +417,103 @@ uint64_t dpo_advect_endpoints(const dpo_grid* g, float* pos, uint64_t* remaining
   return total;
 }
 
+/* ----------------------------------------- dpa-structured CPU variant ---- */
+/* The same arithmetic as dpo_interpolate, with the structure of dpa's
+ * regular_grid::interpolate (regular_grid.hpp:52-75): a heap-allocated corner
+ * list per call (the std::vector reserved at 65-66) filled by permute_for's
+ * recursive walk through a callback (permute_for.hpp:12-34, dimension 0
+ * outermost, one call per level and corner).  Used only as the CPU baseline
+ * that costs what dpa's loop costs; its results equal the oracle's bit for
+ * bit (tests/test_oracle.py). */
+
+typedef struct {
+  const dpo_grid* g;
+  float* list;
+  int count;
+  int dims;
+} corner_list;
+
+typedef void (*corner_fn)(void* ctx, const int64_t idx[4]);
+
+static void push_corner(void* c, const int64_t idx[4]) {
+  corner_list* cl = (corner_list*)c;
+  const float* v = grid_node(cl->g, idx[0], idx[1], idx[2], cl->dims == 4 ? idx[3] : 0);
+  float* o = cl->list + 3 * cl->count++;
+  o[0] = v[0]; o[1] = v[1]; o[2] = v[2];
+}
+
+static void permute_walk(int depth, int dims, int64_t idx[4], const int64_t start[4],
+                         const int64_t end[4], corner_fn fn, void* ctx) {
+  if (depth < dims) {
+    for (int64_t i = start[depth]; i < end[depth]; ++i) {
+      idx[depth] = i;
+      permute_walk(depth + 1, dims, idx, start, end, fn, ctx);
+    }
+  } else {
+    fn(ctx, idx);
+  }
+}
+
+static void interpolate_dpa_structured(const dpo_grid* g, const float p[3], float t, float out[3]) {
+  const int dims = grid_dims(g);
+  float w[4];
+  int64_t start[4] = {0, 0, 0, 0}, end[4] = {0, 0, 0, 0}, idx[4] = {0, 0, 0, 0};
+  for (int i = 0; i < dims; ++i) {
+    float o, h, x; int64_t n;
+    grid_coord(g, i, p, t, &o, &h, &n, &x);
+    const float d = x - o;
+    w[i] = fmodf(d, h) / h;
+    start[i] = (int64_t)floorf(d / h);
+    end[i] = start[i] + 2;
+  }
+  corner_list cl = {g, (float*)malloc(sizeof(float) * 3 * (1u << dims)), 0, dims};
+  permute_walk(0, dims, idx, start, end, push_corner, &cl);
+  float* L = cl.list;
+  for (int i = dims - 1; i >= 0; --i) {
+    const float wi = w[i];
+    const float om = 1.0f - wi;
+    for (int j = 0; j < (1 << i); ++j)
+      for (int e = 0; e < 3; ++e) {
+        const float a = om * L[3 * (2 * j) + e];
+        const float b = wi * L[3 * (2 * j + 1) + e];
+        L[3 * j + e] = a + b;
+      }
+  }
+  out[0] = L[0]; out[1] = L[1]; out[2] = L[2];
+  free(L);
+}
+
+/* dpo_advect_particle's loop (dpa mode) with the dpa-structured interpolation. */
+static uint64_t advect_particle_dpa_structured(const dpo_grid* g, float p[3], uint64_t* remaining,
+                                               float dt, float t0, uint8_t* status) {
+  uint64_t it = 0;
+  uint8_t st = DPO_ACTIVE;
+  for (; *remaining > 0; ++it, --*remaining) {
+    const float ft = (float)it * dt;
+    const float t  = t0 + ft;
+    if (!dpo_contains(g, p, t)) { st = DPO_EXITED; break; }
+    float v[3];
+    interpolate_dpa_structured(g, p, t, v);
+    if (dpo_is_zero(v)) { st = DPO_ZERO; break; }
+    dpo_step_dpa(p, v, dt);
+  }
+  if (*remaining == 0) st = DPO_COMPLETED;
+  *status = st;
+  return it;
+}
+
+uint64_t dpo_advect_endpoints_dpa_structured(const dpo_grid* g, float* pos, uint64_t* remaining,
+                                             uint8_t* status, uint64_t n, float dt, float t0,
+                                             int nthreads) {
+  const int nt = clamp_threads(nthreads);
+  uint64_t total = 0;
+  int64_t sn = (int64_t)n;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) reduction(+ : total)
+  for (int64_t i = 0; i < sn; ++i)
+    total += advect_particle_dpa_structured(g, pos + 3 * i, remaining + i, dt, t0, status + i);
+  return total;
+}
+
 /* ------------------------------------------------------------- seeds ---- */
 
 uint64_t dpo_lattice_seeds(const float offset[3], const float size[3],

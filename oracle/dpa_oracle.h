@@ -93,6 +93,11 @@ void            dpo_rounds_free           (dpo_rounds* r);
 uint64_t dpo_advect_endpoints(const dpo_grid* g, float* pos, uint64_t* remaining,
                               uint8_t* status, uint64_t n, float dt, float t0,
                               int integrator, int nthreads);
+/* The same in dpa's structure (heap corner list, recursive corner walk per
+ * interpolation), dpa mode only: a CPU baseline; equal results. */
+uint64_t dpo_advect_endpoints_dpa_structured(const dpo_grid* g, float* pos, uint64_t* remaining,
+                                             uint8_t* status, uint64_t n, float dt, float t0,
+                                             int nthreads);
 
 /* Literal post-processing stages on one round's vertex array. */
 uint64_t dpo_prune(float* vertices, uint64_t count);                     /* particle_advector.cpp:402-408 */

@@ -59,6 +59,8 @@ def load() -> C.CDLL:
         "dpo_rounds_free": (None, [_vp]),
         "dpo_advect_endpoints": (_u64, [G, _vp, _vp, _vp, _u64, C.c_float, C.c_float, C.c_int,
                                         C.c_int]),
+        "dpo_advect_endpoints_dpa_structured": (_u64, [G, _vp, _vp, _vp, _u64, C.c_float,
+                                                        C.c_float, C.c_int]),
         "dpo_prune": (_u64, [_vp, _u64]),
         "dpo_indices": (_u64, [_vp, _u64, _vp, C.c_int]),
         "dpo_colors": (None, [_vp, _u64, _vp]),
@@ -191,6 +193,21 @@ def advect_endpoints(f: Field, seeds, remaining, dt, t0=0.0, integrator=DPA_RK4,
     total = load().dpo_advect_endpoints(f.ref, _p(pos), _p(rem), _p(st), n,
                                         float(np.float32(dt)), float(np.float32(t0)), integrator,
                                         threads)
+    return dict(position=pos, remaining=rem, status=st, total_steps=int(total))
+
+
+def advect_endpoints_dpa_structured(f: Field, seeds, remaining, dt, t0=0.0, threads=0):
+    """dpa-mode endpoints through the dpa-structured interpolation (heap corner
+    list, recursive corner walk): the CPU baseline with dpa's per-step costs."""
+    seeds = np.ascontiguousarray(seeds, np.float32).reshape(-1, 3)
+    n = seeds.shape[0]
+    pos = seeds.copy()
+    rem = (np.full(n, remaining, np.uint64) if np.isscalar(remaining)
+           else np.ascontiguousarray(remaining, np.uint64).copy())
+    st = np.zeros(n, np.uint8)
+    total = load().dpo_advect_endpoints_dpa_structured(f.ref, _p(pos), _p(rem), _p(st), n,
+                                                       float(np.float32(dt)),
+                                                       float(np.float32(t0)), threads)
     return dict(position=pos, remaining=rem, status=st, total_steps=int(total))
 
 
