@@ -155,7 +155,7 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   AdvectArgs a{};
   a.f = *L.f;
   if (!use_cells(T, L.seed_kind == kSeedArray)) a.f.cells = nullptr;
-  if (!T.fastdiv) a.f.fastdiv = 0;
+  if (!T.fastdiv) a.f.fastdiv = a.f.exactdiv = 0;
   a.segs = L.segs;
   a.nseg = L.nseg;
   a.seed_kind = L.seed_kind;

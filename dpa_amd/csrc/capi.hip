@@ -330,9 +330,15 @@ static void set_division(dpah_context* ctx, FieldView& v) {
   const int32_t n[4] = {v.n0, v.n1, v.n2, v.nt};
   float r[4];
   v.fastdiv = 0;
+  v.exactdiv = 0;
   for (int a = 0; a < 4; ++a) {
     r[a] = 1.0f / h[a];
     if (n[a] < 2 || !ctx->tune.fastdiv) continue;
+    int e = 0;
+    if (std::frexp(h[a], &e) == 0.5f && std::isnormal(r[a])) {   // h = 2^(e-1), r exact
+      v.exactdiv |= 1u << a;
+      continue;
+    }
     const float B = std::nextafter(2.0f * (float)(n[a] - 1) * h[a], INFINITY);
     if (!(B < INFINITY)) continue;
     uint32_t hb, bb;

@@ -32,6 +32,10 @@ struct FieldView {
   // correctly rounded quotient for every float in the axis's range.
   float r0, r1, r2, rt;
   uint32_t fastdiv;
+  // Per axis (same bits): the spacing is a power of two, so its reciprocal is
+  // exact and d * r is the same real number as d / h -- the same rounding, for
+  // every d, with no check needed.
+  uint32_t exactdiv;
 };
 
 enum Status : uint8_t { kActive = 0, kCompleted = 1, kExited = 2, kZero = 3 };
@@ -74,8 +78,8 @@ __device__ __forceinline__ float div_by(float d, float h, float r) {
   return __builtin_fmaf(e, r, q0);
 }
 
-__device__ __forceinline__ float quotient(float d, float h, float r, bool fast) {
-  return fast ? div_by(d, h, r) : d / h;
+__device__ __forceinline__ float quotient(float d, float h, float r, bool fast, bool exact) {
+  return exact ? d * r : fast ? div_by(d, h, r) : d / h;
 }
 
 // Counts the floats x in [-B, B] (bit patterns 0..hi of |x|, both signs) where
@@ -95,14 +99,14 @@ static __global__ __launch_bounds__(256) void verify_division_kernel(float h, fl
 
 // regular_grid::contains for one axis (regular_grid.hpp:41-50) and the weight /
 // start index of regular_grid::interpolate (52-75).  Returns false outside.
-// `fast` is field-uniform (a scalar branch).
-__device__ __forceinline__ bool locate(float x, float o, float h, float rh, bool fast, int32_t n,
-                                       Axis& a) {
+// `fast` and `exact` are field-uniform (scalar branches).
+__device__ __forceinline__ bool locate(float x, float o, float h, float rh, bool fast, bool exact,
+                                       int32_t n, Axis& a) {
   const float d = x - o;
-  const float s = floorf(quotient(d, h, rh, fast));
+  const float s = floorf(quotient(d, h, rh, fast, exact));
   if (!(s >= 0.0f) || s >= (float)(n - 1)) return false;
   const float r = exact_fmod(d, h, s);
-  a.w = quotient(r, h, rh, fast);   // r in [0, h) or -0: inside the verified range
+  a.w = quotient(r, h, rh, fast, exact);   // r in [0, h) or -0: inside the verified range
   a.c = (int32_t)s;
   return true;
 }
@@ -234,12 +238,12 @@ template <bool kUnsteady>
 __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, float p2, float t,
                                        V3& v, CellCache& cc) {
   Axis a0, a1, a2;
-  if (!locate(p0, f.o0, f.h0, f.r0, f.fastdiv & 1u, f.n0, a0)) return false;
-  if (!locate(p1, f.o1, f.h1, f.r1, f.fastdiv & 2u, f.n1, a1)) return false;
-  if (!locate(p2, f.o2, f.h2, f.r2, f.fastdiv & 4u, f.n2, a2)) return false;
+  if (!locate(p0, f.o0, f.h0, f.r0, f.fastdiv & 1u, f.exactdiv & 1u, f.n0, a0)) return false;
+  if (!locate(p1, f.o1, f.h1, f.r1, f.fastdiv & 2u, f.exactdiv & 2u, f.n1, a1)) return false;
+  if (!locate(p2, f.o2, f.h2, f.r2, f.fastdiv & 4u, f.exactdiv & 4u, f.n2, a2)) return false;
   if constexpr (kUnsteady) {
     Axis at;
-    if (!locate(t, f.to, f.th, f.rt, f.fastdiv & 8u, f.nt, at)) return false;
+    if (!locate(t, f.to, f.th, f.rt, f.fastdiv & 8u, f.exactdiv & 8u, f.nt, at)) return false;
     v = quadlinear(f, a0, a1, a2, at, cc);
   } else {
     (void)t;

@@ -391,7 +391,7 @@ bool dpah_server::launch_ready() {
     bool scattered = true;   // FTLE requests seed lattices
     for (auto& rp : L.reqs) scattered &= rp->spec.kind == DPAH_REQUEST_ADVECT;
     if (!use_cells(ctx->tune, scattered)) a.f.cells = nullptr;
-    if (!ctx->tune.fastdiv) a.f.fastdiv = 0;
+    if (!ctx->tune.fastdiv) a.f.fastdiv = a.f.exactdiv = 0;
     a.segs = L.segs.as<TickSeg>();
     a.nseg = (uint32_t)segs.size();
     a.n = q;

@@ -39,5 +39,18 @@ int main() {
     std::printf("%.9g %llu\n", h, (unsigned long long)bad);
     total += bad;
   }
+  // power-of-two spacings: d * (1/h) is d / h exactly (quotient() takes it
+  // without any check); every pattern of [h/4, 4h] and the denormal range
+  for (float h : {1.0f, 0.5f, 2.0f, 0.25f, 4.0f, 0x1p-20f, 0x1p20f}) {
+    const float r = 1.0f / h;
+    uint64_t bad = 0;
+    auto exact = [&](float x) {
+      return (uint64_t)(bits(x * r) != bits(x / h)) + (bits(-x * r) != bits(-x / h));
+    };
+    for (uint32_t u = bits(0.25f * h); u <= bits(4.0f * h); ++u) bad += exact(from(u));
+    for (uint32_t u = 0; u < 0x00800000u; u += 7) bad += exact(from(u));
+    std::printf("%.9g %llu exact\n", h, (unsigned long long)bad);
+    total += bad;
+  }
   return total != 0;
 }

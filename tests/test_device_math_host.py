@@ -7,8 +7,9 @@
 * tests/native/div_check.cpp: the reciprocal division used on verified axes
   must equal the correctly rounded x / h bit for bit (all patterns around the
   switch to plain division and over [h/2, 2h], plus a sample of the whole
-  axis range) for power-of-two, decimal and 2*pi/(n-1) spacings.  The device
-  repeats this check exhaustively per field before it uses the fast form.
+  axis range) for power-of-two, decimal and 2*pi/(n-1) spacings, and the plain
+  reciprocal multiply of power-of-two spacings against x / h.  The device
+  repeats the first check exhaustively per field before it uses the fast form.
 """
 import os
 import shutil
@@ -43,4 +44,4 @@ def test_warm_cache_equals_full_reload(tmp_path):
 def test_reciprocal_division_is_correctly_rounded(tmp_path):
     lines = build_and_run(tmp_path, "div_check").split("\n")
     rows = [l.split() for l in lines if l.strip()]
-    assert len(rows) == 10 and all(r[1] == "0" for r in rows), lines
+    assert len(rows) == 17 and all(r[1] == "0" for r in rows), lines
