@@ -59,9 +59,14 @@ struct QueueOrder {
   DeviceBuffer key, count, order;
 };
 
+// Queue parts start on multiples of 64, so waves that draw 64 positions at a
+// time draw whole aligned tiles of record slots.
 void set_parts(AdvectArgs& a, uint32_t parts) {
   a.parts = parts;
-  for (uint32_t p = 0; p <= parts; ++p) a.part_lo[p] = a.n / parts * p + std::min<uint64_t>(p, a.n % parts);
+  const uint64_t tiles = (a.n + kTileRows - 1) / kTileRows;
+  for (uint32_t p = 0; p <= parts; ++p)
+    a.part_lo[p] = std::min(a.n, (tiles / parts * p + std::min<uint64_t>(p, tiles % parts)) *
+                                     kTileRows);
 }
 
 }  // namespace
@@ -177,7 +182,11 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   a.total_steps = ctx->d_counters + 1;
   a.order = nullptr;
   set_parts(a, 1);
-  a.refill = (uint32_t)(T.refill ? T.refill : (L.seed_kind == kSeedArray ? 16 : 64));
+  // auto: 16 for endpoint launches of seed arrays (keeps lanes busy); 64 for
+  // lattices (keeps neighbouring particles together) and for recorded launches
+  // (a wave's 64 lanes then write one aligned tile of slots per step)
+  a.refill = (uint32_t)(T.refill ? T.refill
+                                 : (L.seed_kind == kSeedArray && !L.slots ? 16 : 64));
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;

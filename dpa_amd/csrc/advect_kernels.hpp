@@ -28,10 +28,27 @@ struct SeedLattice {
   uint64_t first;             // lattice index of particle / point 0 (chunked launches)
 };
 
+// Record slots are tiled by 64 rows: slot j of row r lives at float
+// 3 * ((r / 64 * stride + j) * 64 + r % 64).  The 64 lanes of a wave that hold
+// rows 64t .. 64t+63 at the same step write one contiguous 768-byte run (six
+// whole lines) instead of 64 partial lines of 64 rows, and a row's slots are
+// kSlotPitch floats apart.  Slot buffers hold whole tiles (tiled_rows rows).
+constexpr uint64_t kTileRows = 64;
+constexpr uint64_t kSlotPitch = 3 * kTileRows;
+
+__host__ __device__ inline uint64_t tiled_rows(uint64_t rows) {
+  return (rows + kTileRows - 1) / kTileRows * kTileRows;
+}
+
+__host__ __device__ inline uint64_t slot_float(uint64_t row, uint64_t stride, uint64_t j) {
+  return 3 * ((row / kTileRows * stride + j) * kTileRows + row % kTileRows);
+}
+
 // A contiguous run of particles with one slot layout: one dpa round of one job.
-// The queue hands out ordinals q; segment s owns [qbase, qbase + count) and maps
-// ordinal q to particle first + (q - qbase), whose slot row starts at
-// slots + slot_base + (q - qbase) * stride * 3.
+// The queue hands out ordinals q; segment s owns [qbase, qbase + tiled_rows(count))
+// (qbase a multiple of 64, so a wave never spans two segments) and maps ordinal
+// q < qbase + count to particle first + (q - qbase), whose slot row is row
+// q - qbase of the tiled region at slots + slot_base.
 struct Segment {
   uint64_t qbase;
   uint64_t first;
@@ -54,7 +71,7 @@ struct AdvectArgs {
   uint8_t*        status_out;   // [n] or null
   uint32_t*       steps_out;    // [n] or null
   uint8_t*        special_out;  // [n] or null (record): a vertex equals a sentinel
-  float*          slots;        // record: [n][stride][3], slot 0 = seed
+  float*          slots;        // record: rows tiled by 64 (slot_float), slot 0 = seed
   uint64_t        stride;
   uint64_t        n;
   float           t0;
@@ -131,10 +148,11 @@ __device__ __forceinline__ void seed_position(const AdvectArgs& a, uint64_t pid,
   }
 }
 
-// Queue ordinal -> particle index and slot row (binary search over segments).
+// Queue ordinal -> particle index and slot row (binary search over segments);
+// ~0 for the padding ordinals at the end of a segment's last tile.
 __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, float*& row) {
   if (a.nseg == 0) {
-    row = a.slots + 3 * q * a.stride;
+    row = a.slots + slot_float(q, a.stride, 0);
     return q;
   }
   uint32_t lo = 0, hi = a.nseg;            // last segment with qbase <= q
@@ -144,7 +162,8 @@ __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, flo
   }
   const Segment& s = a.segs[lo];
   const uint64_t local = q - s.qbase;
-  row = a.slots + s.slot_base + 3 * local * s.stride;
+  if (local >= s.count) return ~0ull;
+  row = a.slots + s.slot_base + slot_float(local, s.stride, 0);
   return s.first + local;
 }
 
@@ -153,6 +172,7 @@ __device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uin
   float* row = nullptr;
   const uint64_t pid = resolve(a, q, row);
   L.pid = pid;
+  if (pid == ~0ull) return;   // segment padding: the lane stays idle
   cache_clear(L.cc);
   seed_position(a, pid, L.p0, L.p1, L.p2);
   L.rem = a.rem_in ? a.rem_in[pid] : a.uniform_rem;
@@ -168,7 +188,7 @@ template <bool kRecord>
 __device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const Lane& L, uint8_t st) {
   const uint64_t pid = L.pid;
   if constexpr (kRecord) {
-    put3(L.row + 3 * ((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
+    put3(L.row + kSlotPitch * ((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
     if (a.special_out) a.special_out[pid] = L.special ? 1 : 0;
   }
   put3(a.pos_out + 3 * pid, L.p0, L.p1, L.p2);
@@ -201,7 +221,7 @@ __device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef&
   L.rem -= 1;
   L.it += 1;
   if constexpr (kRecord) {
-    put3(L.row + 3 * (uint64_t)L.it, L.p0, L.p1, L.p2);
+    put3(L.row + kSlotPitch * (uint64_t)L.it, L.p0, L.p1, L.p2);
     L.special |= is_sentinel(L.p0, L.p1, L.p2);
   }
   return L.rem == 0 ? kCompleted : kActive;
@@ -220,11 +240,13 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
   if (pid < a.n) {
     Lane L;
     begin_particle<kRecord>(a, L, pid);
-    uint8_t st;
-    do {
-      st = step_once<kRecord, kIntegrator, kUnsteady>(a, L);
-    } while (st == kActive);
-    mine = finish_particle<kRecord>(a, L, st);
+    if (L.pid != ~0ull) {
+      uint8_t st;
+      do {
+        st = step_once<kRecord, kIntegrator, kUnsteady>(a, L);
+      } while (st == kActive);
+      mine = finish_particle<kRecord>(a, L, st);
+    }
   }
   add_wave_steps(a.total_steps, mine);
 }

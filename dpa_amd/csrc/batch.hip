@@ -106,8 +106,8 @@ void run_group(dpah_context* ctx, const dpah_job* jobs, const std::vector<uint64
     segs[s].count = r.count;
     segs[s].stride = record ? r.stride : 0;
     segs[s].slot_base = slot_floats;
-    q += r.count;
-    if (record) slot_floats += 3 * r.count * r.stride;
+    q += tiled_rows(r.count);   // segments start on 64-ordinal boundaries (whole waves)
+    if (record) slot_floats += 3 * tiled_rows(r.count) * r.stride;
   }
 
   const uint64_t n1 = std::max<uint64_t>(N, 1);

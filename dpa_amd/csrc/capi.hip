@@ -186,6 +186,7 @@ DPAH_EXPORT int dpah_create(int device, const dpah_options* options, dpah_contex
       if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, device) != hipSuccess ||
           xcc < 1)
         xcc = 1;
+      ctx->xcds = xcc;
       ctx->tune.parts = std::min(xcc, dpah::kMaxParts);
       size_t free_b = 0, total_b = 0;
       DPAH_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -548,10 +549,12 @@ dpah_result* dpah::run_advect(dpah_context* ctx, const dpah_field* field, const 
         const uint64_t stride = max_rem + 2;   // compute_round_state (305)
         Round round;
         if (rc) {
+          // chunks of whole 64-row tiles (the slot layout), within the slot budget
           const uint64_t row_bytes = stride * 3 * sizeof(float);
-          uint64_t chunk = std::max<uint64_t>(1, ctx->slot_budget / row_bytes);
-          chunk = std::min(chunk, rc);
-          DeviceBuffer slots(ctx, chunk * row_bytes);
+          uint64_t chunk = std::max<uint64_t>(1, ctx->slot_budget / row_bytes) / kTileRows *
+                           kTileRows;
+          chunk = std::min(std::max<uint64_t>(chunk, kTileRows), rc);
+          DeviceBuffer slots(ctx, tiled_rows(chunk) * row_bytes);
           for (uint64_t c0 = 0; c0 < rc; c0 += chunk) {
             const uint64_t cn = std::min(chunk, rc - c0);
             const uint64_t pid0 = first + c0;

@@ -1,23 +1,24 @@
 // compact_kernels.hpp -- K_compact: prune_integral_curves
 // (particle_advector.cpp:402-408), index_generator::generate
 // (index_generator.cpp:10-39) and color_generator::generate_from_angular_velocities
-// (color_generator.cpp:37-58) fused over one chunk of a round's strided slots.
+// (color_generator.cpp:37-58) fused over one chunk of a round's slots.
 //
-// Slot row p (stride s) holds particle p's seed, one vertex per step and the
-// terminal vertex at k_p + 1; rows are never pre-filled, so the kernels read
-// only slots [0, k_p + 1].  For a row with no sentinel-valued vertex the pruned
-// curve is the whole row, it contributes k_p + 2 vertices and k_p segments
+// Slot row p (stride s, rows tiled by 64: slot_float in advect_kernels.hpp)
+// holds particle p's seed, one vertex per step and the terminal vertex at
+// k_p + 1; rows are never pre-filled, so the kernels read only slots
+// [0, k_p + 1].  For a row with no sentinel-valued vertex the pruned curve is
+// the whole row, it contributes k_p + 2 vertices and k_p segments
 // (b + j, b + j + 1), j < k_p.  A row holding a vertex equal to (-1,-1,-1) or
 // (-2,-2,-2) -- dpa would drop / treat it as terminal -- is walked serially with
 // the literal rules.
 #pragma once
 
-#include "device_math.hpp"
+#include "advect_kernels.hpp"
 
 namespace dpah {
 
 struct CompactArgs {
-  const float*    slots;     // [count][stride][3]
+  const float*    slots;     // rows tiled by 64 (slot_float), whole tiles
   uint64_t        stride;
   const uint32_t* steps;     // k_p
   const uint8_t*  special;   // row holds a sentinel-valued vertex
@@ -29,6 +30,9 @@ struct CompactArgs {
   void*           indices;   // [2 I] u32 or u64
   int             use64;
   uint64_t        vertex_base;  // round-level index of this chunk's first vertex
+  uint32_t        tiles;        // ceil(count / 64)
+  uint32_t        spans;        // slot spans per tile
+  uint32_t        xcds;         // workgroups go to the XCDs round-robin by id
 };
 
 __device__ __forceinline__ bool eq3(const float* v, float s) {
@@ -84,11 +88,11 @@ static __global__ __launch_bounds__(256) void k_compact_count(CompactArgs a) {
     a.icount[p] = k;
     return;
   }
-  const float* row = a.slots + 3 * p * a.stride;
+  const float* row = a.slots + slot_float(p, a.stride, 0);
   uint64_t kept = 0, pairs = 0;
   bool have_prev = false, prev_term = false;
   for (uint64_t j = 0; j <= k + 1; ++j) {
-    const float* v = row + 3 * j;
+    const float* v = row + kSlotPitch * j;
     if (eq3(v, -1.0f)) continue;
     const bool term = eq3(v, -2.0f);
     if (have_prev && !prev_term && !term) ++pairs;
@@ -104,18 +108,18 @@ static __global__ __launch_bounds__(256) void k_compact_count(CompactArgs a) {
 // index and colour rules on the row's kept vertices.
 static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
   const uint64_t k = a.steps[p];
-  const float* row = a.slots + 3 * p * a.stride;
+  const float* row = a.slots + slot_float(p, a.stride, 0);
   const uint64_t vb = a.vcount[p], ib = a.icount[p];
   // list of kept slot indices is walked twice with a small lookahead
   uint64_t m = 0, pairs = 0;
   int64_t prev_j = -1;
   for (uint64_t j = 0; j <= k + 1; ++j) {
-    const float* v = row + 3 * j;
+    const float* v = row + kSlotPitch * j;
     if (eq3(v, -1.0f)) continue;
     // next kept slot
     int64_t next_j = -1;
     for (uint64_t q = j + 1; q <= k + 1; ++q)
-      if (!eq3(row + 3 * q, -1.0f)) { next_j = (int64_t)q; break; }
+      if (!eq3(row + kSlotPitch * q, -1.0f)) { next_j = (int64_t)q; break; }
     const uint64_t g = a.vertex_base + vb + m;
     float* out = a.vertices + 3 * (vb + m);
     out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
@@ -123,12 +127,12 @@ static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
     const bool term = eq3(v, -2.0f);
     if (!term) {
       float prev[3] = {0.0f, 0.0f, 0.0f}, next[3] = {0.0f, 0.0f, 0.0f};
-      if (g > 1 && prev_j >= 0 && !eq3(row + 3 * prev_j, -2.0f)) {
-        const float* u = row + 3 * prev_j;
+      if (g > 1 && prev_j >= 0 && !eq3(row + kSlotPitch * prev_j, -2.0f)) {
+        const float* u = row + kSlotPitch * prev_j;
         normalized3(v[0] - u[0], v[1] - u[1], v[2] - u[2], prev);
       }
-      if (next_j >= 0 && !eq3(row + 3 * next_j, -2.0f)) {
-        const float* u = row + 3 * next_j;
+      if (next_j >= 0 && !eq3(row + kSlotPitch * next_j, -2.0f)) {
+        const float* u = row + kSlotPitch * next_j;
         normalized3(u[0] - v[0], u[1] - v[1], u[2] - v[2], next);
         put_index(a, ib + pairs, g);
         ++pairs;
@@ -141,39 +145,147 @@ static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
   }
 }
 
-// Writes: one wave per row, lanes stride over the row's slots.
+// Writes: one workgroup (four waves) per tile of 64 rows and span of
+// kCompactSpan slots, walked in chunks of kCompactChunk vertices.  Workgroup
+// ids map so that the spans of one tile run one after another on one XCD:
+// the cache lines that two spans of a row share are completed in one L2, and
+// the workgroups in flight at once cover few tiles (few pages).  A chunk's slots [j0 - 1, j0 + kCompactChunk] of the
+// tile's rows are staged in LDS row by row (row pitch kBandRow floats, odd,
+// so 32 lanes reading consecutive slots of one row hit 32 different banks).
+// The rows that still have vertices in the chunk are listed, and each
+// half-wave takes one listed row: lane i computes the direction
+// d = normalized(x[j0+i] - x[j0+i-1]), which is the previous-segment
+// direction of vertex j0 + i and, fetched from lane i + 1, the next-segment
+// direction of vertex j0 + i - 1 (the same expression on the same operands,
+// so each direction is computed once).  Lanes 0..30 emit vertices
+// j0 .. j0 + 30: every store instruction writes runs of consecutive
+// vertices, colours and index pairs.  Rows with sentinel-valued vertices are
+// walked serially by the tile's first span, one thread each.
+constexpr uint32_t kCompactChunk = 31;
+typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+// Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format,
+// no swizzle; loads beyond the resource's byte size return zero.
+constexpr int kBufferWord3 = 0x00020000;
+constexpr uint32_t kCompactSpan = 4 * kCompactChunk;
+constexpr uint32_t kBandRow = 3 * (kCompactChunk + 2);   // 33 staged slots; 99 floats, odd
+
 static __global__ __launch_bounds__(256) void k_compact_write(CompactArgs a) {
-  const uint64_t p = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t lane = threadIdx.x & 63;
-  if (p >= a.count) return;
-  if (a.special[p]) {
-    if (lane == 0) compact_special_row(a, p);
-    return;
+  __shared__ float buf[kTileRows * kBandRow];
+  __shared__ uint32_t len[kTileRows];     // k + 2 slots to emit, 0: none (absent or special)
+  __shared__ uint64_t vbase[kTileRows], ibase[kTileRows];
+  __shared__ uint32_t list[kTileRows];    // rows with vertices in the current chunk
+  __shared__ uint32_t nlist;
+  const uint32_t t = threadIdx.x;
+  const uint32_t xcd = blockIdx.x % a.xcds, local = blockIdx.x / a.xcds;
+  const uint32_t tile = (local / a.spans) * a.xcds + xcd, span = local % a.spans;
+  if (tile >= a.tiles) return;   // padding of the last group (whole workgroup)
+  const uint64_t row0 = (uint64_t)tile * kTileRows;
+  if (t < kTileRows) {
+    const uint64_t r = row0 + t;
+    const bool live = r < a.count && !a.special[r];
+    len[t] = live ? a.steps[r] + 2 : 0u;
+    vbase[t] = live ? a.vcount[r] : 0;
+    ibase[t] = live ? a.icount[r] : 0;
   }
-  const uint64_t k = a.steps[p];
-  const float* row = a.slots + 3 * p * a.stride;
-  const uint64_t vb = a.vcount[p], ib = a.icount[p];
-  for (uint64_t j = lane; j <= k + 1; j += 64) {
-    const float* v = row + 3 * j;
-    const float x = v[0], y = v[1], z = v[2];
-    float* out = a.vertices + 3 * (vb + j);
-    out[0] = x; out[1] = y; out[2] = z;
-    float col = 0.0f;
-    if (j <= k) {
-      const uint64_t g = a.vertex_base + vb + j;
-      float prev[3] = {0.0f, 0.0f, 0.0f}, next[3] = {0.0f, 0.0f, 0.0f};
-      if (j >= 1 && g > 1) {
-        const float* u = v - 3;
-        normalized3(x - u[0], y - u[1], z - u[2], prev);
+  __syncthreads();
+  uint32_t nmax = 0;
+  for (uint32_t i = 0; i < kTileRows; ++i) nmax = max(nmax, len[i]);
+  nmax = __builtin_amdgcn_readfirstlane(nmax);   // same in every lane; keeps the loop scalar
+  const float* band = a.slots + slot_float(row0, a.stride, 0);   // slot j of row l: 3 (64 j + l)
+  const uint32_t wave = t >> 6, lane = t & 63, half = lane >> 5, i = lane & 31;
+  const uint32_t jb = span * kCompactSpan, je = min(nmax, jb + kCompactSpan);
+  for (uint32_t j0 = jb; j0 < je; j0 += kCompactChunk) {
+    // Thread t stages row t % 64, slots lo + t / 64, lo + t / 64 + 4, ...; a
+    // wave's load covers one slot of all 64 rows (768 contiguous bytes).  Only
+    // slots below the row's length are read: they are all a row ever uses.
+    // All of a thread's loads are issued before its first LDS write, so a
+    // chunk waits for one memory latency, not one per slot: they are buffer
+    // loads over the chunk's slots [lo, hi], and a slot past the row's end gets
+    // an offset beyond the buffer's size, which the hardware answers with
+    // zeros without touching memory.  Slot s goes to staged slot s - (j0 - 1).
+    const uint32_t lo = j0 == 0 ? 0u : j0 - 1;
+    const uint32_t hi = min(j0 + kCompactChunk, nmax - 1);
+    {
+      constexpr uint32_t kPer = (kCompactChunk + 2 + 3) / 4;   // slots per thread
+      const uint32_t l = t & 63, n = min(len[l], hi + 1);
+      // base and size are the same in every lane; readfirstlane says so, so the
+      // resource sits in scalar registers (readfirstlane returns int: each half
+      // goes through uint32_t before widening, or bit 31 of the low half would
+      // be sign-extended into the high half)
+      const uint32_t nbytes = (uint32_t)__builtin_amdgcn_readfirstlane(
+          (hi - lo + 1) * (uint32_t)kSlotPitch * 4u);   // <= 33 * 768
+      const uint64_t base = (uint64_t)(band + (uint64_t)lo * kSlotPitch);
+      const uint32_t base_lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
+      const uint32_t base_hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+      const uint64_t ubase = ((uint64_t)base_hi << 32) | (uint64_t)base_lo;
+      const __amdgpu_buffer_rsrc_t rs =
+          __builtin_amdgcn_make_buffer_rsrc((void*)ubase, (short)0, (int)nbytes, kBufferWord3);
+      float* dst = buf + l * kBandRow + 3 * (lo + 1 - j0);
+      u32x3 r[kPer];
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t s = lo + (t >> 6) + 4 * q;
+        const uint32_t off = s < n ? ((s - lo) * (uint32_t)kSlotPitch + 3 * l) * 4u : 0x80000000u;
+        r[q] = __builtin_amdgcn_raw_buffer_load_b96(rs, (int)off, 0, 0);
       }
-      if (j + 1 <= k) {
-        const float* u = v + 3;
-        normalized3(u[0] - x, u[1] - y, u[2] - z, next);
-        put_index(a, ib + j, g);
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t s = lo + (t >> 6) + 4 * q;
+        if (s < n) {
+          float* d = dst + 3 * (s - lo);
+          d[0] = __uint_as_float(r[q].x); d[1] = __uint_as_float(r[q].y);
+          d[2] = __uint_as_float(r[q].z);
+        }
       }
-      col = angle_color(prev, next);
     }
-    a.colors[vb + j] = col;
+    if (wave == 0) {   // list the rows that have vertex j0
+      const bool on = len[lane] > j0;
+      const uint64_t m = __ballot(on);
+      const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      if (on) list[pos] = lane;
+      if (lane == 0) nlist = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    const uint32_t nl = __builtin_amdgcn_readfirstlane(nlist);
+    for (uint32_t base = 0; base < nl; base += 8) {
+      const uint32_t idx = base + 2 * wave + half;
+      const bool on = idx < nl;
+      const uint32_t l = on ? list[idx] : 0u;
+      const uint32_t n = on ? len[l] : 0u;   // vertices j < n; colours and directions j <= n - 2
+      const uint32_t j = j0 + i;             // this lane's vertex
+      const float* v = buf + l * kBandRow + 3 * (i + 1);
+      float d[3] = {0.0f, 0.0f, 0.0f};
+      if (on && j >= 1 && j + 2 <= n) {      // 1 <= j <= k
+        const float* u = v - 3;
+        normalized3(v[0] - u[0], v[1] - u[1], v[2] - u[2], d);
+      }
+      const float n0 = __shfl_down(d[0], 1, 32), n1 = __shfl_down(d[1], 1, 32),
+                  n2 = __shfl_down(d[2], 1, 32);
+      if (on && i < kCompactChunk && j < n) {
+        const uint64_t vb = vbase[l], ib = ibase[l];
+        const float x = v[0], y = v[1], z = v[2];
+        float* out = a.vertices + 3 * (vb + j);
+        out[0] = x; out[1] = y; out[2] = z;
+        float col = 0.0f;
+        if (j + 2 <= n) {                    // j <= k
+          const uint64_t g = a.vertex_base + vb + j;
+          float prev[3] = {0.0f, 0.0f, 0.0f}, next[3] = {0.0f, 0.0f, 0.0f};
+          if (j >= 1 && g > 1) { prev[0] = d[0]; prev[1] = d[1]; prev[2] = d[2]; }
+          if (j + 3 <= n) {                  // j + 1 <= k
+            next[0] = n0; next[1] = n1; next[2] = n2;
+            put_index(a, ib + j, g);
+          }
+          col = angle_color(prev, next);
+        }
+        a.colors[vb + j] = col;
+      }
+    }
+    __syncthreads();
+  }
+  if (span == 0 && t < kTileRows) {
+    const uint64_t r = row0 + t;
+    if (r < a.count && a.special[r]) compact_special_row(a, r);
   }
 }
 

@@ -76,9 +76,9 @@ struct Tuning {
   int blocks_per_cu = 8;    // persistent grid = CUs * blocks_per_cu
   int order = 1;            // 1: spatially ordered queue split across the XCDs
   int parts = 8;            // queue parts (XCDs on the device)
-  // idle lanes that make a wave draw new particles (1..64); 0: 16 for seed
-  // arrays (keeps lanes busy) and 64 for lattice seeds (keeps a wave's
-  // neighbouring particles together)
+  // idle lanes that make a wave draw new particles (1..64); 0: 16 for endpoint
+  // launches of seed arrays (keeps lanes busy), 64 for lattices (keeps a wave's
+  // neighbouring particles together) and recorded launches (aligned slot tiles)
   int refill = 0;
   // 0: node layout only; 1: fields also get a cell layout when it fits, read by
   // launches of seed arrays (scattered particles), while lattice seeds, whose
@@ -100,6 +100,7 @@ struct dpah_context {
   int device = 0;
   hipStream_t stream = nullptr;
   int cu_count = 0;
+  int xcds = 1;             // XCDs (accelerator complexes) on the device
   uint64_t memory_budget = 0;
   uint64_t slot_budget = 0;
   std::atomic<uint64_t> bytes_in_use{0};     // live blocks (rounded sizes)

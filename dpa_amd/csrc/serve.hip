@@ -165,7 +165,7 @@ uint64_t footprint(const Request& r) {
       const uint64_t rc = std::min(ppr, active), first = active - rc;
       uint64_t mx = 0;
       for (uint64_t i = 0; i < rc; ++i) mx = std::max(mx, r.rem[first + i]);
-      slots += rc * (mx + 2) * 12;
+      slots += tiled_rows(rc) * (mx + 2) * 12;
       active = first;
     } while (active > 0);
     b += slots + verts * 16 + verts * 2 * ib;
@@ -286,7 +286,8 @@ void dpah_server::setup(Request& r, hipStream_t s) {
     uint64_t mx = 0;
     for (uint64_t i = 0; i < rs.count && !ftle; ++i) mx = std::max(mx, r.rem[rs.first + i]);
     rs.stride = mx + 2;
-    if (record && rs.count) rs.slots = DeviceBuffer(ctx, rs.count * rs.stride * 3 * sizeof(float));
+    if (record && rs.count)
+      rs.slots = DeviceBuffer(ctx, tiled_rows(rs.count) * rs.stride * 3 * sizeof(float));
     r.rounds.push_back(std::move(rs));
     active = r.rounds.back().first;
   } while (active > 0);
