@@ -58,18 +58,18 @@ Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const
   a.vertices = ch.vertices;
   a.colors = ch.colors;
   a.indices = ch.indices;
-  // one workgroup per tile of 64 rows and span of slots (a row has at most
-  // `stride` slots); tiles padded to a multiple of the XCD count
+  // one workgroup per group of 8 rows of a 64-row tile and span of slots (a
+  // row has at most `stride` slots); tiles padded to a multiple of the XCD count
   const uint64_t tiles = (count + kTileRows - 1) / kTileRows;
   const uint64_t spans = (stride + kCompactSpan - 1) / kCompactSpan;
   const uint64_t xcds = (uint64_t)std::max(1, ctx->xcds);
-  const uint64_t groups = (tiles + xcds - 1) / xcds * xcds * spans;
+  const uint64_t groups = (tiles + xcds - 1) / xcds * xcds * spans * kGroupsPerTile;
   DPAH_REQUIRE(groups < (1ull << 31), "too many rows or slots in one compaction");
   a.tiles = (uint32_t)tiles;
   a.spans = (uint32_t)spans;
   a.xcds = (uint32_t)xcds;
   ctx->timed("compact_write", st, [&] {
-    hipLaunchKernelGGL(k_compact_write, dim3((unsigned)groups), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_compact_write, dim3((unsigned)groups), dim3(kCompactThreads), 0, st, a);
   });
   DPAH_HIP(hipGetLastError());
   sync();

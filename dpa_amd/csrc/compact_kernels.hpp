@@ -145,42 +145,50 @@ static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
   }
 }
 
-// Writes: one workgroup (four waves) per tile of 64 rows and span of
-// kCompactSpan slots, walked in chunks of kCompactChunk vertices.  Workgroup
-// ids map so that the spans of one tile run one after another on one XCD:
-// the cache lines that two spans of a row share are completed in one L2, and
-// the workgroups in flight at once cover few tiles (few pages).  A chunk's slots [j0 - 1, j0 + kCompactChunk] of the
-// tile's rows are staged in LDS row by row (row pitch kBandRow floats, odd,
-// so 32 lanes reading consecutive slots of one row hit 32 different banks).
-// The rows that still have vertices in the chunk are listed, and each
-// half-wave takes one listed row: lane i computes the direction
-// d = normalized(x[j0+i] - x[j0+i-1]), which is the previous-segment
-// direction of vertex j0 + i and, fetched from lane i + 1, the next-segment
-// direction of vertex j0 + i - 1 (the same expression on the same operands,
-// so each direction is computed once).  Lanes 0..30 emit vertices
-// j0 .. j0 + 30: every store instruction writes runs of consecutive
-// vertices, colours and index pairs.  Rows with sentinel-valued vertices are
-// walked serially by the tile's first span, one thread each.
-constexpr uint32_t kCompactChunk = 31;
+// Writes: one workgroup (four waves) per group of kGroupRows = 4 rows of a
+// 64-row tile and span of kCompactSpan slots, walked in chunks of
+// kCompactChunk = 63 vertices; wave h owns row h of the group.  Few rows per
+// workgroup keep few output runs open at once (3 streams per row): on
+// 2^18 x 1000 recorded rows, 64 rows per workgroup left so many lines partly
+// written that they outgrew L2 (stores at 2.6 TB/s, compaction 2.2 ms); 8
+// rows: 1.74 ms; 4 rows: 1.66 ms; 2 rows: 1.83 ms.  Workgroup ids map so that
+// the groups of a span run together, and the spans of a tile one after
+// another, on one XCD: the groups share the slot lines of their span in L2,
+// and the lines that two spans of a row share are completed in one L2.
+//
+// A chunk's slots [j0 - 1, j0 + kCompactChunk] of the group's rows are staged
+// in LDS row by row (row pitch kBandRow floats, odd, so lanes reading
+// consecutive slots of one row hit different banks).  Lane i computes the
+// direction d = normalized(x[j0+i] - x[j0+i-1]), which is the
+// previous-segment direction of vertex j0 + i and, fetched from lane i + 1,
+// the next-segment direction of vertex j0 + i - 1 (the same expression on
+// the same operands, so each direction is computed once).  Lanes 0..62 emit
+// vertices j0 .. j0 + 62.  Rows with sentinel-valued vertices are walked
+// serially by the span-0 workgroup of their group, one thread each.
+constexpr uint32_t kRunLanes = 64;                     // lanes per row (a wave)
+constexpr uint32_t kCompactThreads = 256;              // workgroup size
+constexpr uint32_t kGroupRows = kCompactThreads / kRunLanes;   // rows per workgroup
+constexpr uint32_t kCompactChunk = kRunLanes - 1;
+constexpr uint32_t kCompactSpan = 4 * kCompactChunk;
+constexpr uint32_t kGroupsPerTile = kTileRows / kGroupRows;
+constexpr uint32_t kBandRow = 3 * (kCompactChunk + 2);   // staged slots x 3 floats, odd
 typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
 // Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format,
 // no swizzle; loads beyond the resource's byte size return zero.
 constexpr int kBufferWord3 = 0x00020000;
-constexpr uint32_t kCompactSpan = 4 * kCompactChunk;
-constexpr uint32_t kBandRow = 3 * (kCompactChunk + 2);   // 33 staged slots; 99 floats, odd
 
-static __global__ __launch_bounds__(256) void k_compact_write(CompactArgs a) {
-  __shared__ float buf[kTileRows * kBandRow];
-  __shared__ uint32_t len[kTileRows];     // k + 2 slots to emit, 0: none (absent or special)
-  __shared__ uint64_t vbase[kTileRows], ibase[kTileRows];
-  __shared__ uint32_t list[kTileRows];    // rows with vertices in the current chunk
-  __shared__ uint32_t nlist;
+static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(CompactArgs a) {
+  __shared__ float buf[kGroupRows * kBandRow];
+  __shared__ uint32_t len[kGroupRows];   // k + 2 slots to emit, 0: none (absent or special)
+  __shared__ uint64_t vbase[kGroupRows], ibase[kGroupRows];
   const uint32_t t = threadIdx.x;
   const uint32_t xcd = blockIdx.x % a.xcds, local = blockIdx.x / a.xcds;
-  const uint32_t tile = (local / a.spans) * a.xcds + xcd, span = local % a.spans;
-  if (tile >= a.tiles) return;   // padding of the last group (whole workgroup)
-  const uint64_t row0 = (uint64_t)tile * kTileRows;
-  if (t < kTileRows) {
+  const uint32_t per_tile = a.spans * kGroupsPerTile;
+  const uint32_t tile = (local / per_tile) * a.xcds + xcd, rest = local % per_tile;
+  const uint32_t span = rest / kGroupsPerTile, grp = rest % kGroupsPerTile;
+  if (tile >= a.tiles) return;   // padding of the last tiles (whole workgroup)
+  const uint64_t row0 = (uint64_t)tile * kTileRows + grp * kGroupRows;
+  if (t < kGroupRows) {
     const uint64_t r = row0 + t;
     const bool live = r < a.count && !a.special[r];
     len[t] = live ? a.steps[r] + 2 : 0u;
@@ -189,81 +197,77 @@ static __global__ __launch_bounds__(256) void k_compact_write(CompactArgs a) {
   }
   __syncthreads();
   uint32_t nmax = 0;
-  for (uint32_t i = 0; i < kTileRows; ++i) nmax = max(nmax, len[i]);
-  nmax = __builtin_amdgcn_readfirstlane(nmax);   // same in every lane; keeps the loop scalar
-  const float* band = a.slots + slot_float(row0, a.stride, 0);   // slot j of row l: 3 (64 j + l)
-  const uint32_t wave = t >> 6, lane = t & 63, half = lane >> 5, i = lane & 31;
+  for (uint32_t i = 0; i < kGroupRows; ++i) nmax = max(nmax, len[i]);
+  nmax = (uint32_t)__builtin_amdgcn_readfirstlane(nmax);   // same in every lane
+  // slot j of group row l: float 3 * (64 j + l) past band
+  const float* band = a.slots + slot_float((uint64_t)tile * kTileRows, a.stride, 0) + 3 * grp * kGroupRows;
+  const uint32_t h = t / kRunLanes, i = t % kRunLanes;
   const uint32_t jb = span * kCompactSpan, je = min(nmax, jb + kCompactSpan);
+
+  // Staging: item x = t, t + kCompactThreads of the chunk's (slot, row)
+  // pairs, slot lo + x / kGroupRows of row x % kGroupRows, for slots
+  // [lo, hi] = [j0 - 1, j0 + kCompactChunk]
+  // clamped to the rows' extent.  Only slots below the row's length are read:
+  // they are all a row ever uses.  The loads are buffer loads over the
+  // chunk's slots; an item past its row's end gets an offset beyond the
+  // buffer's size, which the hardware answers with zeros without touching
+  // memory.  A chunk's loads are issued while the previous chunk computes
+  // (software pipeline); slot s goes to staged slot s - (j0 - 1).
+  constexpr uint32_t kPer = ((kCompactChunk + 2) * kGroupRows + kCompactThreads - 1) / kCompactThreads;
+  u32x3 r[kPer];
+  auto chunk_lo = [](uint32_t j0) { return j0 == 0 ? 0u : j0 - 1; };
+  auto chunk_hi = [nmax](uint32_t j0) { return min(j0 + kCompactChunk, nmax - 1); };
+  auto issue = [&](uint32_t j0) {
+    const uint32_t lo = chunk_lo(j0), hi = chunk_hi(j0);
+    // base and size are the same in every lane; readfirstlane says so, so the
+    // resource sits in scalar registers (readfirstlane returns int: each half
+    // goes through uint32_t before widening, or bit 31 of the low half would
+    // be sign-extended into the high half)
+    const uint32_t nbytes = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (hi - lo + 1) * (uint32_t)kSlotPitch * 4u);   // <= (kCompactChunk + 2) * 768
+    const uint64_t base = (uint64_t)(band + (uint64_t)lo * kSlotPitch);
+    const uint32_t base_lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
+    const uint32_t base_hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+    const uint64_t ubase = ((uint64_t)base_hi << 32) | (uint64_t)base_lo;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)ubase, (short)0, (int)nbytes, kBufferWord3);
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+      const uint32_t x = t + kCompactThreads * q, l = x % kGroupRows, s = lo + x / kGroupRows;
+      const bool ok = s <= hi && s < len[l];
+      const uint32_t off = ok ? ((s - lo) * (uint32_t)kSlotPitch + 3 * l) * 4u : 0x80000000u;
+      r[q] = __builtin_amdgcn_raw_buffer_load_b96(rs, (int)off, 0, 0);
+    }
+  };
+  if (jb < je) issue(jb);
   for (uint32_t j0 = jb; j0 < je; j0 += kCompactChunk) {
-    // Thread t stages row t % 64, slots lo + t / 64, lo + t / 64 + 4, ...; a
-    // wave's load covers one slot of all 64 rows (768 contiguous bytes).  Only
-    // slots below the row's length are read: they are all a row ever uses.
-    // All of a thread's loads are issued before its first LDS write, so a
-    // chunk waits for one memory latency, not one per slot: they are buffer
-    // loads over the chunk's slots [lo, hi], and a slot past the row's end gets
-    // an offset beyond the buffer's size, which the hardware answers with
-    // zeros without touching memory.  Slot s goes to staged slot s - (j0 - 1).
-    const uint32_t lo = j0 == 0 ? 0u : j0 - 1;
-    const uint32_t hi = min(j0 + kCompactChunk, nmax - 1);
     {
-      constexpr uint32_t kPer = (kCompactChunk + 2 + 3) / 4;   // slots per thread
-      const uint32_t l = t & 63, n = min(len[l], hi + 1);
-      // base and size are the same in every lane; readfirstlane says so, so the
-      // resource sits in scalar registers (readfirstlane returns int: each half
-      // goes through uint32_t before widening, or bit 31 of the low half would
-      // be sign-extended into the high half)
-      const uint32_t nbytes = (uint32_t)__builtin_amdgcn_readfirstlane(
-          (hi - lo + 1) * (uint32_t)kSlotPitch * 4u);   // <= 33 * 768
-      const uint64_t base = (uint64_t)(band + (uint64_t)lo * kSlotPitch);
-      const uint32_t base_lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
-      const uint32_t base_hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
-      const uint64_t ubase = ((uint64_t)base_hi << 32) | (uint64_t)base_lo;
-      const __amdgpu_buffer_rsrc_t rs =
-          __builtin_amdgcn_make_buffer_rsrc((void*)ubase, (short)0, (int)nbytes, kBufferWord3);
-      float* dst = buf + l * kBandRow + 3 * (lo + 1 - j0);
-      u32x3 r[kPer];
+      const uint32_t lo = chunk_lo(j0), hi = chunk_hi(j0);
 #pragma unroll
       for (uint32_t q = 0; q < kPer; ++q) {
-        const uint32_t s = lo + (t >> 6) + 4 * q;
-        const uint32_t off = s < n ? ((s - lo) * (uint32_t)kSlotPitch + 3 * l) * 4u : 0x80000000u;
-        r[q] = __builtin_amdgcn_raw_buffer_load_b96(rs, (int)off, 0, 0);
-      }
-#pragma unroll
-      for (uint32_t q = 0; q < kPer; ++q) {
-        const uint32_t s = lo + (t >> 6) + 4 * q;
-        if (s < n) {
-          float* d = dst + 3 * (s - lo);
+        const uint32_t x = t + kCompactThreads * q, l = x % kGroupRows, s = lo + x / kGroupRows;
+        if (s <= hi && s < len[l]) {
+          float* d = buf + l * kBandRow + 3 * (s + 1 - j0);
           d[0] = __uint_as_float(r[q].x); d[1] = __uint_as_float(r[q].y);
           d[2] = __uint_as_float(r[q].z);
         }
       }
     }
-    if (wave == 0) {   // list the rows that have vertex j0
-      const bool on = len[lane] > j0;
-      const uint64_t m = __ballot(on);
-      const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-      if (on) list[pos] = lane;
-      if (lane == 0) nlist = (uint32_t)__popcll(m);
-    }
     __syncthreads();
-    const uint32_t nl = __builtin_amdgcn_readfirstlane(nlist);
-    for (uint32_t base = 0; base < nl; base += 8) {
-      const uint32_t idx = base + 2 * wave + half;
-      const bool on = idx < nl;
-      const uint32_t l = on ? list[idx] : 0u;
-      const uint32_t n = on ? len[l] : 0u;   // vertices j < n; colours and directions j <= n - 2
+    if (j0 + kCompactChunk < je) issue(j0 + kCompactChunk);   // in flight during this chunk
+    const uint32_t n = len[h];               // vertices j < n; colours and directions j <= n - 2
+    if (n > j0) {                            // uniform over the row's lanes
       const uint32_t j = j0 + i;             // this lane's vertex
-      const float* v = buf + l * kBandRow + 3 * (i + 1);
+      const float* v = buf + h * kBandRow + 3 * (i + 1);
       float d[3] = {0.0f, 0.0f, 0.0f};
-      if (on && j >= 1 && j + 2 <= n) {      // 1 <= j <= k
+      if (j >= 1 && j + 2 <= n) {            // 1 <= j <= k
         const float* u = v - 3;
         normalized3(v[0] - u[0], v[1] - u[1], v[2] - u[2], d);
       }
-      const float n0 = __shfl_down(d[0], 1, 32), n1 = __shfl_down(d[1], 1, 32),
-                  n2 = __shfl_down(d[2], 1, 32);
-      if (on && i < kCompactChunk && j < n) {
-        const uint64_t vb = vbase[l], ib = ibase[l];
+      const float n0 = __shfl_down(d[0], 1, kRunLanes), n1 = __shfl_down(d[1], 1, kRunLanes),
+                  n2 = __shfl_down(d[2], 1, kRunLanes);
+      if (i < kCompactChunk && j < n) {
+        const uint64_t vb = vbase[h], ib = ibase[h];
         const float x = v[0], y = v[1], z = v[2];
         float* out = a.vertices + 3 * (vb + j);
         out[0] = x; out[1] = y; out[2] = z;
@@ -283,9 +287,9 @@ static __global__ __launch_bounds__(256) void k_compact_write(CompactArgs a) {
     }
     __syncthreads();
   }
-  if (span == 0 && t < kTileRows) {
-    const uint64_t r = row0 + t;
-    if (r < a.count && a.special[r]) compact_special_row(a, r);
+  if (span == 0 && t < kGroupRows) {
+    const uint64_t rr = row0 + t;
+    if (rr < a.count && a.special[rr]) compact_special_row(a, rr);
   }
 }
 
