@@ -10,7 +10,7 @@ namespace dpah {
 
 namespace {
 
-template <bool kRecord, int kIntegrator, bool kUnsteady>
+template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact>
 void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool persistent) {
   if (a.n == 0) return;
   if (persistent) {
@@ -19,28 +19,28 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
     ctx->timed("advect_persistent", [&] {
-      hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady>), dim3((unsigned)grid),
-                         dim3(block), 0, ctx->stream, a);
+      hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kExact>),
+                         dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
     });
   } else {
     const uint64_t grid = (a.n + 255) / 256;
     ctx->timed("advect_simple", [&] {
-      hipLaunchKernelGGL((k_advect_simple<kRecord, kIntegrator, kUnsteady>), dim3((unsigned)grid),
+      hipLaunchKernelGGL((k_advect_simple<kRecord, kIntegrator, kUnsteady, kExact>), dim3((unsigned)grid),
                          dim3(256), 0, ctx->stream, a);
     });
   }
   DPAH_HIP(hipGetLastError());
 }
 
-template <bool kRecord>
+template <bool kRecord, bool kExact>
 void dispatch_integrator(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, int integrator,
                          bool unsteady, bool persistent) {
   if (integrator == kRk4) {
-    if (unsteady) launch_t<kRecord, kRk4, true>(ctx, T, a, persistent);
-    else          launch_t<kRecord, kRk4, false>(ctx, T, a, persistent);
+    if (unsteady) launch_t<kRecord, kRk4, true, kExact>(ctx, T, a, persistent);
+    else          launch_t<kRecord, kRk4, false, kExact>(ctx, T, a, persistent);
   } else {
-    if (unsteady) launch_t<kRecord, kDpaRk4, true>(ctx, T, a, persistent);
-    else          launch_t<kRecord, kDpaRk4, false>(ctx, T, a, persistent);
+    if (unsteady) launch_t<kRecord, kDpaRk4, true, kExact>(ctx, T, a, persistent);
+    else          launch_t<kRecord, kDpaRk4, false, kExact>(ctx, T, a, persistent);
   }
 }
 
@@ -192,8 +192,18 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   QueueOrder q;
   if (ordered_launch(ctx, T, L)) order_queue(ctx, T, a, q, L.order_scratch);
   const bool unsteady = L.f->nt > 1;
-  if (L.slots) dispatch_integrator<true>(ctx, T, a, L.integrator, unsteady, L.persistent);
-  else         dispatch_integrator<false>(ctx, T, a, L.integrator, unsteady, L.persistent);
+  // every axis the kernel locates on has a power-of-two spacing: the
+  // specialised kernels multiply by the exact reciprocals without reading the
+  // per-axis flags
+  const uint32_t axes = unsteady ? 15u : 7u;
+  const bool exact = (a.f.exactdiv & axes) == axes;
+  if (L.slots) {
+    if (exact) dispatch_integrator<true, true>(ctx, T, a, L.integrator, unsteady, L.persistent);
+    else       dispatch_integrator<true, false>(ctx, T, a, L.integrator, unsteady, L.persistent);
+  } else {
+    if (exact) dispatch_integrator<false, true>(ctx, T, a, L.integrator, unsteady, L.persistent);
+    else       dispatch_integrator<false, false>(ctx, T, a, L.integrator, unsteady, L.persistent);
+  }
 }
 
 }  // namespace dpah

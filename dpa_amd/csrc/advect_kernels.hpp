@@ -201,7 +201,7 @@ __device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const L
 // One iteration of the loop at particle_advector.cpp:336-389 on field f with
 // step coefficients c and start time t0.  Returns the retirement status, or
 // kActive while the particle keeps going.
-template <bool kRecord, int kIntegrator, bool kUnsteady>
+template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact = false>
 __device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef& c, float t0,
                                              Lane& L) {
   if (L.rem == 0) return kCompleted;
@@ -211,10 +211,10 @@ __device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef&
     t = t0 + ft;
   }
   V3 v;
-  if (!sample<kUnsteady>(f, L.p0, L.p1, L.p2, t, v, L.cc)) return kExited;
+  if (!sample<kUnsteady, kExact>(f, L.p0, L.p1, L.p2, t, v, L.cc)) return kExited;
   if (is_zero(v)) return kZero;
   if constexpr (kIntegrator == kRk4) {
-    if (!step_rk4<kUnsteady>(f, L.p0, L.p1, L.p2, v, t, c, L.cc)) return kExited;
+    if (!step_rk4<kUnsteady, kExact>(f, L.p0, L.p1, L.p2, v, t, c, L.cc)) return kExited;
   } else {
     step_dpa(L.p0, L.p1, L.p2, v, c);
   }
@@ -227,13 +227,13 @@ __device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef&
   return L.rem == 0 ? kCompleted : kActive;
 }
 
-template <bool kRecord, int kIntegrator, bool kUnsteady>
+template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact>
 __device__ __forceinline__ uint8_t step_once(const AdvectArgs& a, Lane& L) {
-  return step_lane<kRecord, kIntegrator, kUnsteady>(a.f, a.coef, a.t0, L);
+  return step_lane<kRecord, kIntegrator, kUnsteady, kExact>(a.f, a.coef, a.t0, L);
 }
 
 // Per-request form: one thread per particle, runs it to retirement.
-template <bool kRecord, int kIntegrator, bool kUnsteady>
+template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact>
 __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
   const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t mine = 0;
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
     if (L.pid != ~0ull) {
       uint8_t st;
       do {
-        st = step_once<kRecord, kIntegrator, kUnsteady>(a, L);
+        st = step_once<kRecord, kIntegrator, kUnsteady, kExact>(a, L);
       } while (st == kActive);
       mine = finish_particle<kRecord>(a, L, st);
     }
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
 // workgroups are dispatched to the XCDs round-robin, so workgroup b starts on
 // part b % parts (its XCD's share of a spatially ordered queue) and moves on to
 // the following parts once its own is drained.
-template <bool kRecord, int kIntegrator, bool kUnsteady>
+template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact>
 __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   const uint32_t lane = threadIdx.x & 63;
   Lane L;
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
     // a wave whose lanes all found a drained part keeps drawing from the next
     if (!__any(busy) && !open) break;
     if (busy) {
-      const uint8_t st = step_once<kRecord, kIntegrator, kUnsteady>(a, L);
+      const uint8_t st = step_once<kRecord, kIntegrator, kUnsteady, kExact>(a, L);
       if (st != kActive) {
         mine += finish_particle<kRecord>(a, L, st);
         L.pid = ~0ull;

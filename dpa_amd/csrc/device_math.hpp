@@ -43,7 +43,7 @@ enum Integrator : int { kDpaRk4 = 0, kRk4 = 1 };
 
 struct Axis {
   float w;   // fmod(d, h) / h
-  int32_t c; // floor(d / h)
+  float s;   // floor(d / h): the cell's start index, an integral float
 };
 
 // fmodf(d, h) for d >= 0 (or -0) and h > 0, given s = floorf(fl(d / h)).
@@ -97,18 +97,43 @@ static __global__ __launch_bounds__(256) void verify_division_kernel(float h, fl
   }
 }
 
-// regular_grid::contains for one axis (regular_grid.hpp:41-50) and the weight /
-// start index of regular_grid::interpolate (52-75).  Returns false outside.
-// `fast` and `exact` are field-uniform (scalar branches).
-__device__ __forceinline__ bool locate(float x, float o, float h, float rh, bool fast, bool exact,
-                                       int32_t n, Axis& a) {
-  const float d = x - o;
-  const float s = floorf(quotient(d, h, rh, fast, exact));
-  if (!(s >= 0.0f) || s >= (float)(n - 1)) return false;
-  const float r = exact_fmod(d, h, s);
-  a.w = quotient(r, h, rh, fast, exact);   // r in [0, h) or -0: inside the verified range
-  a.c = (int32_t)s;
-  return true;
+// One axis of regular_grid::contains (regular_grid.hpp:41-50) and of the
+// weight / start index of regular_grid::interpolate (52-75), split so that a
+// sample computes every axis's floor, tests them together and computes the
+// weights only inside.  `fast` and `exact` are field-uniform.
+//
+// kExact: the caller knows the spacing is a power of two.  Then fl(d * rh) is
+// the same real number as d / h, so s is the exact floor of the quotient (or
+// 0 / -0 for a quotient that underflows, where |d| < h), and fmaf(-s, h, d) is
+// d - s*h exactly: fmod's result, with -0 and tiny negative d returned
+// unchanged as fmodf returns them.  No field flags are read.
+template <bool kExact>
+__device__ __forceinline__ float axis_floor(float d, float h, float rh, bool fast, bool exact) {
+  if constexpr (kExact) {
+    (void)h; (void)fast; (void)exact;
+    return floorf(d * rh);
+  } else {
+    return floorf(quotient(d, h, rh, fast, exact));
+  }
+}
+
+// 1 when 0 <= s <= n - 2, else 0 (also for NaN); an integer so that the axes
+// combine with & and no branch.
+__device__ __forceinline__ uint32_t axis_inside(float s, int32_t n) {
+  return (s >= 0.0f && s < (float)(n - 1)) ? 1u : 0u;
+}
+
+template <bool kExact>
+__device__ __forceinline__ float axis_weight(float d, float s, float h, float rh, bool fast,
+                                             bool exact) {
+  if constexpr (kExact) {
+    (void)fast; (void)exact;
+    const float r = __builtin_fmaf(-s, h, d);
+    return r * rh;
+  } else {
+    const float r = exact_fmod(d, h, s);
+    return quotient(r, h, rh, fast, exact);   // r in [0, h) or -0: inside the verified range
+  }
 }
 
 struct V3 { float a, b, c; };
@@ -145,16 +170,18 @@ __device__ __forceinline__ void load_cell(const float4* rec, V3& L0, V3& L1, V3&
 // A particle moves a fraction of a cell per step, so most steps sample the
 // same cell again and need no memory access at all; only lanes that cross
 // into a new cell reload.  The values are the grid's, so results are
-// unchanged.  c0 < 0 marks an empty cache; clearing sets every coordinate, so
-// the cell test never compares an indeterminate value.
+// unchanged.  The cell is kept as the floats the sample computes (integral
+// values; -0 and +0 name the same cell and compare equal), so the test needs
+// no conversion.  s0 < 0 marks an empty cache; clearing sets every coordinate,
+// so the cell test never compares an indeterminate value.
 struct CellCache {
-  int32_t c0, c1, c2, ct;
+  float s0, s1, s2, st;
   V3 L0, L1, L2, L3, L4, L5, L6, L7;   // corners of slice ct (the only slice if steady)
   V3 M0, M1, M2, M3, M4, M5, M6, M7;   // corners of slice ct + 1 (time-variant fields)
 };
 
 __device__ __forceinline__ void cache_clear(CellCache& cc) {
-  cc.c0 = -1; cc.c1 = -1; cc.c2 = -1; cc.ct = -1;
+  cc.s0 = -1.0f; cc.s1 = -1.0f; cc.s2 = -1.0f; cc.st = -1.0f;
 }
 
 // Trilinear sample of one time slice, corner order and lerp cascade of
@@ -162,13 +189,14 @@ __device__ __forceinline__ void cache_clear(CellCache& cc) {
 __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
                                         const Axis& a0, const Axis& a1, const Axis& a2,
                                         CellCache& cc) {
-  if (a0.c != cc.c0 || a1.c != cc.c1 || a2.c != cc.c2) {
+  if (a0.s != cc.s0 || a1.s != cc.s1 || a2.s != cc.s2) {
+    const int32_t c0 = (int32_t)a0.s, c1 = (int32_t)a1.s, c2 = (int32_t)a2.s;
     if (f.cells) {   // field-uniform branch
-      const float4* rec = f.cells + 8 * cell_record(f, 0, a0.c, a1.c, a2.c);
+      const float4* rec = f.cells + 8 * cell_record(f, 0, c0, c1, c2);
       load_cell(rec, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
     } else {
       const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
-      const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
+      const int64_t b = (int64_t)c0 * n12 + (int64_t)c1 * n2 + c2;
       cc.L0 = node(f, slice, b);
       cc.L1 = node(f, slice, b + 1);
       cc.L2 = node(f, slice, b + n2);
@@ -178,7 +206,7 @@ __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
       cc.L6 = node(f, slice, b + n12 + n2);
       cc.L7 = node(f, slice, b + n12 + n2 + 1);
     }
-    cc.c0 = a0.c; cc.c1 = a1.c; cc.c2 = a2.c;
+    cc.s0 = a0.s; cc.s1 = a1.s; cc.s2 = a2.s;
   }
   const float w2 = a2.w, m2 = 1.0f - w2;
   const V3 A0 = lerp3(cc.L0, cc.L1, w2, m2), A1 = lerp3(cc.L2, cc.L3, w2, m2);
@@ -195,16 +223,18 @@ __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
 // cache holds the corners of both bracketing slices.
 __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, const Axis& a1,
                                          const Axis& a2, const Axis& at, CellCache& cc) {
-  if (a0.c != cc.c0 || a1.c != cc.c1 || a2.c != cc.c2 || at.c != cc.ct) {
+  if (a0.s != cc.s0 || a1.s != cc.s1 || a2.s != cc.s2 || at.s != cc.st) {
+    const int32_t c0 = (int32_t)a0.s, c1 = (int32_t)a1.s, c2 = (int32_t)a2.s;
+    const int32_t ct = (int32_t)at.s;
     if (f.cells) {   // field-uniform branch
-      const float4* r0 = f.cells + 8 * cell_record(f, at.c, a0.c, a1.c, a2.c);
+      const float4* r0 = f.cells + 8 * cell_record(f, ct, c0, c1, c2);
       const float4* r1 = r0 + 8 * f.cell_slice;
       load_cell(r0, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
       load_cell(r1, cc.M0, cc.M1, cc.M2, cc.M3, cc.M4, cc.M5, cc.M6, cc.M7);
     } else {
       const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
-      const int64_t b = (int64_t)a0.c * n12 + (int64_t)a1.c * n2 + a2.c;
-      const float* s0 = f.data + 3 * (int64_t)at.c * f.slice;
+      const int64_t b = (int64_t)c0 * n12 + (int64_t)c1 * n2 + c2;
+      const float* s0 = f.data + 3 * (int64_t)ct * f.slice;
       const float* s1 = s0 + 3 * f.slice;
       cc.L0 = node(f, s0, b);            cc.M0 = node(f, s1, b);
       cc.L1 = node(f, s0, b + 1);        cc.M1 = node(f, s1, b + 1);
@@ -216,7 +246,7 @@ __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, con
       cc.L7 = node(f, s0, b + n12 + n2 + 1);
       cc.M7 = node(f, s1, b + n12 + n2 + 1);
     }
-    cc.c0 = a0.c; cc.c1 = a1.c; cc.c2 = a2.c; cc.ct = at.c;
+    cc.s0 = a0.s; cc.s1 = a1.s; cc.s2 = a2.s; cc.st = at.s;
   }
   const float wt = at.w, mt = 1.0f - wt;
   const V3 T0 = lerp3(cc.L0, cc.M0, wt, mt), T1 = lerp3(cc.L1, cc.M1, wt, mt);
@@ -234,19 +264,38 @@ __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, con
 
 // contains + interpolate at (p, t).  Returns false when p (or t, for a
 // time-variant field) lies outside; v is the stored (unswizzled) vector.
-template <bool kUnsteady>
+// kExact: every axis the sample locates on (time too, for a time-variant
+// field) has a power-of-two spacing (FieldView::exactdiv has all their bits).
+template <bool kUnsteady, bool kExact = false>
 __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, float p2, float t,
                                        V3& v, CellCache& cc) {
+  const float d0 = p0 - f.o0, d1 = p1 - f.o1, d2 = p2 - f.o2;
+  const bool x0 = f.fastdiv & 1u, x1 = f.fastdiv & 2u, x2 = f.fastdiv & 4u;
+  const bool e0 = f.exactdiv & 1u, e1 = f.exactdiv & 2u, e2 = f.exactdiv & 4u;
   Axis a0, a1, a2;
-  if (!locate(p0, f.o0, f.h0, f.r0, f.fastdiv & 1u, f.exactdiv & 1u, f.n0, a0)) return false;
-  if (!locate(p1, f.o1, f.h1, f.r1, f.fastdiv & 2u, f.exactdiv & 2u, f.n1, a1)) return false;
-  if (!locate(p2, f.o2, f.h2, f.r2, f.fastdiv & 4u, f.exactdiv & 4u, f.n2, a2)) return false;
+  a0.s = axis_floor<kExact>(d0, f.h0, f.r0, x0, e0);
+  a1.s = axis_floor<kExact>(d1, f.h1, f.r1, x1, e1);
+  a2.s = axis_floor<kExact>(d2, f.h2, f.r2, x2, e2);
+  // every axis tested together: one branch instead of one per axis
+  uint32_t in = axis_inside(a0.s, f.n0) & axis_inside(a1.s, f.n1) & axis_inside(a2.s, f.n2);
   if constexpr (kUnsteady) {
+    const float dt = t - f.to;
+    const bool xt = f.fastdiv & 8u, et = f.exactdiv & 8u;
     Axis at;
-    if (!locate(t, f.to, f.th, f.rt, f.fastdiv & 8u, f.exactdiv & 8u, f.nt, at)) return false;
+    at.s = axis_floor<kExact>(dt, f.th, f.rt, xt, et);
+    in = in & axis_inside(at.s, f.nt);
+    if (!in) return false;
+    a0.w = axis_weight<kExact>(d0, a0.s, f.h0, f.r0, x0, e0);
+    a1.w = axis_weight<kExact>(d1, a1.s, f.h1, f.r1, x1, e1);
+    a2.w = axis_weight<kExact>(d2, a2.s, f.h2, f.r2, x2, e2);
+    at.w = axis_weight<kExact>(dt, at.s, f.th, f.rt, xt, et);
     v = quadlinear(f, a0, a1, a2, at, cc);
   } else {
     (void)t;
+    if (!in) return false;
+    a0.w = axis_weight<kExact>(d0, a0.s, f.h0, f.r0, x0, e0);
+    a1.w = axis_weight<kExact>(d1, a1.s, f.h1, f.r1, x1, e1);
+    a2.w = axis_weight<kExact>(d2, a2.s, f.h2, f.r2, x2, e2);
     v = trilinear(f, f.data, a0, a1, a2, cc);
   }
   return true;
@@ -293,7 +342,7 @@ __device__ __forceinline__ void step_dpa(float& p0, float& p1, float& p2, const 
 // Classic four-sample RK4 in odeint's explicit_generic_rk order (not dpa's
 // mode; an extension).  A stage point outside the grid returns false and leaves
 // the position unchanged.
-template <bool kUnsteady>
+template <bool kUnsteady, bool kExact = false>
 __device__ __forceinline__ bool step_rk4(const FieldView& f, float& p0, float& p1, float& p2,
                                          const V3& v, float t, const StepCoef& c, CellCache& cc) {
   const float k10 = v.c, k11 = v.b, k12 = v.a;
@@ -302,20 +351,20 @@ __device__ __forceinline__ bool step_rk4(const FieldView& f, float& p0, float& p
   { const float s = c.half_dt * k11; x1 = p1 + s; }
   { const float s = c.half_dt * k12; x2 = p2 + s; }
   V3 s2;
-  if (!sample<kUnsteady>(f, x0, x1, x2, t + c.half_dt, s2, cc)) return false;
+  if (!sample<kUnsteady, kExact>(f, x0, x1, x2, t + c.half_dt, s2, cc)) return false;
   const float k20 = s2.c, k21 = s2.b, k22 = s2.a;
   { float y = p0; const float a = c.zero_dt * k10; y = y + a; const float b = c.half_dt * k20; x0 = y + b; }
   { float y = p1; const float a = c.zero_dt * k11; y = y + a; const float b = c.half_dt * k21; x1 = y + b; }
   { float y = p2; const float a = c.zero_dt * k12; y = y + a; const float b = c.half_dt * k22; x2 = y + b; }
   V3 s3;
-  if (!sample<kUnsteady>(f, x0, x1, x2, t + c.half_dt, s3, cc)) return false;
+  if (!sample<kUnsteady, kExact>(f, x0, x1, x2, t + c.half_dt, s3, cc)) return false;
   const float k30 = s3.c, k31 = s3.b, k32 = s3.a;
   const float one_dt = 1.0f * c.dt;
   { float y = p0; const float a = c.zero_dt * k10; y = y + a; const float b = c.zero_dt * k20; y = y + b; const float d = one_dt * k30; x0 = y + d; }
   { float y = p1; const float a = c.zero_dt * k11; y = y + a; const float b = c.zero_dt * k21; y = y + b; const float d = one_dt * k31; x1 = y + d; }
   { float y = p2; const float a = c.zero_dt * k12; y = y + a; const float b = c.zero_dt * k22; y = y + b; const float d = one_dt * k32; x2 = y + d; }
   V3 s4;
-  if (!sample<kUnsteady>(f, x0, x1, x2, t + one_dt, s4, cc)) return false;
+  if (!sample<kUnsteady, kExact>(f, x0, x1, x2, t + one_dt, s4, cc)) return false;
   const float k40 = s4.c, k41 = s4.b, k42 = s4.a;
   auto fin = [&](float x, float q1, float q2, float q3, float q4) {
     const float a = c.a16 * q1; x = x + a;

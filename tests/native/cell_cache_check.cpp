@@ -1,13 +1,17 @@
 // Host build of the device sampling code: a sample through a warm cell cache
-// (kept across steps, with its face-neighbour partial reloads) must equal, bit
-// for bit, a sample through an empty cache (full eight-corner load), for steady
-// and time-variant fields, over random walks that cross cells in every
-// direction.  Prints "samples S shifts F diffs D" and exits non-zero on a diff.
+// (kept across steps) must equal, bit for bit, a sample through an empty cache
+// (full eight-corner load) that divides by the spacings, for steady and
+// time-variant fields, over random walks that cross cells in every direction.
+// On grids whose spacings are all powers of two the warm sample also takes the
+// specialised path (reciprocal multiply and one fmaf per axis, no field flags),
+// so that path is pinned against the correctly rounded divisions and fmodf.
+// Prints "samples S shifts F diffs D" and exits non-zero on a diff.
 #include <hip/hip_runtime.h>
 #undef __device__
 #define __device__ __attribute__((host)) __attribute__((device))
 #include "../../dpa_amd/csrc/device_math.hpp"
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -15,32 +19,74 @@
 
 using namespace dpah;
 
-template <bool kUnsteady>
+template <bool kUnsteady, bool kExact>
 static long walk(const FieldView& f, std::mt19937& rng, long& samples, long& shifts) {
+  FieldView plain = f;   // the cold reference divides on every axis
+  plain.fastdiv = plain.exactdiv = 0;
   std::uniform_real_distribution<float> u(-1.0f, 1.0f);
   std::uniform_real_distribution<float> p0(0.0f, (float)(f.n0 - 1)), p1(0.0f, (float)(f.n1 - 1)),
       p2(0.0f, (float)(f.n2 - 1)), pt(0.0f, (float)(f.nt - 1));
   long diffs = 0;
   for (int k = 0; k < 4000; ++k) {
-    float x0 = p0(rng), x1 = p1(rng), x2 = p2(rng), t = kUnsteady ? pt(rng) : 0.0f;
+    float x0 = f.o0 + f.h0 * p0(rng), x1 = f.o1 + f.h1 * p1(rng), x2 = f.o2 + f.h2 * p2(rng);
+    float t = kUnsteady ? f.to + f.th * pt(rng) : 0.0f;
     CellCache warm;
     cache_clear(warm);
     const float reach = (k & 1) ? 1.2f : 0.3f;   // cell-crossing and in-cell walks
     for (int s = 0; s < 300; ++s) {
-      const int32_t c0 = warm.c0, c1 = warm.c1, c2 = warm.c2, ct = warm.ct;
+      const float c0 = warm.s0, c1 = warm.s1, c2 = warm.s2, ct = warm.st;
       V3 a{}, b{};
       CellCache cold;
       cache_clear(cold);
-      const bool ok = sample<kUnsteady>(f, x0, x1, x2, t, a, warm);
-      const bool ok2 = sample<kUnsteady>(f, x0, x1, x2, t, b, cold);
+      const bool ok = sample<kUnsteady, kExact>(f, x0, x1, x2, t, a, warm);
+      const bool ok2 = sample<kUnsteady, false>(plain, x0, x1, x2, t, b, cold);
       if (ok != ok2) { ++diffs; break; }
       if (!ok) break;
       ++samples;
-      const int moved = (warm.c0 != c0) + (warm.c1 != c1) + (warm.c2 != c2);
-      if (c0 >= 0 && moved == 1 && (!kUnsteady || warm.ct == ct)) ++shifts;
+      const int moved = (warm.s0 != c0) + (warm.s1 != c1) + (warm.s2 != c2);
+      if (c0 >= 0 && moved == 1 && (!kUnsteady || warm.st == ct)) ++shifts;
       if (std::memcmp(&a, &b, sizeof a) != 0) ++diffs;
-      x0 += reach * u(rng); x1 += reach * u(rng); x2 += reach * u(rng);
-      if (kUnsteady) t += 0.05f * reach * u(rng);
+      x0 += reach * f.h0 * u(rng); x1 += reach * f.h1 * u(rng); x2 += reach * f.h2 * u(rng);
+      if (kUnsteady) t += 0.05f * reach * f.th * u(rng);
+    }
+  }
+  return diffs;
+}
+
+// Positions at the edges of the power-of-two path, one axis at a time with the
+// others mid-grid: zeros of both signs, denormals, tiny negatives, exact cell
+// faces, the last valid cell and just past it.  Returns the number of samples
+// (inside flag or value bits) that differ from the dividing path.
+static long probes(const FieldView& e, long& samples) {
+  FieldView plain = e;
+  plain.fastdiv = plain.exactdiv = 0;
+  const float tiny[] = {0.0f, -0.0f, 1e-45f, -1e-45f, 1e-38f, -1e-38f, 1e-30f, -1e-30f,
+                        -1e-7f, 1e-7f};
+  long diffs = 0;
+  for (int ax = 0; ax < 3; ++ax) {
+    const float o = ax == 0 ? e.o0 : ax == 1 ? e.o1 : e.o2;
+    const float h = ax == 0 ? e.h0 : ax == 1 ? e.h1 : e.h2;
+    const int32_t n = ax == 0 ? e.n0 : ax == 1 ? e.n1 : e.n2;
+    std::vector<float> xs;
+    for (float d : tiny) xs.push_back(o + d);
+    for (int k = 0; k < n; ++k) {
+      const float face = o + h * (float)k;
+      xs.push_back(face);
+      xs.push_back(std::nextafter(face, -1e30f));
+      xs.push_back(std::nextafter(face, 1e30f));
+    }
+    for (float x : xs) {
+      float p[3] = {e.o0 + e.h0 * (e.n0 - 1) * 0.5f, e.o1 + e.h1 * (e.n1 - 1) * 0.5f,
+                    e.o2 + e.h2 * (e.n2 - 1) * 0.5f};
+      p[ax] = x;
+      V3 a{}, b{};
+      CellCache c1, c2;
+      cache_clear(c1);
+      cache_clear(c2);
+      const bool ok = sample<false, true>(e, p[0], p[1], p[2], 0.0f, a, c1);
+      const bool ok2 = sample<false, false>(plain, p[0], p[1], p[2], 0.0f, b, c2);
+      ++samples;
+      if (ok != ok2 || (ok && std::memcmp(&a, &b, sizeof a) != 0)) ++diffs;
     }
   }
   return diffs;
@@ -59,14 +105,28 @@ int main() {
   f.o0 = -1.5f; f.o1 = 0.25f; f.o2 = 2.0f; f.to = 0.0f;
   f.h0 = 0.5f; f.h1 = 1.0f; f.h2 = 0.75f; f.th = 1.0f;
   long samples = 0, shifts = 0;
+  f.r0 = 1.0f / f.h0; f.r1 = 1.0f / f.h1; f.r2 = 1.0f / f.h2; f.rt = 1.0f / f.th;
   // positions in cell units: scale the walk into physical coordinates
   FieldView unit = f;
   unit.o0 = unit.o1 = unit.o2 = 0.0f; unit.h0 = unit.h1 = unit.h2 = 1.0f;
-  long diffs = walk<false>(unit, rng, samples, shifts);
-  diffs += walk<false>(f, rng, samples, shifts);
+  unit.r0 = unit.r1 = unit.r2 = 1.0f;
+  long diffs = walk<false, false>(unit, rng, samples, shifts);
+  diffs += walk<false, false>(f, rng, samples, shifts);
   FieldView g = unit;
   g.nt = nt;
-  diffs += walk<true>(g, rng, samples, shifts);
+  diffs += walk<true, false>(g, rng, samples, shifts);
+  // power-of-two spacings (0.5, 2, 0.25, time 0.5) through the specialised path
+  FieldView e = f;
+  e.h0 = 0.5f; e.h1 = 2.0f; e.h2 = 0.25f; e.th = 0.5f;
+  e.r0 = 2.0f; e.r1 = 0.5f; e.r2 = 4.0f; e.rt = 2.0f;
+  e.exactdiv = 15u;
+  diffs += walk<false, true>(unit, rng, samples, shifts);
+  diffs += walk<false, true>(e, rng, samples, shifts);
+  FieldView et = e;
+  et.nt = nt;
+  diffs += walk<true, true>(et, rng, samples, shifts);
+  diffs += probes(e, samples);
+  diffs += probes(unit, samples);
   std::printf("samples %ld shifts %ld diffs %ld\n", samples, shifts, diffs);
   return diffs != 0;
 }
