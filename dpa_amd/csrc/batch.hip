@@ -156,6 +156,45 @@ void run_group(dpah_context* ctx, const dpah_job* jobs, const std::vector<uint64
   L.persistent = true;
   if (L.nseg) launch_advect(ctx, L);
 
+  // Every recorded segment's counts and scan go on the stream behind the
+  // launch; their totals come back with the final states in one
+  // synchronisation, then every write is launched and the group waits once.
+  std::vector<uint64_t> row_off(segs.size(), 0), sums_off(segs.size(), 0);
+  uint64_t rows_total = 0, sums_total = 0;
+  if (record) {
+    for (size_t s = 0; s < segs.size(); ++s) {
+      row_off[s] = rows_total;
+      sums_off[s] = sums_total;
+      rows_total += segs[s].count;
+      sums_total += compact_sums_words(segs[s].count);
+    }
+  }
+  DeviceBuffer d_vcount, d_icount, d_sums;
+  std::vector<uint64_t> h_sums(sums_total);
+  auto rows_of = [&](size_t s) {
+    return CompactRows{d_slots.as<float>() + segs[s].slot_base, segs[s].stride,
+                       d_steps.as<uint32_t>() + segs[s].first,
+                       d_special.as<uint8_t>() + segs[s].first, segs[s].count, 0, 0};
+  };
+  std::vector<int> seg_use64(segs.size(), 0);
+  for (const JobPlan& p : plans)
+    for (const RoundPlan& r : p.rounds)
+      if (r.seg >= 0) seg_use64[r.seg] = jobs[p.job].params.use_64_bit_indices ? 1 : 0;
+  if (record && rows_total) {
+    d_vcount = DeviceBuffer(ctx, rows_total * sizeof(uint64_t));
+    d_icount = DeviceBuffer(ctx, rows_total * sizeof(uint64_t));
+    d_sums = DeviceBuffer(ctx, sums_total * sizeof(uint64_t));
+    for (size_t s = 0; s < segs.size(); ++s) {
+      CompactRows r = rows_of(s);
+      r.use64 = seg_use64[s];
+      compact_count(ctx, r, d_vcount.as<uint64_t>() + row_off[s],
+                    d_icount.as<uint64_t>() + row_off[s], d_sums.as<uint64_t>() + sums_off[s],
+                    ctx->stream);
+    }
+    DPAH_HIP(hipMemcpyAsync(h_sums.data(), d_sums.ptr, sums_total * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, ctx->stream));
+  }
+
   std::vector<float> h_pos(3 * n1);
   std::vector<uint64_t> h_rem_out(n1);
   std::vector<uint8_t> h_status(n1);
@@ -181,22 +220,26 @@ void run_group(dpah_context* ctx, const dpah_job* jobs, const std::vector<uint64
       for (const RoundPlan& r : p.rounds) {
         Round round;
         if (r.count) {
-          const Segment& s = segs[r.seg];
-          Chunk ch = compact_rows(ctx, d_slots.as<float>() + s.slot_base, s.stride,
-                                  d_steps.as<uint32_t>() + s.first,
-                                  d_special.as<uint8_t>() + s.first, s.count, 0, res->use64);
+          const size_t s = (size_t)r.seg;
+          CompactRows rows = rows_of(s);
+          rows.use64 = res->use64;
+          const uint64_t* tot = h_sums.data() + sums_off[s] + compact_sums_words(segs[s].count) - 2;
+          if (!res->use64)
+            DPAH_REQUIRE(tot[0] <= 0xffffffffull,
+                         "round has more than 2^32-1 vertices; 32-bit indices cannot address it");
+          Chunk ch = compact_write(ctx, rows, d_vcount.as<uint64_t>() + row_off[s],
+                                   d_icount.as<uint64_t>() + row_off[s], tot[0], tot[1],
+                                   ctx->stream);
           round.V = ch.V;
           round.I = ch.I;
           round.chunks.push_back(ch);
-          if (!res->use64)
-            DPAH_REQUIRE(round.V <= 0xffffffffull,
-                         "round has more than 2^32-1 vertices; 32-bit indices cannot address it");
         }
         res->rounds.push_back(std::move(round));
       }
     }
     out[p.job] = std::move(res);
   }
+  if (record) ctx->sync();   // the chunks are complete when the results are handed out
 }
 
 }  // namespace

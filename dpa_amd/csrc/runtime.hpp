@@ -292,8 +292,31 @@ dpah_result* run_ftle_aux(dpah_context* ctx, const dpah_field* field, const floa
                           float start_time, int integrator, bool persistent);
 
 // Prunes, indexes and colours `count` slot rows; allocates the chunk's outputs.
+// Synchronises the stream twice (the output size, then the written chunk).
 Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const uint32_t* steps,
                    const uint8_t* special, uint64_t count, uint64_t vertex_base, int use64,
                    hipStream_t stream = nullptr);
+
+// The same in two phases, for callers that compact many row sets with one
+// synchronisation: compact_count launches the per-row counts and their scan
+// (vcount / icount: `count` words each; sums: compact_sums_words(count) words,
+// whose last two hold the vertex and segment totals once the stream reaches
+// them); compact_write allocates the chunk for those totals and launches the
+// write.  Neither waits for the device.
+struct CompactRows {
+  const float* slots;
+  uint64_t stride;
+  const uint32_t* steps;
+  const uint8_t* special;
+  uint64_t count;
+  uint64_t vertex_base;
+  int use64;
+};
+uint64_t compact_sums_words(uint64_t count);
+void compact_count(dpah_context* ctx, const CompactRows& r, uint64_t* vcount, uint64_t* icount,
+                   uint64_t* sums, hipStream_t stream);
+Chunk compact_write(dpah_context* ctx, const CompactRows& r, const uint64_t* vcount,
+                    const uint64_t* icount, uint64_t vertices, uint64_t segments,
+                    hipStream_t stream);
 
 }  // namespace dpah
