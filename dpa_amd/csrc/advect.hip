@@ -87,7 +87,8 @@ void order_shape(const FieldView& f, uint32_t& shift, uint32_t& bits, uint32_t& 
 
 // Seed arrays and plain lattices.  A lattice in C order gives a wave 64 points
 // along one row; bricks give it a 4x4x4 block (dpa-mode FTLE on 512x512x128:
-// 16.8 ms advection ordered, 39.7 ms not, for 4 ms of sorting).  The auxiliary
+// 16.8 ms advection ordered, 39.7 ms not; a lattice is unranked into bricks in
+// closed form, a seed array sorted by brick Morton key).  The auxiliary
 // lattice already gives a wave the six neighbours of ~11 adjacent points, and
 // it has six times the particles: sorting it cost 50 ms against a 66 ms
 // advection that it did not speed up.  Endpoint launches only: a recorded
@@ -190,7 +191,16 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;
-  if (ordered_launch(ctx, T, L)) order_queue(ctx, T, a, q, L.order_scratch);
+  if (ordered_launch(ctx, T, L)) {
+    if (L.seed_kind == kSeedLattice) {
+      // a plain lattice needs no sort: the kernel unranks queue positions
+      // into 4x4x4 bricks directly
+      a.lat.brick = kLatticeBrick;
+      set_parts(a, (uint32_t)std::max(1, std::min(T.parts, kMaxParts)));
+    } else {
+      order_queue(ctx, T, a, q, L.order_scratch);
+    }
+  }
   const bool unsteady = L.f->nt > 1;
   // every axis the kernel locates on has a power-of-two spacing: the
   // specialised kernels multiply by the exact reciprocals without reading the

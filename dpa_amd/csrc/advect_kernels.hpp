@@ -26,7 +26,39 @@ struct SeedLattice {
   uint64_t d0, d1, d2;        // points per axis
   float dl0, dl1, dl2;        // auxiliary offsets (kSeedAux)
   uint64_t first;             // lattice index of particle / point 0 (chunked launches)
+  // kSeedLattice endpoint launches: the queue walks the lattice in bricks of
+  // brick^3 points (lattice_brick_index); 0: in C order
+  uint32_t brick;
 };
+
+// Queue position q -> C-order index of the lattice point it starts, when the
+// queue walks a d0 x d1 x d2 lattice (fewer than 2^32 points) in bricks of
+// B^3 points: bricks in C order, points in C order inside a brick, the bricks
+// on the far faces cut to the lattice.  Every brick before the last along an
+// axis is whole, so each level of the walk is one division: the slab of
+// bricks along axis 0, the row of bricks along axis 1 inside it, the brick
+// along axis 2 inside that, then the point inside the brick.  With B = 4 a
+// wave that draws 64 aligned positions starts a 4x4x4 block of neighbours.
+__host__ __device__ inline uint64_t lattice_brick_index(uint32_t d0, uint32_t d1, uint32_t d2,
+                                                        uint32_t B, uint32_t q) {
+  const uint32_t slab = B * d1 * d2;
+  const uint32_t b0 = q / slab;
+  uint32_t r = q - b0 * slab;
+  const uint32_t h0 = d0 - B * b0 < B ? d0 - B * b0 : B;
+  const uint32_t row = h0 * B * d2;
+  const uint32_t b1 = r / row;
+  r -= b1 * row;
+  const uint32_t h1 = d1 - B * b1 < B ? d1 - B * b1 : B;
+  const uint32_t brick = h0 * h1 * B;
+  const uint32_t b2 = r / brick;
+  r -= b2 * brick;
+  const uint32_t h2 = d2 - B * b2 < B ? d2 - B * b2 : B;
+  const uint32_t i0 = r / (h1 * h2);
+  r -= i0 * h1 * h2;
+  const uint32_t i1 = r / h2;
+  const uint32_t i2 = r - i1 * h2;
+  return ((uint64_t)(B * b0 + i0) * d1 + (B * b1 + i1)) * (uint64_t)d2 + (B * b2 + i2);
+}
 
 // Record slots are tiled by 64 rows: slot j of row r lives at float
 // 3 * ((r / 64 * stride + j) * 64 + r % 64).  The 64 lanes of a wave that hold
@@ -153,6 +185,9 @@ __device__ __forceinline__ void seed_position(const AdvectArgs& a, uint64_t pid,
 __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, float*& row) {
   if (a.nseg == 0) {
     row = a.slots + slot_float(q, a.stride, 0);
+    if (a.lat.brick)   // brick-ordered lattice (endpoint launches only)
+      return lattice_brick_index((uint32_t)a.lat.d0, (uint32_t)a.lat.d1, (uint32_t)a.lat.d2,
+                                 a.lat.brick, (uint32_t)q);
     return q;
   }
   uint32_t lo = 0, hi = a.nseg;            // last segment with qbase <= q

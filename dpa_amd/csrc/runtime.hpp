@@ -230,6 +230,7 @@ struct OrderScratch {
 // Particles worth ordering: enough to fill the persistent grid several times,
 // and few enough for 32-bit positions.
 constexpr uint64_t kOrderMinParticles = 1ull << 15;
+constexpr uint32_t kLatticeBrick = 4;   // edge of a brick-ordered lattice queue, in points
 
 // Launchers (advect.hip / compact.hip).
 struct AdvectLaunch {

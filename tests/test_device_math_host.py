@@ -4,6 +4,10 @@
   of a particle's last cell; every sample through a warm cache must equal the
   sample through an empty one, bit for bit, over random walks on steady and
   time-variant grids, with the node layout.
+* tests/native/brick_order_check.cpp: the closed-form brick order of lattice
+  launches (queue position -> lattice index) is a bijection that visits
+  bricks in C order, one contiguous run per brick, points in C order inside,
+  also on lattices whose sides are not multiples of the brick.
 * tests/native/div_check.cpp: the reciprocal division used on verified axes
   must equal the correctly rounded x / h bit for bit (all patterns around the
   switch to plain division and over [h/2, 2h], plus a sample of the whole
@@ -45,3 +49,9 @@ def test_reciprocal_division_is_correctly_rounded(tmp_path):
     lines = build_and_run(tmp_path, "div_check").split("\n")
     rows = [l.split() for l in lines if l.strip()]
     assert len(rows) == 17 and all(r[1] == "0" for r in rows), lines
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
+def test_lattice_brick_order_is_a_bijection(tmp_path):
+    words = build_and_run(tmp_path, "brick_order_check").split()
+    assert int(words[1]) == 27 and int(words[3]) > 100000000 and int(words[5]) == 0, words
