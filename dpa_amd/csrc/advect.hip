@@ -85,20 +85,30 @@ void order_shape(const FieldView& f, uint32_t& shift, uint32_t& bits, uint32_t& 
   tiles = (keys + kOrderTile - 1) / kOrderTile;   // <= 129 for 6 bits
 }
 
-// Seed arrays and plain lattices.  A lattice in C order gives a wave 64 points
-// along one row; bricks give it a 4x4x4 block (dpa-mode FTLE on 512x512x128:
-// 16.8 ms advection ordered, 39.7 ms not; a lattice is unranked into bricks in
-// closed form, a seed array sorted by brick Morton key).  The auxiliary
-// lattice already gives a wave the six neighbours of ~11 adjacent points, and
-// it has six times the particles: sorting it cost 50 ms against a 66 ms
-// advection that it did not speed up.  Endpoint launches only: a recorded
-// launch writes one slot row per particle, and with the queue in seed order a
-// wave's rows are neighbours in memory (2^18 cfg2 seeds x 1000 recorded
-// steps: 4.0 ms in seed order, 5.1 ms ordered).
+// Seed arrays sorted by brick Morton key before the launch.  Endpoint
+// launches only: a recorded launch writes one slot row per particle, and with
+// the queue in seed order a wave's rows are neighbours in memory (2^18 cfg2
+// seeds x 1000 recorded steps: 4.0 ms in seed order, 5.1 ms ordered).
 bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch& L) {
   return L.persistent && T.order && (ctx->caching || L.order_scratch) && L.nseg == 0 &&
-         L.slots == nullptr && L.seed_kind != kSeedAux && L.n >= kOrderMinParticles &&
+         L.slots == nullptr && L.seed_kind == kSeedArray && L.n >= kOrderMinParticles &&
          L.n < (1ull << 32);
+}
+
+// Lattices need no sort: the kernel unranks queue positions into 4x4x4 bricks
+// of lattice points (lattice_brick_index), six consecutive particles per point
+// for the auxiliary form.  A lattice in C order gives a wave 64 points along
+// one row; bricks give it a 4x4x4 block (dpa-mode FTLE on 512x512x128: 16.1 ms
+// advection, 39.7 ms in C order).  Whole lattices in one launch only, since
+// the walk is over the whole lattice and the outputs are indexed by lattice
+// point.
+bool brick_launch(const Tuning& T, const AdvectLaunch& L) {
+  if (!(L.persistent && T.order && L.nseg == 0 && L.slots == nullptr && L.lat.first == 0))
+    return false;
+  const uint64_t points = L.lat.d0 * L.lat.d1 * L.lat.d2;
+  if (L.seed_kind == kSeedLattice) return L.n == points && L.n < (1ull << 32);
+  if (L.seed_kind == kSeedAux) return L.n == 6 * points && L.n < (1ull << 32);
+  return false;
 }
 
 namespace {
@@ -192,14 +202,10 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   DPAH_HIP(hipGetLastError());
   QueueOrder q;
   if (ordered_launch(ctx, T, L)) {
-    if (L.seed_kind == kSeedLattice) {
-      // a plain lattice needs no sort: the kernel unranks queue positions
-      // into 4x4x4 bricks directly
-      a.lat.brick = kLatticeBrick;
-      set_parts(a, (uint32_t)std::max(1, std::min(T.parts, kMaxParts)));
-    } else {
-      order_queue(ctx, T, a, q, L.order_scratch);
-    }
+    order_queue(ctx, T, a, q, L.order_scratch);
+  } else if (brick_launch(T, L)) {
+    a.lat.brick = kLatticeBrick;
+    set_parts(a, (uint32_t)std::max(1, std::min(T.parts, kMaxParts)));
   }
   const bool unsteady = L.f->nt > 1;
   // every axis the kernel locates on has a power-of-two spacing: the

@@ -26,8 +26,9 @@ struct SeedLattice {
   uint64_t d0, d1, d2;        // points per axis
   float dl0, dl1, dl2;        // auxiliary offsets (kSeedAux)
   uint64_t first;             // lattice index of particle / point 0 (chunked launches)
-  // kSeedLattice endpoint launches: the queue walks the lattice in bricks of
-  // brick^3 points (lattice_brick_index); 0: in C order
+  // whole-lattice endpoint launches: the queue walks the lattice points in
+  // bricks of brick^3 points (lattice_brick_index), the six particles of an
+  // auxiliary point together; 0: in C order
   uint32_t brick;
 };
 
@@ -185,9 +186,14 @@ __device__ __forceinline__ void seed_position(const AdvectArgs& a, uint64_t pid,
 __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, float*& row) {
   if (a.nseg == 0) {
     row = a.slots + slot_float(q, a.stride, 0);
-    if (a.lat.brick)   // brick-ordered lattice (endpoint launches only)
-      return lattice_brick_index((uint32_t)a.lat.d0, (uint32_t)a.lat.d1, (uint32_t)a.lat.d2,
-                                 a.lat.brick, (uint32_t)q);
+    if (a.lat.brick) {   // brick-ordered lattice (whole-lattice endpoint launches)
+      const uint32_t d0 = (uint32_t)a.lat.d0, d1 = (uint32_t)a.lat.d1, d2 = (uint32_t)a.lat.d2;
+      if (a.seed_kind == kSeedAux) {
+        const uint32_t point = (uint32_t)q / 6u;
+        return 6 * lattice_brick_index(d0, d1, d2, a.lat.brick, point) + ((uint32_t)q - 6u * point);
+      }
+      return lattice_brick_index(d0, d1, d2, a.lat.brick, (uint32_t)q);
+    }
     return q;
   }
   uint32_t lo = 0, hi = a.nseg;            // last segment with qbase <= q

@@ -260,8 +260,10 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
 // Brick shape of the queue ordering of a field (order_kernels.hpp).
 void order_shape(const FieldView& f, uint32_t& shift, uint32_t& bits, uint32_t& keys,
                  uint32_t& tiles);
-// Whether a launch orders its queue (and so needs order scratch).
+// Whether a launch sorts its queue (and so needs order scratch).
 bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch& L);
+// Whether a lattice launch walks its queue in closed-form bricks (no scratch).
+bool brick_launch(const Tuning& T, const AdvectLaunch& L);
 // The tuning a launch of this key uses (an autotuned entry or the context's).
 const Tuning& tuning_for(const dpah_context* ctx, const std::string& key);
 // Whether a launch reads the cell layout (when its field has one).

@@ -101,3 +101,38 @@ def test_ftle_estimate_from_given_positions(ctx):
             ctx.free(p)
     assert coll == 0
     assert_ftle_close(got, want, 0.5)
+
+
+@pytest.mark.parametrize("kind", ["lattice", "aux"])
+def test_brick_walk_matches_c_order(ctx, kind):
+    # whole-lattice launches walk their points in 4x4x4 bricks cut at the far
+    # faces (odd sides here); the volume and the step count must equal the
+    # C-order walk bit for bit
+    say(f"case brick walk vs C order, {kind}")
+    field, h = W.abc_field((23, 17, 13), spacing=(2 * math.pi / 22,) * 3)
+    off = np.zeros(3, f32)
+    iters, dt = 80, f32(0.02)
+    shape = field.shape[:3]
+    n = int(np.prod(shape))
+    fld = ctx.field(field, off, h)
+    before = ctx.tuning()
+    out = []
+    try:
+        for order in (1, 0):
+            ctx.set_tuning(order=order)
+            if kind == "lattice":
+                vol, steps = ctx.ftle_lattice(fld, (1.0, 1.0, 1.0), iters, dt)
+            else:
+                d_out = ctx.alloc(4 * n)
+                try:
+                    steps = ctx.ftle_aux_device(fld, off, h, shape, (f32(0.5) * h).astype(f32),
+                                                iters, dt, d_out)
+                    vol = ctx.to_host(d_out, np.empty(n, f32))
+                finally:
+                    ctx.free(d_out)
+            out.append((vol, steps))
+    finally:
+        ctx.set_tuning(**before)
+        fld.release()
+    assert out[0][1] == out[1][1] and out[0][1] > 1000
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
