@@ -10,7 +10,7 @@ namespace dpah {
 
 namespace {
 
-template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact>
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv>
 void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool persistent) {
   if (a.n == 0) return;
   if (persistent) {
@@ -19,28 +19,38 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
     ctx->timed("advect_persistent", [&] {
-      hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kExact>),
+      hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv>),
                          dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
     });
   } else {
     const uint64_t grid = (a.n + 255) / 256;
     ctx->timed("advect_simple", [&] {
-      hipLaunchKernelGGL((k_advect_simple<kRecord, kIntegrator, kUnsteady, kExact>), dim3((unsigned)grid),
+      hipLaunchKernelGGL((k_advect_simple<kRecord, kIntegrator, kUnsteady, kDiv>), dim3((unsigned)grid),
                          dim3(256), 0, ctx->stream, a);
     });
   }
   DPAH_HIP(hipGetLastError());
 }
 
-template <bool kRecord, bool kExact>
+template <bool kRecord, int kDiv>
 void dispatch_integrator(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, int integrator,
                          bool unsteady, bool persistent) {
   if (integrator == kRk4) {
-    if (unsteady) launch_t<kRecord, kRk4, true, kExact>(ctx, T, a, persistent);
-    else          launch_t<kRecord, kRk4, false, kExact>(ctx, T, a, persistent);
+    if (unsteady) launch_t<kRecord, kRk4, true, kDiv>(ctx, T, a, persistent);
+    else          launch_t<kRecord, kRk4, false, kDiv>(ctx, T, a, persistent);
   } else {
-    if (unsteady) launch_t<kRecord, kDpaRk4, true, kExact>(ctx, T, a, persistent);
-    else          launch_t<kRecord, kDpaRk4, false, kExact>(ctx, T, a, persistent);
+    if (unsteady) launch_t<kRecord, kDpaRk4, true, kDiv>(ctx, T, a, persistent);
+    else          launch_t<kRecord, kDpaRk4, false, kDiv>(ctx, T, a, persistent);
+  }
+}
+
+template <bool kRecord>
+void dispatch_division(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, int integrator,
+                       bool unsteady, bool persistent) {
+  switch (div_mode(a.f.exactdiv, a.f.fastdiv, unsteady)) {
+    case kDivExact: dispatch_integrator<kRecord, kDivExact>(ctx, T, a, integrator, unsteady, persistent); break;
+    case kDivFast:  dispatch_integrator<kRecord, kDivFast>(ctx, T, a, integrator, unsteady, persistent); break;
+    default:        dispatch_integrator<kRecord, kDivide>(ctx, T, a, integrator, unsteady, persistent); break;
   }
 }
 
@@ -208,18 +218,8 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
     set_parts(a, (uint32_t)std::max(1, std::min(T.parts, kMaxParts)));
   }
   const bool unsteady = L.f->nt > 1;
-  // every axis the kernel locates on has a power-of-two spacing: the
-  // specialised kernels multiply by the exact reciprocals without reading the
-  // per-axis flags
-  const uint32_t axes = unsteady ? 15u : 7u;
-  const bool exact = (a.f.exactdiv & axes) == axes;
-  if (L.slots) {
-    if (exact) dispatch_integrator<true, true>(ctx, T, a, L.integrator, unsteady, L.persistent);
-    else       dispatch_integrator<true, false>(ctx, T, a, L.integrator, unsteady, L.persistent);
-  } else {
-    if (exact) dispatch_integrator<false, true>(ctx, T, a, L.integrator, unsteady, L.persistent);
-    else       dispatch_integrator<false, false>(ctx, T, a, L.integrator, unsteady, L.persistent);
-  }
+  if (L.slots) dispatch_division<true>(ctx, T, a, L.integrator, unsteady, L.persistent);
+  else         dispatch_division<false>(ctx, T, a, L.integrator, unsteady, L.persistent);
 }
 
 }  // namespace dpah

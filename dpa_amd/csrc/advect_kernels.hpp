@@ -225,9 +225,13 @@ __device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uin
   }
 }
 
-template <bool kRecord>
+// kCountIt: the loop counted the particle's steps in L.it; otherwise they are
+// its starting iterations minus the ones left.
+template <bool kRecord, bool kCountIt = true>
 __device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const Lane& L, uint8_t st) {
   const uint64_t pid = L.pid;
+  uint32_t it = L.it;
+  if constexpr (!kCountIt) it = (uint32_t)((a.rem_in ? a.rem_in[pid] : a.uniform_rem) - L.rem);
   if constexpr (kRecord) {
     put3(L.row + kSlotPitch * ((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
     if (a.special_out) a.special_out[pid] = L.special ? 1 : 0;
@@ -235,14 +239,14 @@ __device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const L
   put3(a.pos_out + 3 * pid, L.p0, L.p1, L.p2);
   if (a.rem_out) a.rem_out[pid] = L.rem;
   if (a.status_out) a.status_out[pid] = st;
-  if (a.steps_out) a.steps_out[pid] = L.it;
-  return L.it;
+  if (a.steps_out) a.steps_out[pid] = it;
+  return it;
 }
 
 // One iteration of the loop at particle_advector.cpp:336-389 on field f with
 // step coefficients c and start time t0.  Returns the retirement status, or
 // kActive while the particle keeps going.
-template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact = false>
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv, bool kCountIt = true>
 __device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef& c, float t0,
                                              Lane& L) {
   if (L.rem == 0) return kCompleted;
@@ -252,15 +256,15 @@ __device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef&
     t = t0 + ft;
   }
   V3 v;
-  if (!sample<kUnsteady, kExact>(f, L.p0, L.p1, L.p2, t, v, L.cc)) return kExited;
+  if (!sample<kUnsteady, kDiv>(f, L.p0, L.p1, L.p2, t, v, L.cc)) return kExited;
   if (is_zero(v)) return kZero;
   if constexpr (kIntegrator == kRk4) {
-    if (!step_rk4<kUnsteady, kExact>(f, L.p0, L.p1, L.p2, v, t, c, L.cc)) return kExited;
+    if (!step_rk4<kUnsteady, kDiv>(f, L.p0, L.p1, L.p2, v, t, c, L.cc)) return kExited;
   } else {
     step_dpa(L.p0, L.p1, L.p2, v, c);
   }
   L.rem -= 1;
-  L.it += 1;
+  if constexpr (kCountIt) L.it += 1;
   if constexpr (kRecord) {
     put3(L.row + kSlotPitch * (uint64_t)L.it, L.p0, L.p1, L.p2);
     L.special |= is_sentinel(L.p0, L.p1, L.p2);
@@ -268,13 +272,13 @@ __device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef&
   return L.rem == 0 ? kCompleted : kActive;
 }
 
-template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact>
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv>
 __device__ __forceinline__ uint8_t step_once(const AdvectArgs& a, Lane& L) {
-  return step_lane<kRecord, kIntegrator, kUnsteady, kExact>(a.f, a.coef, a.t0, L);
+  return step_lane<kRecord, kIntegrator, kUnsteady, kDiv>(a.f, a.coef, a.t0, L);
 }
 
 // Per-request form: one thread per particle, runs it to retirement.
-template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact>
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv>
 __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
   const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t mine = 0;
@@ -284,7 +288,7 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
     if (L.pid != ~0ull) {
       uint8_t st;
       do {
-        st = step_once<kRecord, kIntegrator, kUnsteady, kExact>(a, L);
+        st = step_once<kRecord, kIntegrator, kUnsteady, kDiv>(a, L);
       } while (st == kActive);
       mine = finish_particle<kRecord>(a, L, st);
     }
@@ -297,8 +301,11 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
 // workgroups are dispatched to the XCDs round-robin, so workgroup b starts on
 // part b % parts (its XCD's share of a spatially ordered queue) and moves on to
 // the following parts once its own is drained.
-template <bool kRecord, int kIntegrator, bool kUnsteady, bool kExact>
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv>
 __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
+  // slot index (record) and time (time-variant fields, classic RK4) need the
+  // step count inside the loop; steady dpa-mode endpoints only at the end
+  constexpr bool kCountIt = kRecord || kUnsteady || kIntegrator == kRk4;
   const uint32_t lane = threadIdx.x & 63;
   Lane L;
   cache_clear(L.cc);
@@ -335,9 +342,9 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
     // a wave whose lanes all found a drained part keeps drawing from the next
     if (!__any(busy) && !open) break;
     if (busy) {
-      const uint8_t st = step_once<kRecord, kIntegrator, kUnsteady, kExact>(a, L);
+      const uint8_t st = step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt>(a.f, a.coef, a.t0, L);
       if (st != kActive) {
-        mine += finish_particle<kRecord>(a, L, st);
+        mine += finish_particle<kRecord, kCountIt>(a, L, st);
         L.pid = ~0ull;
       }
     }

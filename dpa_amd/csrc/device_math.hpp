@@ -78,10 +78,6 @@ __device__ __forceinline__ float div_by(float d, float h, float r) {
   return __builtin_fmaf(e, r, q0);
 }
 
-__device__ __forceinline__ float quotient(float d, float h, float r, bool fast, bool exact) {
-  return exact ? d * r : fast ? div_by(d, h, r) : d / h;
-}
-
 // Counts the floats x in [-B, B] (bit patterns 0..hi of |x|, both signs) where
 // div_by(x, h, r) and x / h differ in any bit.  Grid-stride over 2^32-bounded
 // indices; one atomic per mismatch.
@@ -97,24 +93,45 @@ static __global__ __launch_bounds__(256) void verify_division_kernel(float h, fl
   }
 }
 
+// How a launch divides by the grid spacings, chosen per launch from the
+// field's per-axis checks (FieldView::exactdiv / fastdiv) over every axis it
+// locates on (time too, for a time-variant field):
+//   kDivExact  every spacing is a power of two: its reciprocal is exact, so
+//              d * r is the same real number as d / h and rounds the same;
+//   kDivFast   every axis is a power of two or passed verify_division_kernel:
+//              div_by() (on a power-of-two axis its correction is zero);
+//   kDivide    otherwise (or with the tuning's fastdiv off): correctly rounded
+//              d / h.
+// The mode is a template parameter, so the step loop reads no per-axis flags.
+enum DivMode : int { kDivide = 0, kDivExact = 1, kDivFast = 2 };
+
+__host__ __device__ inline int div_mode(uint32_t exactdiv, uint32_t fastdiv, bool unsteady) {
+  const uint32_t axes = unsteady ? 15u : 7u;
+  if ((exactdiv & axes) == axes) return kDivExact;
+  if (((exactdiv | fastdiv) & axes) == axes) return kDivFast;
+  return kDivide;
+}
+
+template <int kDiv>
+__device__ __forceinline__ float div_spacing(float d, float h, float rh) {
+  if constexpr (kDiv == kDivExact) return d * rh;
+  else if constexpr (kDiv == kDivFast) return div_by(d, h, rh);
+  else return d / h;
+}
+
 // One axis of regular_grid::contains (regular_grid.hpp:41-50) and of the
 // weight / start index of regular_grid::interpolate (52-75), split so that a
 // sample computes every axis's floor, tests them together and computes the
-// weights only inside.  `fast` and `exact` are field-uniform.
+// weights only inside.
 //
-// kExact: the caller knows the spacing is a power of two.  Then fl(d * rh) is
-// the same real number as d / h, so s is the exact floor of the quotient (or
-// 0 / -0 for a quotient that underflows, where |d| < h), and fmaf(-s, h, d) is
+// kDivExact: s = floor(d * rh) is the exact floor of the quotient (or 0 / -0
+// for a quotient that underflows, where |d| < h), and fmaf(-s, h, d) is
 // d - s*h exactly: fmod's result, with -0 and tiny negative d returned
-// unchanged as fmodf returns them.  No field flags are read.
-template <bool kExact>
-__device__ __forceinline__ float axis_floor(float d, float h, float rh, bool fast, bool exact) {
-  if constexpr (kExact) {
-    (void)h; (void)fast; (void)exact;
-    return floorf(d * rh);
-  } else {
-    return floorf(quotient(d, h, rh, fast, exact));
-  }
+// unchanged as fmodf returns them.  The other modes take fmod from
+// exact_fmod(), which also handles a rounded quotient one above the floor.
+template <int kDiv>
+__device__ __forceinline__ float axis_floor(float d, float h, float rh) {
+  return floorf(div_spacing<kDiv>(d, h, rh));
 }
 
 // 1 when 0 <= s <= n - 2, else 0 (also for NaN); an integer so that the axes
@@ -123,16 +140,14 @@ __device__ __forceinline__ uint32_t axis_inside(float s, int32_t n) {
   return (s >= 0.0f && s < (float)(n - 1)) ? 1u : 0u;
 }
 
-template <bool kExact>
-__device__ __forceinline__ float axis_weight(float d, float s, float h, float rh, bool fast,
-                                             bool exact) {
-  if constexpr (kExact) {
-    (void)fast; (void)exact;
+template <int kDiv>
+__device__ __forceinline__ float axis_weight(float d, float s, float h, float rh) {
+  if constexpr (kDiv == kDivExact) {
     const float r = __builtin_fmaf(-s, h, d);
     return r * rh;
   } else {
     const float r = exact_fmod(d, h, s);
-    return quotient(r, h, rh, fast, exact);   // r in [0, h) or -0: inside the verified range
+    return div_spacing<kDiv>(r, h, rh);   // r in [0, h) or -0: inside the verified range
   }
 }
 
@@ -264,38 +279,34 @@ __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, con
 
 // contains + interpolate at (p, t).  Returns false when p (or t, for a
 // time-variant field) lies outside; v is the stored (unswizzled) vector.
-// kExact: every axis the sample locates on (time too, for a time-variant
-// field) has a power-of-two spacing (FieldView::exactdiv has all their bits).
-template <bool kUnsteady, bool kExact = false>
+// kDiv: the launch's DivMode.
+template <bool kUnsteady, int kDiv = kDivide>
 __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, float p2, float t,
                                        V3& v, CellCache& cc) {
   const float d0 = p0 - f.o0, d1 = p1 - f.o1, d2 = p2 - f.o2;
-  const bool x0 = f.fastdiv & 1u, x1 = f.fastdiv & 2u, x2 = f.fastdiv & 4u;
-  const bool e0 = f.exactdiv & 1u, e1 = f.exactdiv & 2u, e2 = f.exactdiv & 4u;
   Axis a0, a1, a2;
-  a0.s = axis_floor<kExact>(d0, f.h0, f.r0, x0, e0);
-  a1.s = axis_floor<kExact>(d1, f.h1, f.r1, x1, e1);
-  a2.s = axis_floor<kExact>(d2, f.h2, f.r2, x2, e2);
+  a0.s = axis_floor<kDiv>(d0, f.h0, f.r0);
+  a1.s = axis_floor<kDiv>(d1, f.h1, f.r1);
+  a2.s = axis_floor<kDiv>(d2, f.h2, f.r2);
   // every axis tested together: one branch instead of one per axis
   uint32_t in = axis_inside(a0.s, f.n0) & axis_inside(a1.s, f.n1) & axis_inside(a2.s, f.n2);
   if constexpr (kUnsteady) {
     const float dt = t - f.to;
-    const bool xt = f.fastdiv & 8u, et = f.exactdiv & 8u;
     Axis at;
-    at.s = axis_floor<kExact>(dt, f.th, f.rt, xt, et);
+    at.s = axis_floor<kDiv>(dt, f.th, f.rt);
     in = in & axis_inside(at.s, f.nt);
     if (!in) return false;
-    a0.w = axis_weight<kExact>(d0, a0.s, f.h0, f.r0, x0, e0);
-    a1.w = axis_weight<kExact>(d1, a1.s, f.h1, f.r1, x1, e1);
-    a2.w = axis_weight<kExact>(d2, a2.s, f.h2, f.r2, x2, e2);
-    at.w = axis_weight<kExact>(dt, at.s, f.th, f.rt, xt, et);
+    a0.w = axis_weight<kDiv>(d0, a0.s, f.h0, f.r0);
+    a1.w = axis_weight<kDiv>(d1, a1.s, f.h1, f.r1);
+    a2.w = axis_weight<kDiv>(d2, a2.s, f.h2, f.r2);
+    at.w = axis_weight<kDiv>(dt, at.s, f.th, f.rt);
     v = quadlinear(f, a0, a1, a2, at, cc);
   } else {
     (void)t;
     if (!in) return false;
-    a0.w = axis_weight<kExact>(d0, a0.s, f.h0, f.r0, x0, e0);
-    a1.w = axis_weight<kExact>(d1, a1.s, f.h1, f.r1, x1, e1);
-    a2.w = axis_weight<kExact>(d2, a2.s, f.h2, f.r2, x2, e2);
+    a0.w = axis_weight<kDiv>(d0, a0.s, f.h0, f.r0);
+    a1.w = axis_weight<kDiv>(d1, a1.s, f.h1, f.r1);
+    a2.w = axis_weight<kDiv>(d2, a2.s, f.h2, f.r2);
     v = trilinear(f, f.data, a0, a1, a2, cc);
   }
   return true;
@@ -342,7 +353,7 @@ __device__ __forceinline__ void step_dpa(float& p0, float& p1, float& p2, const 
 // Classic four-sample RK4 in odeint's explicit_generic_rk order (not dpa's
 // mode; an extension).  A stage point outside the grid returns false and leaves
 // the position unchanged.
-template <bool kUnsteady, bool kExact = false>
+template <bool kUnsteady, int kDiv = kDivide>
 __device__ __forceinline__ bool step_rk4(const FieldView& f, float& p0, float& p1, float& p2,
                                          const V3& v, float t, const StepCoef& c, CellCache& cc) {
   const float k10 = v.c, k11 = v.b, k12 = v.a;
@@ -351,20 +362,20 @@ __device__ __forceinline__ bool step_rk4(const FieldView& f, float& p0, float& p
   { const float s = c.half_dt * k11; x1 = p1 + s; }
   { const float s = c.half_dt * k12; x2 = p2 + s; }
   V3 s2;
-  if (!sample<kUnsteady, kExact>(f, x0, x1, x2, t + c.half_dt, s2, cc)) return false;
+  if (!sample<kUnsteady, kDiv>(f, x0, x1, x2, t + c.half_dt, s2, cc)) return false;
   const float k20 = s2.c, k21 = s2.b, k22 = s2.a;
   { float y = p0; const float a = c.zero_dt * k10; y = y + a; const float b = c.half_dt * k20; x0 = y + b; }
   { float y = p1; const float a = c.zero_dt * k11; y = y + a; const float b = c.half_dt * k21; x1 = y + b; }
   { float y = p2; const float a = c.zero_dt * k12; y = y + a; const float b = c.half_dt * k22; x2 = y + b; }
   V3 s3;
-  if (!sample<kUnsteady, kExact>(f, x0, x1, x2, t + c.half_dt, s3, cc)) return false;
+  if (!sample<kUnsteady, kDiv>(f, x0, x1, x2, t + c.half_dt, s3, cc)) return false;
   const float k30 = s3.c, k31 = s3.b, k32 = s3.a;
   const float one_dt = 1.0f * c.dt;
   { float y = p0; const float a = c.zero_dt * k10; y = y + a; const float b = c.zero_dt * k20; y = y + b; const float d = one_dt * k30; x0 = y + d; }
   { float y = p1; const float a = c.zero_dt * k11; y = y + a; const float b = c.zero_dt * k21; y = y + b; const float d = one_dt * k31; x1 = y + d; }
   { float y = p2; const float a = c.zero_dt * k12; y = y + a; const float b = c.zero_dt * k22; y = y + b; const float d = one_dt * k32; x2 = y + d; }
   V3 s4;
-  if (!sample<kUnsteady, kExact>(f, x0, x1, x2, t + one_dt, s4, cc)) return false;
+  if (!sample<kUnsteady, kDiv>(f, x0, x1, x2, t + one_dt, s4, cc)) return false;
   const float k40 = s4.c, k41 = s4.b, k42 = s4.a;
   auto fin = [&](float x, float q1, float q2, float q3, float q4) {
     const float a = c.a16 * q1; x = x + a;

@@ -72,15 +72,24 @@ uint64_t ftle_points(const dpah_request& r) {
   return r.lattice_shape[0] * r.lattice_shape[1] * r.lattice_shape[2];
 }
 
-template <bool kRecord, int kIntegrator, bool kUnsteady>
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv>
 void launch_tick_t(dpah_context* ctx, const TickArgs& a, hipStream_t s) {
   const uint64_t grid = (a.n + 255) / 256;
   DPAH_REQUIRE(grid < (1ull << 31), "too many particles in one launch");
   ctx->timed("advect_tick", s, [&] {
-    hipLaunchKernelGGL((k_advect_tick<kRecord, kIntegrator, kUnsteady>), dim3((unsigned)grid),
+    hipLaunchKernelGGL((k_advect_tick<kRecord, kIntegrator, kUnsteady, kDiv>), dim3((unsigned)grid),
                        dim3(256), 0, s, a);
   });
   DPAH_HIP(hipGetLastError());
+}
+
+template <bool kRecord, int kIntegrator, bool kUnsteady>
+void launch_tick_div(dpah_context* ctx, const TickArgs& a, hipStream_t s) {
+  switch (div_mode(a.f.exactdiv, a.f.fastdiv, kUnsteady)) {
+    case kDivExact: launch_tick_t<kRecord, kIntegrator, kUnsteady, kDivExact>(ctx, a, s); break;
+    case kDivFast:  launch_tick_t<kRecord, kIntegrator, kUnsteady, kDivFast>(ctx, a, s); break;
+    default:        launch_tick_t<kRecord, kIntegrator, kUnsteady, kDivide>(ctx, a, s); break;
+  }
 }
 
 void launch_tick(dpah_context* ctx, const TickArgs& a, int integrator, bool record,
@@ -89,15 +98,15 @@ void launch_tick(dpah_context* ctx, const TickArgs& a, int integrator, bool reco
   const bool unsteady = a.f.nt > 1;
   if (record) {
     if (integrator == kRk4) {
-      if (unsteady) launch_tick_t<true, kRk4, true>(ctx, a, s); else launch_tick_t<true, kRk4, false>(ctx, a, s);
+      if (unsteady) launch_tick_div<true, kRk4, true>(ctx, a, s); else launch_tick_div<true, kRk4, false>(ctx, a, s);
     } else {
-      if (unsteady) launch_tick_t<true, kDpaRk4, true>(ctx, a, s); else launch_tick_t<true, kDpaRk4, false>(ctx, a, s);
+      if (unsteady) launch_tick_div<true, kDpaRk4, true>(ctx, a, s); else launch_tick_div<true, kDpaRk4, false>(ctx, a, s);
     }
   } else {
     if (integrator == kRk4) {
-      if (unsteady) launch_tick_t<false, kRk4, true>(ctx, a, s); else launch_tick_t<false, kRk4, false>(ctx, a, s);
+      if (unsteady) launch_tick_div<false, kRk4, true>(ctx, a, s); else launch_tick_div<false, kRk4, false>(ctx, a, s);
     } else {
-      if (unsteady) launch_tick_t<false, kDpaRk4, true>(ctx, a, s); else launch_tick_t<false, kDpaRk4, false>(ctx, a, s);
+      if (unsteady) launch_tick_div<false, kDpaRk4, true>(ctx, a, s); else launch_tick_div<false, kDpaRk4, false>(ctx, a, s);
     }
   }
 }

@@ -84,7 +84,7 @@ __device__ __forceinline__ void tally_flush_wave(const TickArgs& a, Tally& t) {
 // until all of them are done.  Segments start at multiples of 64, so a wave
 // never straddles two: the segment is wave-uniform and its pointers live in
 // scalar registers, which keeps occupancy at the per-request kernel's level.
-template <bool kRecord, int kIntegrator, bool kUnsteady>
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv>
 static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
   const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Tally tally{~0u, 0u, 0ull};
@@ -111,7 +111,7 @@ static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
       const uint32_t stop = L.it + a.budget;
       uint8_t st;
       do {
-        st = step_lane<kRecord, kIntegrator, kUnsteady>(a.f, coef, s.t0, L);
+        st = step_lane<kRecord, kIntegrator, kUnsteady, kDiv>(a.f, coef, s.t0, L);
       } while (st == kActive && L.it != stop);
       s.pos[3 * i] = L.p0; s.pos[3 * i + 1] = L.p1; s.pos[3 * i + 2] = L.p2;
       s.rem[i] = L.rem;

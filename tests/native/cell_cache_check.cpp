@@ -2,9 +2,12 @@
 // (kept across steps) must equal, bit for bit, a sample through an empty cache
 // (full eight-corner load) that divides by the spacings, for steady and
 // time-variant fields, over random walks that cross cells in every direction.
-// On grids whose spacings are all powers of two the warm sample also takes the
-// specialised path (reciprocal multiply and one fmaf per axis, no field flags),
-// so that path is pinned against the correctly rounded divisions and fmodf.
+// The warm sample runs each division mode a launch can take: plain division,
+// the reciprocal with one exact-residual correction (kDivFast, on power-of-two
+// and on a non-power-of-two grid), and on grids whose spacings are all powers
+// of two the reciprocal multiply with fmod as one fmaf (kDivExact); so every
+// mode is pinned against the correctly rounded divisions and fmodf, also at
+// boundary probes (signed zeros, denormals, faces, the last cell).
 // Prints "samples S shifts F diffs D" and exits non-zero on a diff.
 #include <hip/hip_runtime.h>
 #undef __device__
@@ -19,10 +22,9 @@
 
 using namespace dpah;
 
-template <bool kUnsteady, bool kExact>
+template <bool kUnsteady, int kDiv>
 static long walk(const FieldView& f, std::mt19937& rng, long& samples, long& shifts) {
-  FieldView plain = f;   // the cold reference divides on every axis
-  plain.fastdiv = plain.exactdiv = 0;
+  const FieldView& plain = f;   // the cold reference divides on every axis
   std::uniform_real_distribution<float> u(-1.0f, 1.0f);
   std::uniform_real_distribution<float> p0(0.0f, (float)(f.n0 - 1)), p1(0.0f, (float)(f.n1 - 1)),
       p2(0.0f, (float)(f.n2 - 1)), pt(0.0f, (float)(f.nt - 1));
@@ -38,8 +40,8 @@ static long walk(const FieldView& f, std::mt19937& rng, long& samples, long& shi
       V3 a{}, b{};
       CellCache cold;
       cache_clear(cold);
-      const bool ok = sample<kUnsteady, kExact>(f, x0, x1, x2, t, a, warm);
-      const bool ok2 = sample<kUnsteady, false>(plain, x0, x1, x2, t, b, cold);
+      const bool ok = sample<kUnsteady, kDiv>(f, x0, x1, x2, t, a, warm);
+      const bool ok2 = sample<kUnsteady, kDivide>(plain, x0, x1, x2, t, b, cold);
       if (ok != ok2) { ++diffs; break; }
       if (!ok) break;
       ++samples;
@@ -57,9 +59,9 @@ static long walk(const FieldView& f, std::mt19937& rng, long& samples, long& shi
 // others mid-grid: zeros of both signs, denormals, tiny negatives, exact cell
 // faces, the last valid cell and just past it.  Returns the number of samples
 // (inside flag or value bits) that differ from the dividing path.
+template <int kDiv>
 static long probes(const FieldView& e, long& samples) {
-  FieldView plain = e;
-  plain.fastdiv = plain.exactdiv = 0;
+  const FieldView& plain = e;
   const float tiny[] = {0.0f, -0.0f, 1e-45f, -1e-45f, 1e-38f, -1e-38f, 1e-30f, -1e-30f,
                         -1e-7f, 1e-7f};
   long diffs = 0;
@@ -83,8 +85,8 @@ static long probes(const FieldView& e, long& samples) {
       CellCache c1, c2;
       cache_clear(c1);
       cache_clear(c2);
-      const bool ok = sample<false, true>(e, p[0], p[1], p[2], 0.0f, a, c1);
-      const bool ok2 = sample<false, false>(plain, p[0], p[1], p[2], 0.0f, b, c2);
+      const bool ok = sample<false, kDiv>(e, p[0], p[1], p[2], 0.0f, a, c1);
+      const bool ok2 = sample<false, kDivide>(plain, p[0], p[1], p[2], 0.0f, b, c2);
       ++samples;
       if (ok != ok2 || (ok && std::memcmp(&a, &b, sizeof a) != 0)) ++diffs;
     }
@@ -110,23 +112,27 @@ int main() {
   FieldView unit = f;
   unit.o0 = unit.o1 = unit.o2 = 0.0f; unit.h0 = unit.h1 = unit.h2 = 1.0f;
   unit.r0 = unit.r1 = unit.r2 = 1.0f;
-  long diffs = walk<false, false>(unit, rng, samples, shifts);
-  diffs += walk<false, false>(f, rng, samples, shifts);
+  long diffs = walk<false, kDivide>(unit, rng, samples, shifts);
+  diffs += walk<false, kDivide>(f, rng, samples, shifts);
+  diffs += walk<false, kDivFast>(f, rng, samples, shifts);   // 0.75 is not a power of two
   FieldView g = unit;
   g.nt = nt;
-  diffs += walk<true, false>(g, rng, samples, shifts);
-  // power-of-two spacings (0.5, 2, 0.25, time 0.5) through the specialised path
+  diffs += walk<true, kDivide>(g, rng, samples, shifts);
+  diffs += walk<true, kDivFast>(g, rng, samples, shifts);
+  // power-of-two spacings (0.5, 2, 0.25, time 0.5)
   FieldView e = f;
   e.h0 = 0.5f; e.h1 = 2.0f; e.h2 = 0.25f; e.th = 0.5f;
   e.r0 = 2.0f; e.r1 = 0.5f; e.r2 = 4.0f; e.rt = 2.0f;
-  e.exactdiv = 15u;
-  diffs += walk<false, true>(unit, rng, samples, shifts);
-  diffs += walk<false, true>(e, rng, samples, shifts);
+  diffs += walk<false, kDivExact>(unit, rng, samples, shifts);
+  diffs += walk<false, kDivExact>(e, rng, samples, shifts);
+  diffs += walk<false, kDivFast>(e, rng, samples, shifts);
   FieldView et = e;
   et.nt = nt;
-  diffs += walk<true, true>(et, rng, samples, shifts);
-  diffs += probes(e, samples);
-  diffs += probes(unit, samples);
+  diffs += walk<true, kDivExact>(et, rng, samples, shifts);
+  diffs += probes<kDivExact>(e, samples);
+  diffs += probes<kDivExact>(unit, samples);
+  diffs += probes<kDivFast>(e, samples);
+  diffs += probes<kDivFast>(f, samples);
   std::printf("samples %ld shifts %ld diffs %ld\n", samples, shifts, diffs);
   return diffs != 0;
 }

@@ -40,10 +40,17 @@ def test_checker_on_exact_and_wrong_reciprocals(ctx):
         say(f"  h={float(h):.9g}: {n} mismatches")
 
 
+# every axis verified (reciprocal with one correction); power-of-two and verified
+# axes mixed (the same form); every axis a power of two (exact reciprocals, fmod
+# as one fmaf).  With fastdiv 0 every launch divides.
+SPACINGS = [(0.75, float(f32(0.1)), float(f32(2 * math.pi / 511))), (0.5, 0.75, 1.0),
+            (0.5, 2.0, 0.25)]
+
+
+@pytest.mark.parametrize("spacing", SPACINGS)
 @pytest.mark.parametrize("fastdiv", [0, 1])
-def test_advect_matches_oracle_with_and_without(ctx, fastdiv):
-    say(f"case fastdiv={fastdiv}")
-    spacing = (0.75, float(f32(0.1)), float(f32(2 * math.pi / 511)))
+def test_advect_matches_oracle_with_and_without(ctx, fastdiv, spacing):
+    say(f"case fastdiv={fastdiv} spacing={spacing}")
     offset = (0.25, -0.5, 1.0)
     data = W.vortex_field(36, count=10)
     before = ctx.tuning()
