@@ -49,17 +49,17 @@ struct Axis {
 // fmodf(d, h) for d >= 0 (or -0) and h > 0, given s = floorf(fl(d / h)).
 // fl(d/h) is monotone in the exact quotient, so s is the exact floor k or k+1.
 // When s == k the exact remainder d - k*h is representable (it is fmod's
-// result), so one fmaf returns it without rounding; when s == k+1 the fmaf
-// result is negative (rounding keeps the sign of a non-zero value) and the
-// second fmaf with k is exact.  s == 0 returns d itself, keeping -0.
+// result), so one fmaf returns it without rounding -- for s = +-0 that is d
+// itself, signed zero included; when s == k+1 the fmaf result is negative
+// (rounding keeps the sign of a non-zero value) and the fmaf with k is exact.
+// s = +-0 with d < 0 (a tiny negative d whose quotient underflowed to -0)
+// keeps d, as fmodf does.  Both candidates are computed and one selected: no
+// branch.
 __device__ __forceinline__ float exact_fmod(float d, float h, float s) {
-  if (s == 0.0f) return d;
-  float r = __builtin_fmaf(-s, h, d);
-  if (r < 0.0f) {
-    const float k = s - 1.0f;
-    r = (k == 0.0f) ? d : __builtin_fmaf(-k, h, d);
-  }
-  return r;
+  const float r = __builtin_fmaf(-s, h, d);
+  const float k = s - 1.0f;
+  const float r1 = __builtin_fmaf(-k, h, d);
+  return (r < 0.0f && s != 0.0f) ? r1 : r;
 }
 
 // d / h from r = fl(1/h): q0 = fl(d*r), the residual d - q0*h is exact in one
@@ -210,16 +210,17 @@ __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
       const float4* rec = f.cells + 8 * cell_record(f, 0, c0, c1, c2);
       load_cell(rec, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
     } else {
+      // one base address per reload; the four rows sit at launch-uniform
+      // distances and each row's two nodes 12 bytes apart
       const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
-      const int64_t b = (int64_t)c0 * n12 + (int64_t)c1 * n2 + c2;
-      cc.L0 = node(f, slice, b);
-      cc.L1 = node(f, slice, b + 1);
-      cc.L2 = node(f, slice, b + n2);
-      cc.L3 = node(f, slice, b + n2 + 1);
-      cc.L4 = node(f, slice, b + n12);
-      cc.L5 = node(f, slice, b + n12 + 1);
-      cc.L6 = node(f, slice, b + n12 + n2);
-      cc.L7 = node(f, slice, b + n12 + n2 + 1);
+      const float* p0 = slice + 3 * ((int64_t)c0 * n12 + (int64_t)c1 * n2 + c2);
+      const float* p1 = p0 + 3 * n2;
+      const float* p2 = p0 + 3 * n12;
+      const float* p3 = p2 + 3 * n2;
+      cc.L0 = V3{p0[0], p0[1], p0[2]}; cc.L1 = V3{p0[3], p0[4], p0[5]};
+      cc.L2 = V3{p1[0], p1[1], p1[2]}; cc.L3 = V3{p1[3], p1[4], p1[5]};
+      cc.L4 = V3{p2[0], p2[1], p2[2]}; cc.L5 = V3{p2[3], p2[4], p2[5]};
+      cc.L6 = V3{p3[0], p3[1], p3[2]}; cc.L7 = V3{p3[3], p3[4], p3[5]};
     }
     cc.s0 = a0.s; cc.s1 = a1.s; cc.s2 = a2.s;
   }

@@ -133,6 +133,16 @@ int main() {
   diffs += probes<kDivExact>(unit, samples);
   diffs += probes<kDivFast>(e, samples);
   diffs += probes<kDivFast>(f, samples);
+  // zero origins, so that the tiny probes reach the divisions unabsorbed (a
+  // tiny negative d whose quotient underflows to -0 stays inside and keeps d
+  // as its fmod)
+  FieldView e0 = e, f0 = f;
+  e0.o0 = e0.o1 = e0.o2 = 0.0f;
+  f0.o0 = f0.o1 = f0.o2 = 0.0f;
+  diffs += probes<kDivExact>(e0, samples);
+  diffs += probes<kDivFast>(e0, samples);
+  diffs += probes<kDivFast>(f0, samples);
+  diffs += probes<kDivFast>(unit, samples);
   std::printf("samples %ld shifts %ld diffs %ld\n", samples, shifts, diffs);
   return diffs != 0;
 }
