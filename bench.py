@@ -11,7 +11,7 @@ remaining iterations and status.  A short served-query leg (continuous
 batching vs one launch per request on the same Poisson trace) adds p50/p99
 latency.
 
-Other configurations: --workload cfg1 | cfg3 | cfg3-dpa | cfg4 | cfg5.
+Other workloads: --workload abc64-record | ftle-aux | ftle-lattice | batch64 | serve.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -198,13 +198,13 @@ def serving_leg(ctx, fields, trace, steps_per_tick=0):
 
 # --------------------------------------------------------------- workloads
 
-def run_cfg2(args):
+def run_vortex256(args):
     import dpa_amd
     from dpa_amd import workloads as W
 
     t = time.perf_counter()
     field = W.vortex_field(256)
-    seeds = W.cfg2_seeds(args.seeds)
+    seeds = W.vortex_seeds(args.seeds)
     log(f"inputs ready in {time.perf_counter() - t:.1f} s")
     dt = np.float32(0.25)
     n = len(seeds)
@@ -337,11 +337,11 @@ def base_line(args, value, elapsed, dtype):
             "data": "synthetic (seeded fields and seeds; no dataset)"}
 
 
-def run_cfg1(args):
+def run_abc64(args):
     """ABC 64^3, 4096 seeds x 256 steps, record + prune + indices + colours."""
     import dpa_amd
     from dpa_amd import workloads as W
-    c = W.cfg1()
+    c = W.abc64_streamlines()
     ctx = make_context(args)
     fld = ctx.field(c["field"], c["offset"], c["spacing"])
 
@@ -368,7 +368,7 @@ def run_cfg1(args):
     value = args.steps * steps / elapsed
     adv_ms = ta / max(na, 1)
     out = base_line(args, value, elapsed, "f32")
-    out["config"] = {"workload": "cfg1 ABC 64^3, 4096 seeds x 256 steps dt 0.05 h, record, u32; "
+    out["config"] = {"workload": "ABC 64^3, 4096 seeds x 256 steps dt 0.05 h, record, u32; "
                                  "value includes host upload / download and allocation",
                      "particle_steps_per_pass": steps}
     out["kernels"] = {"advect_persistent_ms": adv_ms, "compact_write_ms": tc / max(nc, 1),
@@ -387,12 +387,12 @@ def run_cfg1(args):
     return out
 
 
-def run_cfg3(args, dpa_mode=False):
+def run_ftle(args, dpa_mode=False):
     """FTLE on ABC 512x512x128: auxiliary-neighbour form (default) or dpa's
     lattice estimator."""
     import dpa_amd
     from dpa_amd import workloads as W
-    c = W.cfg3()
+    c = W.ftle_abc512()
     ctx = make_context(args)
     fld = ctx.field(c["field"], c["offset"], c["spacing"])
     shape = c["lattice_shape"]
@@ -421,8 +421,8 @@ def run_cfg3(args, dpa_mode=False):
     ctx.close()
     adv_ms = ta / max(na, 1)
     out = base_line(args, args.steps * steps / elapsed, elapsed, "f32")
-    out["config"] = {"workload": ("cfg3 FTLE dpa lattice mode" if dpa_mode else
-                                  "cfg3 FTLE auxiliary +-0.5 cell neighbours") +
+    out["config"] = {"workload": ("FTLE dpa lattice mode" if dpa_mode else
+                                  "FTLE auxiliary +-0.5 cell neighbours") +
                                  ", ABC 512x512x128, T=4 at dt 0.02",
                      "lattice_points": points, "particle_steps_per_pass": steps,
                      "points_per_s": args.steps * points / elapsed}
@@ -431,13 +431,13 @@ def run_cfg3(args, dpa_mode=False):
     return out
 
 
-def run_cfg4(args):
+def run_batch(args):
     """64 uneven recorded jobs on the vortex field: one persistent launch
     (dpah_advect_batch) against the per-request form on the same jobs."""
     import dpa_amd
     from dpa_amd import workloads as W
     field = W.vortex_field(256)
-    jobs = W.cfg4_jobs()
+    jobs = W.batch_jobs()
     ctx = make_context(args)
     fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
     for j in jobs:
@@ -472,7 +472,7 @@ def run_cfg4(args):
     ctx.close()
     value = args.steps * steps / elapsed
     out = base_line(args, value, elapsed, "f32")
-    out["config"] = {"workload": "cfg4 64 recorded jobs (256..262144 seeds, 64..4096 steps) on "
+    out["config"] = {"workload": "64 recorded jobs (256..262144 seeds, 64..4096 steps) on "
                                  "the 256^3 vortex field, one persistent launch + compaction; "
                                  "host upload / download included",
                      "jobs": len(jobs), "seeds": int(sum(len(j['seeds']) for j in jobs)),
@@ -485,7 +485,7 @@ def run_cfg4(args):
     return out
 
 
-def run_cfg5(args):
+def run_serve(args):
     """4 resident time-variant fields (16 slices of 256^3), Poisson traces at
     several rates, continuous batching vs per-request launch."""
     import dpa_amd
@@ -511,7 +511,7 @@ def run_cfg5(args):
     best = max(s["batched"]["throughput"] for s in sweep)
     out = base_line(args, best, 0.0, "f32")
     out["steps"] = 0
-    out["config"] = {"workload": "cfg5 served pathlines + FTLE slabs on 4 resident 256^3 x 16 "
+    out["config"] = {"workload": "served pathlines + FTLE slabs on 4 resident 256^3 x 16 "
                                  "time-variant fields (12.9 GB)", "requests_per_rate": args.serve_requests}
     out["sweep"] = sweep
     return out
@@ -522,8 +522,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg2",
-                    choices=["cfg1", "cfg2", "cfg3", "cfg3-dpa", "cfg4", "cfg5"])
+    ap.add_argument("--workload", default="vortex256",
+                    choices=["abc64-record", "vortex256", "ftle-aux", "ftle-lattice", "batch64",
+                             "serve"])
     ap.add_argument("--seeds", type=int, default=1 << 20)
     ap.add_argument("--iterations", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -545,12 +546,12 @@ def main() -> int:
                     help="skip the launch-shape autotuner before the timed passes")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(seeds shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path here (with the field "
+              "replicated, seeds would shard across GPUs with no exchange)", file=sys.stderr)
         return 2
-    run = {"cfg1": run_cfg1, "cfg2": run_cfg2, "cfg3": run_cfg3,
-           "cfg3-dpa": lambda a: run_cfg3(a, dpa_mode=True), "cfg4": run_cfg4,
-           "cfg5": run_cfg5}[args.workload]
+    run = {"abc64-record": run_abc64, "vortex256": run_vortex256, "ftle-aux": run_ftle,
+           "ftle-lattice": lambda a: run_ftle(a, dpa_mode=True), "batch64": run_batch,
+           "serve": run_serve}[args.workload]
     out = run(args)
     print(json.dumps(out), flush=True)
     return 0

@@ -132,7 +132,7 @@ def uniform_seeds(count: int, lo, hi, seed_offset: int) -> np.ndarray:
 
 # ------------------------------------------------------------- configs ----
 
-def cfg1():
+def abc64_streamlines():
     """Plain steady streamlines: ABC on 64^3 over [0, 2pi]^3, 4096 seeds,
     dt = 0.05 cell (0.05 h in time units), 256 iterations, record, uint32."""
     field, h = abc_field((64, 64, 64))
@@ -143,19 +143,19 @@ def cfg1():
                 particles_per_round=4096)
 
 
-def cfg2_seeds(count=1 << 20):
+def vortex_seeds(count=1 << 20):
     return uniform_seeds(count, (0, 0, 0), (255, 255, 255), seed_offset=2)
 
 
-def cfg2(count=1 << 20, n=256):
+def vortex256_endpoints(count=1 << 20, n=256):
     """Large steady endpoint advection: vortex field 256^3 (h = 1), 2^20 seeds
     uniform in [0, 255)^3, dt = 0.25, 1000 iterations, endpoints only."""
     return dict(field=vortex_field(n), spacing=np.ones(3, np.float32),
-                offset=np.zeros(3, np.float32), seeds=cfg2_seeds(count), iterations=1000,
+                offset=np.zeros(3, np.float32), seeds=vortex_seeds(count), iterations=1000,
                 step_size=np.float32(0.25), record=False)
 
 
-def cfg3(shape=(128, 512, 512)):
+def ftle_abc512(shape=(128, 512, 512)):
     """FTLE: ABC on 512 x 512 x 128 nodes (z = axis 0 has 128, extent 127 h)
     with isotropic spacing h = 2 pi / 511 (origin 0); every node is a lattice
     point with auxiliary neighbours at +-0.5 h; dt 0.02 for 200 steps (T = 4)."""
@@ -167,7 +167,7 @@ def cfg3(shape=(128, 512, 512)):
                 iterations=200, step_size=np.float32(0.02))
 
 
-def cfg4_jobs(count=64, n=256, seed_offset=4):
+def batch_jobs(count=64, n=256, seed_offset=4):
     """64 jobs on the 256^3 vortex field: seed counts round(exp(U[ln 256,
     ln 262144])), steps uniform integer in [64, 4096], seeds uniform in a random
     axis-aligned box (extent U[16, 128] per axis) inside [0, 255)^3; full curves

@@ -69,9 +69,9 @@ def run_both(ctx, field_np, offset, spacing, seeds, remaining, ppr, dt, integrat
         fld.release()
 
 
-def test_cfg1_abc_streamlines_bit_exact(ctx):
-    say("case cfg1: ABC 64^3, 4096 seeds x 256 steps, record, u32")
-    c = W.cfg1()
+def test_abc64_streamlines_bit_exact(ctx):
+    say("case ABC 64^3, 4096 seeds x 256 steps, record, u32")
+    c = W.abc64_streamlines()
     ora = run_both(ctx, c["field"], c["offset"], c["spacing"], c["seeds"], c["iterations"],
                    c["particles_per_round"], c["step_size"])
     assert ora["total_steps"] > 100000
@@ -79,7 +79,7 @@ def test_cfg1_abc_streamlines_bit_exact(ctx):
 
 def test_rounds_tail_first_uneven_remaining(ctx):
     say("case rounds: 1000 seeds, ppr 300, remaining 0..299")
-    c = W.cfg1()
+    c = W.abc64_streamlines()
     rng = np.random.default_rng(11)
     seeds = c["seeds"][:1000]
     rem = rng.integers(0, 300, 1000)
@@ -89,7 +89,7 @@ def test_rounds_tail_first_uneven_remaining(ctx):
 
 def test_u64_indices(ctx):
     say("case u64 indices")
-    c = W.cfg1()
+    c = W.abc64_streamlines()
     run_both(ctx, c["field"], c["offset"], c["spacing"], c["seeds"][:700], 64, 256,
              c["step_size"], use64=True)
 
@@ -113,7 +113,7 @@ def test_sentinel_valued_vertices_and_edges(ctx):
 
 def test_empty_and_zero_remaining(ctx):
     say("case empty input / zero remaining")
-    c = W.cfg1()
+    c = W.abc64_streamlines()
     run_both(ctx, c["field"], c["offset"], c["spacing"], np.zeros((0, 3), f32), 10, 5,
              c["step_size"])
     run_both(ctx, c["field"], c["offset"], c["spacing"], c["seeds"][:50], 0, 50, c["step_size"])
@@ -129,7 +129,7 @@ def test_vortex_endpoints_bit_exact(ctx):
 
 def test_rk4_mode(ctx):
     say("case classic rk4, record")
-    c = W.cfg1()
+    c = W.abc64_streamlines()
     run_both(ctx, c["field"], c["offset"], c["spacing"], c["seeds"][:512], 200, 512,
              c["step_size"], integrator=po.RK4)
 
@@ -145,7 +145,7 @@ def test_unsteady_pathlines(ctx):
 
 def test_endpoints_device_path(ctx):
     say("case device-resident endpoints")
-    c = W.cfg1()
+    c = W.abc64_streamlines()
     fld = ctx.field(c["field"], c["offset"], c["spacing"])
     seeds = c["seeds"]
     n = len(seeds)
@@ -171,12 +171,12 @@ def test_endpoints_device_path(ctx):
         fld.release()
 
 
-def test_cfg2_full_field_sample_and_full_run_properties(ctx):
-    say("case cfg2: vortex 256^3; 16384-seed sample vs oracle; full 2^20 run properties")
+def test_vortex256_sample_and_full_run_properties(ctx):
+    say("case vortex 256^3; 16384-seed sample vs oracle; full 2^20 run properties")
     field = W.vortex_field(256)
     fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
     ofld = po.Field(field, (0, 0, 0), (1, 1, 1))
-    seeds = W.cfg2_seeds()
+    seeds = W.vortex_seeds()
     n = len(seeds)
     d_seeds = ctx.to_device(seeds)
     d_pos = ctx.alloc(seeds.nbytes)
