@@ -116,6 +116,8 @@ bool brick_launch(const Tuning& T, const AdvectLaunch& L) {
   if (!(L.persistent && T.order && L.nseg == 0 && L.slots == nullptr && L.lat.first == 0))
     return false;
   const uint64_t points = L.lat.d0 * L.lat.d1 * L.lat.d2;
+  // the walk's 32-bit arithmetic: positions and a whole slab of bricks
+  if ((uint64_t)kLatticeBrick * L.lat.d1 * L.lat.d2 >= (1ull << 32)) return false;
   if (L.seed_kind == kSeedLattice) return L.n == points && L.n < (1ull << 32);
   if (L.seed_kind == kSeedAux) return L.n == 6 * points && L.n < (1ull << 32);
   return false;

@@ -33,13 +33,14 @@ struct SeedLattice {
 };
 
 // Queue position q -> C-order index of the lattice point it starts, when the
-// queue walks a d0 x d1 x d2 lattice (fewer than 2^32 points) in bricks of
-// B^3 points: bricks in C order, points in C order inside a brick, the bricks
-// on the far faces cut to the lattice.  Every brick before the last along an
-// axis is whole, so each level of the walk is one division: the slab of
-// bricks along axis 0, the row of bricks along axis 1 inside it, the brick
-// along axis 2 inside that, then the point inside the brick.  With B = 4 a
-// wave that draws 64 aligned positions starts a 4x4x4 block of neighbours.
+// queue walks a d0 x d1 x d2 lattice (fewer than 2^32 points, and B*d1*d2 below
+// 2^32) in bricks of B^3 points: bricks in C order, points in C order inside a
+// brick, the bricks on the far faces cut to the lattice.  Every brick before
+// the last along an axis is whole, so each level of the walk is one division:
+// the slab of bricks along axis 0, the row of bricks along axis 1 inside it,
+// the brick along axis 2 inside that, then the point inside the brick.  With
+// B = 4 a wave that draws 64 aligned positions starts a 4x4x4 block of
+// neighbours.
 __host__ __device__ inline uint64_t lattice_brick_index(uint32_t d0, uint32_t d1, uint32_t d2,
                                                         uint32_t B, uint32_t q) {
   const uint32_t slab = B * d1 * d2;
