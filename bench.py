@@ -321,7 +321,8 @@ def make_context(args):
     ctx = dpa_amd.Context(0, kernel_timing=True)
     over = {k: v for k, v in (("order", args.order), ("blocks_per_cu", args.blocks_per_cu),
                               ("block", args.block), ("refill", args.refill),
-                              ("cells", args.cells), ("fastdiv", args.fastdiv)) if v is not None}
+                              ("cells", args.cells), ("fastdiv", args.fastdiv),
+                              ("cell_brick", args.cell_brick)) if v is not None}
     if over:
         ctx.set_tuning(**over)
         args.autotune = False   # explicit launch shapes are measured as given
@@ -540,6 +541,8 @@ def main() -> int:
     ap.add_argument("--refill", type=int, default=None)
     ap.add_argument("--cells", type=int, choices=[0, 1, 2], default=None)
     ap.add_argument("--fastdiv", type=int, choices=[0, 1], default=None)
+    ap.add_argument("--cell-brick", type=int, choices=[0, 2, 3, 4, 5], default=None,
+                    help="cell layout record order of the fields (0: C order; k: 2^k-cell bricks)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="skip the hipGraph replay leg")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",

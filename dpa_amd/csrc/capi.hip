@@ -283,16 +283,36 @@ DPAH_EXPORT int dpah_copy_to_host(dpah_context* ctx, void* dst, const void* src,
 
 // -------------------------------------------------------------- fields ----
 
-// One thread per cell record: gathers the cell's eight corners from the node
-// array and writes them, in corner order 4a+2b+c, as one 128-B record.
+// One thread per cell record: finds the record's cell, gathers its eight
+// corners from the node array and writes them, in corner order 4a+2b+c, as one
+// 128-B record.  Padding records of a brick-ordered layout (cells beyond the
+// grid) are left unwritten: no sample inside the grid reads them.
 static __global__ __launch_bounds__(256) void k_build_cells(FieldView v, float4* out,
                                                             uint64_t records) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= records) return;
   const uint64_t m2 = (uint64_t)v.n2 - 1, m1 = (uint64_t)v.n1 - 1, m0 = (uint64_t)v.n0 - 1;
-  const uint64_t c2 = r % m2, r1 = r / m2;
-  const uint64_t c1 = r1 % m1, r0 = r1 / m1;
-  const uint64_t c0 = r0 % m0, it = r0 / m0;
+  uint64_t c0, c1, c2, it;
+  if (v.cell_bits) {
+    const uint32_t k = v.cell_bits;
+    const uint64_t m = (1ull << k) - 1;
+    it = r / (uint64_t)v.cell_slice;
+    const uint64_t rr = r - it * (uint64_t)v.cell_slice;
+    const uint64_t brick = rr >> (3 * k), inner = rr & ((1ull << (3 * k)) - 1);
+    const uint64_t b2 = brick % (uint64_t)v.cb2, b1 = brick / (uint64_t)v.cb2 % (uint64_t)v.cb1;
+    const uint64_t b0 = brick / (uint64_t)v.cb2 / (uint64_t)v.cb1;
+    c0 = (b0 << k) | (inner >> (2 * k));
+    c1 = (b1 << k) | ((inner >> k) & m);
+    c2 = (b2 << k) | (inner & m);
+    if (c0 >= m0 || c1 >= m1 || c2 >= m2) return;
+  } else {
+    c2 = r % m2;
+    const uint64_t r1 = r / m2;
+    c1 = r1 % m1;
+    const uint64_t r0 = r1 / m1;
+    c0 = r0 % m0;
+    it = r0 / m0;
+  }
   const uint64_t n2 = (uint64_t)v.n2, n12 = (uint64_t)v.n1 * v.n2;
   const float* s = v.data + 3 * it * (uint64_t)v.slice;
   const uint64_t b = c0 * n12 + c1 * n2 + c2;
@@ -363,9 +383,21 @@ static void set_division(dpah_context* ctx, FieldView& v) {
 // (128 B per cell and time slice) fits a tenth of the memory budget.
 static void build_cells(dpah_context* ctx, dpah_field* f) {
   FieldView& v = f->view;
-  const uint64_t records = (uint64_t)(v.n0 - 1) * (v.n1 - 1) * (v.n2 - 1) * (uint64_t)v.nt;
+  const uint32_t k = (uint32_t)ctx->tune.cell_brick;
+  uint64_t per_slice;
+  if (k) {   // the cell grid padded to whole bricks of 2^k cells per axis
+    auto bricks = [&](int32_t n) { return ((uint64_t)n - 1 + (1ull << k) - 1) >> k; };
+    v.cb1 = (int32_t)bricks(v.n1);
+    v.cb2 = (int32_t)bricks(v.n2);
+    per_slice = (bricks(v.n0) * bricks(v.n1) * bricks(v.n2)) << (3 * k);
+  } else {
+    per_slice = (uint64_t)(v.n0 - 1) * (v.n1 - 1) * (v.n2 - 1);
+  }
+  const uint64_t records = per_slice * (uint64_t)v.nt;
   const uint64_t bytes = records * 128;
   if (ctx->tune.cells == 0 || bytes > ctx->memory_budget / 10) return;
+  v.cell_bits = k;
+  v.cell_slice = (int64_t)per_slice;
   f->d_cells = static_cast<float4*>(ctx->alloc(bytes));
   f->cell_bytes = bytes;
   const uint64_t grid = (records + 255) / 256;
@@ -375,7 +407,6 @@ static void build_cells(dpah_context* ctx, dpah_field* f) {
   DPAH_HIP(hipGetLastError());
   ctx->sync();
   v.cells = f->d_cells;
-  v.cell_slice = (int64_t)(records / (uint64_t)v.nt);
 }
 
 static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shape[3], uint64_t nt,
@@ -415,6 +446,8 @@ static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shap
   v.h0 = spacing[0]; v.h1 = spacing[1]; v.h2 = spacing[2]; v.th = nt > 1 ? t_sp : 1.0f;
   v.cells = nullptr;
   v.cell_slice = 0;
+  v.cell_bits = 0;
+  v.cb1 = v.cb2 = 0;
   try {
     set_division(ctx, v);
     build_cells(ctx, f);
@@ -763,6 +796,7 @@ DPAH_EXPORT int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out) {
     out->refill = ctx->tune.refill;
     out->cells = ctx->tune.cells;
     out->fastdiv = ctx->tune.fastdiv;
+    out->cell_brick = ctx->tune.cell_brick;
   });
 }
 
@@ -777,6 +811,8 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(t->refill >= 0 && t->refill <= 64, "refill must be 0 (auto) or 1..64");
     DPAH_REQUIRE(t->cells >= 0 && t->cells <= 2, "cells must be 0, 1 or 2");
     DPAH_REQUIRE(t->fastdiv == 0 || t->fastdiv == 1, "fastdiv must be 0 or 1");
+    DPAH_REQUIRE(t->cell_brick == 0 || (t->cell_brick >= 2 && t->cell_brick <= 5),
+                 "cell_brick must be 0 or 2..5");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     ctx->tune.block = t->block;
     ctx->tune.blocks_per_cu = t->blocks_per_cu;
@@ -785,6 +821,7 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     ctx->tune.refill = t->refill;
     ctx->tune.cells = t->cells;
     ctx->tune.fastdiv = t->fastdiv;
+    ctx->tune.cell_brick = t->cell_brick;
     ctx->tuned.clear();   // explicit settings replace autotuned ones
   });
 }

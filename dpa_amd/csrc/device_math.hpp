@@ -27,6 +27,14 @@ struct FieldView {
   // of four rows.  The values are copies of the nodes; null when absent.
   const float4* cells;
   int64_t cell_slice;  // records per time slice
+  // Record order of the cell layout: 0 = C order over (c0, c1, c2); k > 0 =
+  // bricks of 2^k cells per axis stored one after another (bricks in C order,
+  // cells in C order inside a brick), so that the cells a particle crosses
+  // next, and those of its neighbours, share DRAM pages and address
+  // translations.  cb1, cb2: bricks along axes 1 and 2 (the cell grid is
+  // padded to whole bricks; padding records are never read).
+  uint32_t cell_bits;
+  int32_t cb1, cb2;
   // Reciprocals of the spacings and, per axis (bit 0..2, bit 3 = time), whether
   // div_by() with that reciprocal was verified at field creation to equal the
   // correctly rounded quotient for every float in the axis's range.
@@ -168,8 +176,14 @@ __device__ __forceinline__ V3 node(const FieldView& f, const float* base, int64_
 }
 
 // Record index of cell (c0, c1, c2) of time slice it in the cell layout.
-__device__ __forceinline__ int64_t cell_record(const FieldView& f, int32_t it, int32_t c0,
+__host__ __device__ inline int64_t cell_record(const FieldView& f, int32_t it, int32_t c0,
                                                int32_t c1, int32_t c2) {
+  if (f.cell_bits) {   // field-uniform
+    const uint32_t k = f.cell_bits, m = (1u << k) - 1u;
+    const int64_t brick = ((int64_t)(c0 >> k) * f.cb1 + (c1 >> k)) * f.cb2 + (c2 >> k);
+    const int64_t inner = ((int64_t)(c0 & m) << (2 * k)) | ((c1 & m) << k) | (c2 & m);
+    return (int64_t)it * f.cell_slice + (brick << (3 * k)) + inner;
+  }
   return (((int64_t)it * (f.n0 - 1) + c0) * (f.n1 - 1) + c1) * (int64_t)(f.n2 - 1) + c2;
 }
 

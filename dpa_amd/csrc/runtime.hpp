@@ -85,6 +85,10 @@ struct Tuning {
   // neighbouring lanes share node rows, read the nodes; 2: cells for every launch
   int cells = 1;
   int fastdiv = 1;          // reciprocal division on axes where it was verified exact
+  // cell layout record order of fields created afterwards: 0 = C order, k =
+  // bricks of 2^k cells per axis stored contiguously (8^3-cell bricks: the
+  // 256^3 vortex workload 5.06 -> 5.01 ms, about 1%, the same for 4^3 and 16^3)
+  int cell_brick = 3;
 };
 
 // Device counters of a context: [1] particle-steps, [2] FTLE misses,

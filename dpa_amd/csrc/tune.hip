@@ -22,7 +22,8 @@ struct Candidate {
 };
 
 // Cache file: one line per launch key,
-// "<arch> <cus> <key> <block> <blocks_per_cu> <order> <parts> <refill> <cells> <fastdiv> <ms>".
+// "<arch> <cus> <key> <block> <blocks_per_cu> <order> <parts> <refill> <cells> <fastdiv>
+// <cell_brick> <ms>" (cell_brick is a field-creation setting, kept for the record).
 void save_cache(const dpah_context* ctx, const std::string& key, const Tuning& t, double ms) {
   if (ctx->autotune_path.empty()) return;
   std::vector<std::string> keep;
@@ -41,7 +42,7 @@ void save_cache(const dpah_context* ctx, const std::string& key, const Tuning& t
   for (const auto& l : keep) out << l << "\n";
   out << ctx->arch << " " << ctx->cu_count << " " << key << " " << t.block << " "
       << t.blocks_per_cu << " " << t.order << " " << t.parts << " " << t.refill << " " << t.cells
-      << " " << t.fastdiv << " " << ms << "\n";
+      << " " << t.fastdiv << " " << t.cell_brick << " " << ms << "\n";
   if (!out) throw Error(DPAH_ERR_INVALID, "cannot write the autotune cache " + ctx->autotune_path);
 }
 
@@ -61,13 +62,14 @@ void load_autotune_cache(dpah_context* ctx) {
     Tuning t;
     double ms = 0;
     if (!(ss >> arch >> cus >> key >> t.block >> t.blocks_per_cu >> t.order >> t.parts >>
-          t.refill >> t.cells >> t.fastdiv >> ms))
+          t.refill >> t.cells >> t.fastdiv >> t.cell_brick >> ms))
       continue;
     if (arch != ctx->arch || cus != ctx->cu_count) continue;
     const bool ok = t.block >= 64 && t.block <= 256 && t.block % 64 == 0 &&
                     t.blocks_per_cu >= 1 && t.blocks_per_cu <= 32 && (t.order == 0 || t.order == 1) &&
                     t.parts >= 1 && t.parts <= kMaxParts && t.refill >= 0 && t.refill <= 64 &&
-                    t.cells >= 0 && t.cells <= 2 && (t.fastdiv == 0 || t.fastdiv == 1);
+                    t.cells >= 0 && t.cells <= 2 && (t.fastdiv == 0 || t.fastdiv == 1) &&
+                    (t.cell_brick == 0 || (t.cell_brick >= 2 && t.cell_brick <= 5));
     if (ok) ctx->tuned[key] = t;
   }
 }
@@ -151,6 +153,7 @@ DPAH_EXPORT int dpah_autotune_endpoints(dpah_context* ctx, const dpah_field* fie
       best_out->refill = best.refill;
       best_out->cells = best.cells;
       best_out->fastdiv = best.fastdiv;
+      best_out->cell_brick = best.cell_brick;
     }
     if (best_ms) *best_ms = best_t;
   });

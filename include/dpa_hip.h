@@ -338,6 +338,9 @@ typedef struct dpah_tuning {
                           one correction on axes where field creation verified
                           it bit for bit against the correctly rounded quotient
                           over the axis's range; 0: always divide              */
+  int cell_brick;      /* record order of the cell layout of fields created
+                          afterwards: 0 C order over the cells; 2..5 bricks of
+                          2^k cells per axis stored one after another         */
 } dpah_tuning;
 int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
 /* Sets the context's tuning and drops autotuned entries. */

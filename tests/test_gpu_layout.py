@@ -97,3 +97,37 @@ def test_ftle_aux_same_on_both_layouts(ctx, unsteady):
                 fld.release()
     assert out[0][1] == out[1][1] and out[0][1] > 0
     assert np.array_equal(bits(out[0][0]), bits(out[1][0]))
+
+
+@pytest.mark.parametrize("cell_brick", [2, 3, 5])
+@pytest.mark.parametrize("unsteady", [False, True])
+def test_brick_ordered_cells_match_oracle(ctx, cell_brick, unsteady):
+    # cell records stored brick by brick (2^k cells per axis, the cell grid
+    # padded to whole bricks: 39 and 23 cells per axis here); every launch
+    # reads the cells
+    say(f"case brick-ordered cells k={cell_brick} unsteady={unsteady}")
+    if unsteady:
+        data = W.unsteady_vortex_field(24, time_steps=5, count=6)
+        ofld = po.Field(data, OFFSET, SPACING, 0.5, 0.5)
+    else:
+        data = W.vortex_field(40, count=12)
+        ofld = po.Field(data, OFFSET, SPACING)
+    n = data.shape[-2]
+    lo = np.array(OFFSET, f32)
+    hi = lo + np.array(SPACING, f32) * (n - 1)
+    seeds = W.uniform_seeds(3000, lo, hi, seed_offset=950 + cell_brick)
+    rem = np.full(len(seeds), 120, np.uint64)
+    dt, t0 = (f32(0.01) if unsteady else f32(0.2)), f32(0.75)
+    with tuned(ctx, cells=2, cell_brick=cell_brick):
+        fld = make_field(ctx, data, unsteady)
+        try:
+            res = ctx.advect(fld, seeds, remaining=rem, particles_per_round=1000, step_size=dt,
+                             start_time=t0)
+            try:
+                ora = po.run_rounds(ofld, seeds, rem, 1000, dt, t0)
+                compare_advect(res, ora, len(seeds))
+                assert ora["total_steps"] > 50000
+            finally:
+                res.free()
+        finally:
+            fld.release()
