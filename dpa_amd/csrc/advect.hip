@@ -15,7 +15,9 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
   if (a.n == 0) return;
   if (persistent) {
     const int block = T.block;
-    uint64_t grid = (uint64_t)ctx->cu_count * T.blocks_per_cu;
+    const int bpc = T.blocks_per_cu ? T.blocks_per_cu
+                  : kRecord ? 8 : a.seed_kind == kSeedArray ? 4 : 3;
+    uint64_t grid = (uint64_t)ctx->cu_count * bpc;
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
     ctx->timed("advect_persistent", [&] {

@@ -805,7 +805,8 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(ctx && t, "null argument");
     DPAH_REQUIRE(t->block >= 64 && t->block <= 256 && t->block % 64 == 0,
                  "block must be 64, 128, 192 or 256");
-    DPAH_REQUIRE(t->blocks_per_cu >= 1 && t->blocks_per_cu <= 32, "blocks_per_cu must be 1..32");
+    DPAH_REQUIRE(t->blocks_per_cu >= 0 && t->blocks_per_cu <= 32,
+                 "blocks_per_cu must be 0 (by launch kind) or 1..32");
     DPAH_REQUIRE(t->order == 0 || t->order == 1, "order must be 0 or 1");
     DPAH_REQUIRE(t->parts >= 1 && t->parts <= dpah::kMaxParts, "parts must be 1..8");
     DPAH_REQUIRE(t->refill >= 0 && t->refill <= 64, "refill must be 0 (auto) or 1..64");

@@ -73,7 +73,13 @@ struct PendingTiming {
 // Launch-shape parameters picked by the autotuner (or defaults).
 struct Tuning {
   int block = 256;          // threads per block of the persistent advect kernel
-  int blocks_per_cu = 8;    // persistent grid = CUs * blocks_per_cu
+  // persistent grid = CUs * blocks_per_cu; 0: by launch kind (4 for endpoint
+  // launches of seed arrays, 3 for lattices, 8 for recorded launches).  Fewer
+  // particles in flight shrink what competes for each XCD's L2 and give each
+  // lane more particles, against less latency hiding: 256^3 vortex endpoints
+  // 5.04 ms at 8, 4.56 at 4, 4.95 at 3, 6.39 at 2; dpa-mode FTLE lattice
+  // 14.8 ms at 8, 11.9 at 3; 64 recorded jobs 30.6 ms at 8, 30.2 at 6, 33.0 at 4
+  int blocks_per_cu = 0;
   int order = 1;            // 1: spatially ordered queue split across the XCDs
   int parts = 8;            // queue parts (XCDs on the device)
   // idle lanes that make a wave draw new particles (1..64); 0: 16 for endpoint

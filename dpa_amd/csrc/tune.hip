@@ -66,7 +66,7 @@ void load_autotune_cache(dpah_context* ctx) {
       continue;
     if (arch != ctx->arch || cus != ctx->cu_count) continue;
     const bool ok = t.block >= 64 && t.block <= 256 && t.block % 64 == 0 &&
-                    t.blocks_per_cu >= 1 && t.blocks_per_cu <= 32 && (t.order == 0 || t.order == 1) &&
+                    t.blocks_per_cu >= 0 && t.blocks_per_cu <= 32 && (t.order == 0 || t.order == 1) &&
                     t.parts >= 1 && t.parts <= kMaxParts && t.refill >= 0 && t.refill <= 64 &&
                     t.cells >= 0 && t.cells <= 2 && (t.fastdiv == 0 || t.fastdiv == 1) &&
                     (t.cell_brick == 0 || (t.cell_brick >= 2 && t.cell_brick <= 5));
@@ -115,8 +115,8 @@ DPAH_EXPORT int dpah_autotune_endpoints(dpah_context* ctx, const dpah_field* fie
     };
     const Tuning base = ctx->tune;
     std::vector<Candidate> cands;
-    for (int refill : {4, 16, 32, 64})
-      for (int bpc : {6, 8})
+    for (int refill : {4, 8, 16, 64})
+      for (int bpc : {3, 4, 5, 6, 8})
         for (int order : {0, 1})
           for (int cells : {0, 1}) {
             if (cells && !field->view.cells) continue;

@@ -317,7 +317,9 @@ int dpah_kernel_timing_reset(dpah_context* ctx);
  * dpah_autotune_endpoints picks these by timing a sample of a workload. */
 typedef struct dpah_tuning {
   int block;           /* threads per workgroup: 64, 128, 192 or 256          */
-  int blocks_per_cu;   /* persistent grid = CUs x blocks_per_cu, 1..32         */
+  int blocks_per_cu;   /* persistent grid = CUs x blocks_per_cu, 1..32; 0: by
+                          launch kind (4 endpoint seed arrays, 3 lattices,
+                          8 recorded launches)                                 */
   int order;           /* 1: endpoint launches of seed arrays and lattices take
                           their queue in brick Morton order, split across the
                           XCDs (auxiliary FTLE lattices are already compact;
@@ -346,7 +348,7 @@ int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
 /* Sets the context's tuning and drops autotuned entries. */
 int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* tuning);
 
-/* Times candidate tunings (refill 4/16/32/64 x 6/8 workgroups per CU x queue
+/* Times candidate tunings (refill 4/8/16/64 x 3/4/5/6/8 workgroups per CU x queue
  * order on/off x cell/node layout) on a sample of an endpoint workload --
  * seeds: count x 3 device floats, as dpah_advect_endpoints_device takes them --
  * and keeps the fastest for launches with the same key (field shape,

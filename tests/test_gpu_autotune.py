@@ -51,7 +51,7 @@ def test_autotune_keeps_results_and_persists(tmp_path):
         finally:
             ctx.free(d_sample)
         say(f"  best {best} {ms:.3f} ms")
-        assert ms > 0 and 1 <= best["refill"] <= 64 and best["blocks_per_cu"] in (6, 8)
+        assert ms > 0 and 1 <= best["refill"] <= 64 and best["blocks_per_cu"] in (3, 4, 5, 6, 8)
         pos, steps = endpoints(ctx, fld, seeds, iters, dt)
         assert np.array_equal(bits(pos), bits(ora["position"])) and steps == ora["total_steps"]
         fld.release()

@@ -100,7 +100,8 @@ def test_ordered_recorded_rounds_match_oracle(ctx, vortex):
 
 
 def test_set_tuning_rejects_bad_shapes(ctx):
-    for bad in (dict(block=96), dict(block=512), dict(blocks_per_cu=0), dict(order=2),
+    for bad in (dict(block=96), dict(block=512), dict(blocks_per_cu=-1), dict(blocks_per_cu=33),
+                dict(order=2),
                 dict(parts=9), dict(parts=0), dict(cell_brick=1), dict(cell_brick=6)):
         with pytest.raises(dpa_amd.DpahError):
             ctx.set_tuning(**bad)
