@@ -545,8 +545,12 @@ def main() -> int:
                     help="cell layout record order of the fields (0: C order; k: 2^k-cell bricks)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="skip the hipGraph replay leg")
+    ap.add_argument("--autotune", dest="autotune", action="store_true", default=False,
+                    help="run the launch-shape autotuner on the workload before the timed "
+                         "passes (off by default, so that every dispatch of the advect kernel "
+                         "in a profile of the default command is a measured pass)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="skip the launch-shape autotuner before the timed passes")
+                    help="the default: time the library's launch shapes as they are")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path here (with the field "

@@ -207,11 +207,11 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   a.total_steps = ctx->d_counters + 1;
   a.order = nullptr;
   set_parts(a, 1);
-  // auto: 16 for endpoint launches of seed arrays (keeps lanes busy); 64 for
+  // auto: 8 for endpoint launches of seed arrays (keeps lanes busy); 64 for
   // lattices (keeps neighbouring particles together) and for recorded launches
   // (a wave's 64 lanes then write one aligned tile of slots per step)
   a.refill = (uint32_t)(T.refill ? T.refill
-                                 : (L.seed_kind == kSeedArray && !L.slots ? 16 : 64));
+                                 : (L.seed_kind == kSeedArray && !L.slots ? 8 : 64));
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;

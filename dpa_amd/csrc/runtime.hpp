@@ -82,7 +82,7 @@ struct Tuning {
   int blocks_per_cu = 0;
   int order = 1;            // 1: spatially ordered queue split across the XCDs
   int parts = 8;            // queue parts (XCDs on the device)
-  // idle lanes that make a wave draw new particles (1..64); 0: 16 for endpoint
+  // idle lanes that make a wave draw new particles (1..64); 0: 8 for endpoint
   // launches of seed arrays (keeps lanes busy), 64 for lattices (keeps a wave's
   // neighbouring particles together) and recorded launches (aligned slot tiles)
   int refill = 0;

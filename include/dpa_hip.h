@@ -327,7 +327,7 @@ typedef struct dpah_tuning {
   int parts;           /* queue parts, 1..8 (the XCD count by default)         */
   int refill;          /* idle lanes (1..64) that make a wave draw new particles:
                           1 keeps every lane busy, 64 keeps a wave's particles
-                          together until all have retired; 0: 16 for endpoint
+                          together until all have retired; 0: 8 for endpoint
                           launches of seed arrays, 64 for lattices and for
                           recorded launches                                    */
   int cells;           /* 0: node layout only.  1: fields created afterwards also
