@@ -322,7 +322,8 @@ def make_context(args):
     over = {k: v for k, v in (("order", args.order), ("blocks_per_cu", args.blocks_per_cu),
                               ("block", args.block), ("refill", args.refill),
                               ("cells", args.cells), ("fastdiv", args.fastdiv),
-                              ("cell_brick", args.cell_brick)) if v is not None}
+                              ("cell_brick", args.cell_brick), ("order_brick", args.order_brick))
+            if v is not None}
     if over:
         ctx.set_tuning(**over)
         args.autotune = False   # explicit launch shapes are measured as given
@@ -541,6 +542,8 @@ def main() -> int:
     ap.add_argument("--refill", type=int, default=None)
     ap.add_argument("--cells", type=int, choices=[0, 1, 2], default=None)
     ap.add_argument("--fastdiv", type=int, choices=[0, 1], default=None)
+    ap.add_argument("--order-brick", type=int, choices=[0, 1, 2, 3, 4, 5], default=None,
+                    help="queue ordering bricks of at least 2^k cells per axis (0: 4 cells)")
     ap.add_argument("--cell-brick", type=int, choices=[0, 2, 3, 4, 5], default=None,
                     help="cell layout record order of the fields (0: C order; k: 2^k-cell bricks)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",

@@ -66,7 +66,7 @@ class _Request(C.Structure):
 class _Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("order", C.c_int),
                 ("parts", C.c_int), ("refill", C.c_int), ("cells", C.c_int),
-                ("fastdiv", C.c_int), ("cell_brick", C.c_int)]
+                ("fastdiv", C.c_int), ("cell_brick", C.c_int), ("order_brick", C.c_int)]
 
 
 class _RequestInfo(C.Structure):
@@ -387,20 +387,21 @@ class Context:
                                                 float(start_time), int(integrator), C.byref(t),
                                                 C.byref(ms)))
         return (dict(block=t.block, blocks_per_cu=t.blocks_per_cu, order=t.order, parts=t.parts,
-                     refill=t.refill, cells=t.cells, fastdiv=t.fastdiv, cell_brick=t.cell_brick),
-                ms.value)
+                     refill=t.refill, cells=t.cells, fastdiv=t.fastdiv, cell_brick=t.cell_brick,
+                     order_brick=t.order_brick), ms.value)
 
     def tuning(self) -> dict:
         t = _Tuning()
         _check(self.lib.dpah_get_tuning(self.h, C.byref(t)))
         return dict(block=t.block, blocks_per_cu=t.blocks_per_cu, order=t.order, parts=t.parts,
-                    refill=t.refill, cells=t.cells, fastdiv=t.fastdiv, cell_brick=t.cell_brick)
+                    refill=t.refill, cells=t.cells, fastdiv=t.fastdiv, cell_brick=t.cell_brick,
+                    order_brick=t.order_brick)
 
     def set_tuning(self, **kw) -> None:
         cur = self.tuning()
         cur.update(kw)
         t = _Tuning(cur["block"], cur["blocks_per_cu"], cur["order"], cur["parts"], cur["refill"],
-                    cur["cells"], cur["fastdiv"], cur["cell_brick"])
+                    cur["cells"], cur["fastdiv"], cur["cell_brick"], cur["order_brick"])
         _check(self.lib.dpah_set_tuning(self.h, C.byref(t)))
 
 

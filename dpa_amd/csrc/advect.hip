@@ -84,12 +84,13 @@ void set_parts(AdvectArgs& a, uint32_t parts) {
 }  // namespace
 
 // Brick size and key count of a field's queue ordering: bricks of 2^shift
-// cells per axis, at most 2^kOrderMaxBits per axis; one more key for seeds
-// outside the grid; tiles of the histogram scan.
-void order_shape(const FieldView& f, uint32_t& shift, uint32_t& bits, uint32_t& keys,
-                 uint32_t& tiles) {
+// cells per axis (at least 2^min_shift; 0 asks for 4-cell bricks), at most
+// 2^kOrderMaxBits bricks per axis; one more key for seeds outside the grid;
+// tiles of the histogram scan.
+void order_shape(const FieldView& f, uint32_t min_shift, uint32_t& shift, uint32_t& bits,
+                 uint32_t& keys, uint32_t& tiles) {
   const int64_t D = std::max({(int64_t)f.n0, (int64_t)f.n1, (int64_t)f.n2});
-  shift = 2;
+  shift = min_shift ? min_shift : 2;
   while (((D + (1ll << shift) - 1) >> shift) > (1ll << kOrderMaxBits)) ++shift;
   bits = 1;
   while ((1ll << (bits + shift)) < D) ++bits;
@@ -131,7 +132,7 @@ void order_queue(dpah_context* ctx, const Tuning& T, AdvectArgs& a, QueueOrder& 
                  const OrderScratch* keep) {
   OrderArgs o{};
   uint32_t tiles = 0;
-  order_shape(a.f, o.shift, o.bits, o.keys, tiles);
+  order_shape(a.f, (uint32_t)T.order_brick, o.shift, o.bits, o.keys, tiles);
   if (keep) {   // caller-owned buffers (a captured pass replays with the same ones)
     DPAH_REQUIRE(keep->n_cap >= a.n && keep->count_cap >= (uint64_t)o.keys + tiles,
                  "order scratch too small");

@@ -797,6 +797,7 @@ DPAH_EXPORT int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out) {
     out->cells = ctx->tune.cells;
     out->fastdiv = ctx->tune.fastdiv;
     out->cell_brick = ctx->tune.cell_brick;
+    out->order_brick = ctx->tune.order_brick;
   });
 }
 
@@ -814,6 +815,7 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(t->fastdiv == 0 || t->fastdiv == 1, "fastdiv must be 0 or 1");
     DPAH_REQUIRE(t->cell_brick == 0 || (t->cell_brick >= 2 && t->cell_brick <= 5),
                  "cell_brick must be 0 or 2..5");
+    DPAH_REQUIRE(t->order_brick >= 0 && t->order_brick <= 5, "order_brick must be 0..5");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     ctx->tune.block = t->block;
     ctx->tune.blocks_per_cu = t->blocks_per_cu;
@@ -823,6 +825,7 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     ctx->tune.cells = t->cells;
     ctx->tune.fastdiv = t->fastdiv;
     ctx->tune.cell_brick = t->cell_brick;
+    ctx->tune.order_brick = t->order_brick;
     ctx->tuned.clear();   // explicit settings replace autotuned ones
   });
 }

@@ -67,7 +67,7 @@ DPAH_EXPORT int dpah_pass_create_endpoints(dpah_context* ctx, const dpah_field* 
       L.order_scratch = nullptr;
       if (ordered) {
         uint32_t shift, bits, keys, tiles;
-        order_shape(field->view, shift, bits, keys, tiles);
+        order_shape(field->view, (uint32_t)T.order_brick, shift, bits, keys, tiles);
         p->key = DeviceBuffer(ctx, count * sizeof(uint32_t));
         p->order = DeviceBuffer(ctx, count * sizeof(uint32_t));
         p->count = DeviceBuffer(ctx, ((uint64_t)keys + tiles) * sizeof(uint32_t));

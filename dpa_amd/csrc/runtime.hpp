@@ -95,6 +95,9 @@ struct Tuning {
   // bricks of 2^k cells per axis stored contiguously (8^3-cell bricks: the
   // 256^3 vortex workload 5.06 -> 5.01 ms, about 1%, the same for 4^3 and 16^3)
   int cell_brick = 3;
+  // queue ordering of seed arrays: bricks of at least 2^order_brick cells per
+  // axis (0: 4-cell bricks; larger when the grid needs more than 64 per axis)
+  int order_brick = 0;
 };
 
 // Device counters of a context: [1] particle-steps, [2] FTLE misses,
@@ -268,8 +271,8 @@ struct AdvectLaunch {
 };
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
 // Brick shape of the queue ordering of a field (order_kernels.hpp).
-void order_shape(const FieldView& f, uint32_t& shift, uint32_t& bits, uint32_t& keys,
-                 uint32_t& tiles);
+void order_shape(const FieldView& f, uint32_t min_shift, uint32_t& shift, uint32_t& bits,
+                 uint32_t& keys, uint32_t& tiles);
 // Whether a launch sorts its queue (and so needs order scratch).
 bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch& L);
 // Whether a lattice launch walks its queue in closed-form bricks (no scratch).

@@ -18,12 +18,13 @@ using namespace dpah;
 namespace {
 
 struct Candidate {
-  int refill, blocks_per_cu, order, cells;
+  int block, refill, blocks_per_cu, order, cells, order_brick;
 };
 
 // Cache file: one line per launch key,
 // "<arch> <cus> <key> <block> <blocks_per_cu> <order> <parts> <refill> <cells> <fastdiv>
-// <cell_brick> <ms>" (cell_brick is a field-creation setting, kept for the record).
+// <cell_brick> <order_brick> <ms>" (cell_brick is a field-creation setting, kept for the
+// record).
 void save_cache(const dpah_context* ctx, const std::string& key, const Tuning& t, double ms) {
   if (ctx->autotune_path.empty()) return;
   std::vector<std::string> keep;
@@ -42,7 +43,7 @@ void save_cache(const dpah_context* ctx, const std::string& key, const Tuning& t
   for (const auto& l : keep) out << l << "\n";
   out << ctx->arch << " " << ctx->cu_count << " " << key << " " << t.block << " "
       << t.blocks_per_cu << " " << t.order << " " << t.parts << " " << t.refill << " " << t.cells
-      << " " << t.fastdiv << " " << t.cell_brick << " " << ms << "\n";
+      << " " << t.fastdiv << " " << t.cell_brick << " " << t.order_brick << " " << ms << "\n";
   if (!out) throw Error(DPAH_ERR_INVALID, "cannot write the autotune cache " + ctx->autotune_path);
 }
 
@@ -62,14 +63,15 @@ void load_autotune_cache(dpah_context* ctx) {
     Tuning t;
     double ms = 0;
     if (!(ss >> arch >> cus >> key >> t.block >> t.blocks_per_cu >> t.order >> t.parts >>
-          t.refill >> t.cells >> t.fastdiv >> t.cell_brick >> ms))
+          t.refill >> t.cells >> t.fastdiv >> t.cell_brick >> t.order_brick >> ms))
       continue;
     if (arch != ctx->arch || cus != ctx->cu_count) continue;
     const bool ok = t.block >= 64 && t.block <= 256 && t.block % 64 == 0 &&
                     t.blocks_per_cu >= 0 && t.blocks_per_cu <= 32 && (t.order == 0 || t.order == 1) &&
                     t.parts >= 1 && t.parts <= kMaxParts && t.refill >= 0 && t.refill <= 64 &&
                     t.cells >= 0 && t.cells <= 2 && (t.fastdiv == 0 || t.fastdiv == 1) &&
-                    (t.cell_brick == 0 || (t.cell_brick >= 2 && t.cell_brick <= 5));
+                    (t.cell_brick == 0 || (t.cell_brick >= 2 && t.cell_brick <= 5)) &&
+                    t.order_brick >= 0 && t.order_brick <= 5;
     if (ok) ctx->tuned[key] = t;
   }
 }
@@ -115,19 +117,26 @@ DPAH_EXPORT int dpah_autotune_endpoints(dpah_context* ctx, const dpah_field* fie
     };
     const Tuning base = ctx->tune;
     std::vector<Candidate> cands;
-    for (int refill : {4, 8, 16, 64})
-      for (int bpc : {3, 4, 5, 6, 8})
-        for (int order : {0, 1})
-          for (int cells : {0, 1}) {
-            if (cells && !field->view.cells) continue;
-            cands.push_back({refill, bpc, order, cells});
-          }
+    // workgroup size x workgroups per CU as waves per CU (12..32), refill,
+    // queue order with its brick size, field layout
+    for (int block : {128, 256})
+      for (int waves : {12, 16, 20, 24, 32})
+        for (int refill : {4, 8, 16, 64})
+          for (int order : {0, 1})
+            for (int obrick : {2, 3})
+              for (int cells : {0, 1}) {
+                if (cells && !field->view.cells) continue;
+                if (!order && obrick != 2) continue;   // the brick only shapes the sort
+                cands.push_back({block, refill, waves * 64 / block, order, cells, obrick});
+              }
     Tuning best = base;
     double best_t = 1e300;
     try {
       time_once(base);   // warm-up: code objects, caches, allocator blocks
       for (const Candidate& c : cands) {
         Tuning t = base;
+        t.block = c.block;
+        t.order_brick = c.order_brick;
         t.refill = c.refill;
         t.blocks_per_cu = c.blocks_per_cu;
         t.order = c.order;
@@ -154,6 +163,7 @@ DPAH_EXPORT int dpah_autotune_endpoints(dpah_context* ctx, const dpah_field* fie
       best_out->cells = best.cells;
       best_out->fastdiv = best.fastdiv;
       best_out->cell_brick = best.cell_brick;
+      best_out->order_brick = best.order_brick;
     }
     if (best_ms) *best_ms = best_t;
   });

@@ -343,13 +343,16 @@ typedef struct dpah_tuning {
   int cell_brick;      /* record order of the cell layout of fields created
                           afterwards: 0 C order over the cells; 2..5 bricks of
                           2^k cells per axis stored one after another         */
+  int order_brick;     /* queue ordering of seed arrays: bricks of at least
+                          2^order_brick cells per axis, 1..5; 0: 4 cells      */
 } dpah_tuning;
 int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
 /* Sets the context's tuning and drops autotuned entries. */
 int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* tuning);
 
-/* Times candidate tunings (refill 4/8/16/64 x 3/4/5/6/8 workgroups per CU x queue
- * order on/off x cell/node layout) on a sample of an endpoint workload --
+/* Times candidate tunings (workgroups of 128 or 256 threads x 12/16/20/24/32
+ * waves per CU x refill 4/8/16/64 x queue order off or on with 4- or 8-cell
+ * sort bricks x cell/node layout) on a sample of an endpoint workload --
  * seeds: count x 3 device floats, as dpah_advect_endpoints_device takes them --
  * and keeps the fastest for launches with the same key (field shape,
  * integrator, seed array, endpoints).  With dpah_options.autotune_cache_path

@@ -51,7 +51,9 @@ def test_autotune_keeps_results_and_persists(tmp_path):
         finally:
             ctx.free(d_sample)
         say(f"  best {best} {ms:.3f} ms")
-        assert ms > 0 and 1 <= best["refill"] <= 64 and best["blocks_per_cu"] in (3, 4, 5, 6, 8)
+        assert ms > 0 and 1 <= best["refill"] <= 64 and best["block"] in (128, 256)
+        assert best["block"] * best["blocks_per_cu"] // 64 in (12, 16, 20, 24, 32)
+        assert best["order_brick"] in (2, 3)
         pos, steps = endpoints(ctx, fld, seeds, iters, dt)
         assert np.array_equal(bits(pos), bits(ora["position"])) and steps == ora["total_steps"]
         fld.release()

@@ -58,7 +58,9 @@ def endpoints(ctx, fld, seeds, iters, dt):
 
 @pytest.mark.parametrize("tuning", [dict(order=1, parts=8), dict(order=0, parts=8),
                                     dict(order=1, parts=1), dict(order=1, parts=8, blocks_per_cu=2),
-                                    dict(order=1, parts=3, block=128, blocks_per_cu=5)])
+                                    dict(order=1, parts=3, block=128, blocks_per_cu=5),
+                                    dict(order=1, parts=8, order_brick=3),
+                                    dict(order=1, parts=8, order_brick=5, block=64)])
 def test_ordered_endpoints_match_oracle(ctx, vortex, tuning):
     field, fld = vortex
     say(f"case ordered endpoints {tuning}")
@@ -102,7 +104,8 @@ def test_ordered_recorded_rounds_match_oracle(ctx, vortex):
 def test_set_tuning_rejects_bad_shapes(ctx):
     for bad in (dict(block=96), dict(block=512), dict(blocks_per_cu=-1), dict(blocks_per_cu=33),
                 dict(order=2),
-                dict(parts=9), dict(parts=0), dict(cell_brick=1), dict(cell_brick=6)):
+                dict(parts=9), dict(parts=0), dict(cell_brick=1), dict(cell_brick=6),
+                dict(order_brick=-1), dict(order_brick=6)):
         with pytest.raises(dpa_amd.DpahError):
             ctx.set_tuning(**bad)
 
