@@ -461,6 +461,25 @@ def run_batch(args):
     na, ta = ctx.kernel_timing("advect_persistent")
     nc, tc = ctx.kernel_timing("compact_write")
     log(f"batched: {args.steps * steps / elapsed:.3e} steps/s")
+    graph = None
+    if args.graph:
+        # the same batch captured once (dpah_batch_pass_*): replay + results per pass
+        with ctx.batch_pass(jobs) as bp:
+            def replayed():
+                bp.launch()
+                res = bp.results()
+                s = sum(r.total_steps for r in res)
+                for r in res:
+                    r.free()
+                return s
+            gsteps = replayed()
+            assert gsteps == steps, (gsteps, steps)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                replayed()
+            gel = time.perf_counter() - t0
+        graph = {"value": args.steps * steps / gel, "ms_per_pass": 1e3 * gel / args.steps}
+        log(f"batched, graph replays: {graph['value']:.3e} steps/s")
     # per-request form: each job alone, one thread per particle
     t0 = time.perf_counter()
     with dpa_amd.Server(ctx, mode=dpa_amd.SERVE_PER_REQUEST) as server:
@@ -481,6 +500,8 @@ def run_batch(args):
                      "particle_steps_per_pass": steps}
     out["kernels"] = {"advect_persistent_ms": ta / max(na, 1),
                       "compact_write_ms_total_per_pass": tc / max(args.steps, 1)}
+    if graph:
+        out["graph_replay"] = graph
     out["per_request_launch"] = {"value": steps / per_req, "seconds": per_req}
     out["vs_per_request"] = value / (steps / per_req)
     out["roofline"] = roofline(steps, ta / max(na, 1), bytes_per_step=108)

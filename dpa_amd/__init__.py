@@ -153,6 +153,10 @@ def load_library() -> C.CDLL:
         "dpah_pass_launch": (C.c_int, [vp]),
         "dpah_pass_total_steps": (C.c_int, [vp, _u64p]),
         "dpah_pass_destroy": (C.c_int, [vp]),
+        "dpah_batch_pass_create": (C.c_int, [vp, C.POINTER(_Job), C.c_uint64, C.POINTER(vp)]),
+        "dpah_batch_pass_launch": (C.c_int, [vp]),
+        "dpah_batch_pass_results": (C.c_int, [vp, C.POINTER(vp)]),
+        "dpah_batch_pass_destroy": (C.c_int, [vp]),
         "dpah_autotune_endpoints": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, C.c_float,
                                               C.c_float, C.c_int, C.POINTER(_Tuning),
                                               C.POINTER(C.c_double)]),
@@ -272,8 +276,9 @@ class Context:
                                     int(uniform_remaining), n, C.byref(p), C.byref(h)))
         return Result(self, h, n)
 
-    def advect_batch(self, jobs: list[dict]) -> list["Result"]:
-        """jobs: dicts with field, seeds and the keyword arguments of advect()."""
+    @staticmethod
+    def _job_array(jobs: list[dict]):
+        """The dpah_job array of `jobs` and the numpy arrays it points into."""
         keep = []
         arr = (_Job * max(len(jobs), 1))()
         for i, j in enumerate(jobs):
@@ -293,9 +298,21 @@ class Context:
                                           int(j.get("integrator", INTEGRATOR_DPA_RK4)),
                                           1 if j.get("record", True) else 0,
                                           1 if j.get("use_64_bit", False) else 0)
+        return arr, keep
+
+    def advect_batch(self, jobs: list[dict]) -> list["Result"]:
+        """jobs: dicts with field, seeds and the keyword arguments of advect()."""
+        arr, _keep = self._job_array(jobs)
         hs = (C.c_void_p * max(len(jobs), 1))()
         _check(self.lib.dpah_advect_batch(self.h, arr, len(jobs), hs))
         return [Result(self, C.c_void_p(hs[i]), arr[i].count) for i in range(len(jobs))]
+
+    def batch_pass(self, jobs: list[dict]) -> "BatchPass":
+        """advect_batch's jobs captured once into a hipGraph (dpah_batch_pass_*)."""
+        arr, _keep = self._job_array(jobs)
+        h = C.c_void_p()
+        _check(self.lib.dpah_batch_pass_create(self.h, arr, len(jobs), C.byref(h)))
+        return BatchPass(self, h, [arr[i].count for i in range(len(jobs))])
 
     def advect_endpoints_device(self, field: "Field", d_seeds: int, count: int, d_pos_out: int,
                                 d_remaining: int | None = None, uniform_remaining: int = 0,
@@ -422,6 +439,33 @@ class Pass:
     def destroy(self) -> None:
         if self.h:
             _check(self.ctx.lib.dpah_pass_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+class BatchPass:
+    """A captured batch: launch() replays it; results() returns one Result per job."""
+
+    def __init__(self, ctx: "Context", h, counts: list[int]):
+        self.ctx, self.h, self.counts = ctx, h, counts
+
+    def launch(self) -> None:
+        _check(self.ctx.lib.dpah_batch_pass_launch(self.h))
+
+    def results(self) -> list["Result"]:
+        k = len(self.counts)
+        hs = (C.c_void_p * max(k, 1))()
+        _check(self.ctx.lib.dpah_batch_pass_results(self.h, hs))
+        return [Result(self.ctx, C.c_void_p(hs[i]), self.counts[i]) for i in range(k)]
+
+    def destroy(self) -> None:
+        if self.h:
+            _check(self.ctx.lib.dpah_batch_pass_destroy(self.h))
             self.h = None
 
     def __enter__(self):

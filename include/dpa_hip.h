@@ -295,7 +295,8 @@ int dpah_result_ftle(const dpah_result* r, float* out, uint64_t* count);
  * replayed by dpah_pass_launch on the context's stream with no host launch
  * work and no allocation.  Each replay recomputes every output from the seeds.
  * dpah_pass_total_steps waits and returns the last replay's particle-steps.
- * No counterpart in dpa (one CPU pass per run). */
+ * The field and the buffers must outlive the pass.  No counterpart in dpa (one
+ * CPU pass per run). */
 typedef struct dpah_pass dpah_pass;
 int dpah_pass_create_endpoints(dpah_context* ctx, const dpah_field* field, const float* seeds,
                                const uint64_t* remaining, uint64_t uniform_remaining,
@@ -305,6 +306,27 @@ int dpah_pass_create_endpoints(dpah_context* ctx, const dpah_field* field, const
 int dpah_pass_launch(dpah_pass* pass);
 int dpah_pass_total_steps(dpah_pass* pass, uint64_t* steps);
 int dpah_pass_destroy(dpah_pass* pass);
+
+/* A repeated batch captured into a hipGraph: the jobs of dpah_advect_batch
+ * (host arrays, read at creation only) planned, uploaded and captured once --
+ * per group of jobs sharing field, step size, start time, integrator and
+ * record flag: the counter reset, the persistent launch over the group's
+ * segment table and, for recorded groups, every round's vertex / index counts
+ * and scan.  dpah_batch_pass_launch replays them on the context's stream with
+ * no host launch work and no allocation; each replay recomputes every output
+ * from the uploaded seeds.  dpah_batch_pass_results waits for the last
+ * replay, reads the totals and final states back and writes every round's
+ * vertices, colours and indices (sizes the replay decides, so this part is
+ * not captured), returning one result per job exactly as dpah_advect_batch
+ * does; the caller frees them with dpah_result_free.  The jobs' fields must
+ * outlive the pass (the graph holds their device addresses).  No counterpart
+ * in dpa (one CPU pass per run). */
+typedef struct dpah_batch_pass dpah_batch_pass;
+int dpah_batch_pass_create(dpah_context* ctx, const dpah_job* jobs, uint64_t job_count,
+                           dpah_batch_pass** out);
+int dpah_batch_pass_launch(dpah_batch_pass* pass);
+int dpah_batch_pass_results(dpah_batch_pass* pass, dpah_result** results);
+int dpah_batch_pass_destroy(dpah_batch_pass* pass);
 
 /* Kernel timing recorded with HIP events on the context's stream (enabled by
  * dpah_options.kernel_timing): launches and summed milliseconds of the named

@@ -59,3 +59,52 @@ def test_graph_replays_match_eager_and_oracle(ctx, n):
         for p in (d_s, d_p, d_r, d_t, e_p, e_r, e_t):
             ctx.free(p)
         fld.release()
+
+
+def test_batch_pass_replays_match_eager_batch_and_oracle(ctx):
+    """A captured batch (three launch groups: recorded at two step sizes and an
+    endpoint group with both integrators' jobs split by integrator) replayed
+    three times, with an eager batch of the same jobs before every replay."""
+    from test_gpu_batch import result_arrays, same
+    from test_gpu_parity import compare_advect
+    say("case batch pass: 9 jobs, recorded and endpoint groups, 3 replays")
+    field = W.vortex_field(64, count=24)
+    fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
+    ofld = po.Field(field, (0, 0, 0), (1, 1, 1))
+    rng = np.random.default_rng(77)
+    jobs = []
+    for k in range(9):
+        n = int(np.exp(rng.uniform(np.log(16), np.log(1500))))
+        lo = rng.uniform(0, 40, 3)
+        seeds = W.uniform_seeds(n, lo, lo + rng.uniform(4, 20, 3), seed_offset=700 + k)
+        rem = rng.integers(8, 200, n) if k % 3 == 0 else None
+        jobs.append(dict(field=fld, seeds=seeds, remaining=rem,
+                         uniform_remaining=int(rng.integers(8, 200)),
+                         particles_per_round=None if k % 2 else max(1, n // 3),
+                         step_size=f32(0.25 if k < 6 else 0.2), record=k not in (2, 5),
+                         integrator=po.RK4 if k == 5 else po.DPA_RK4, use_64_bit=(k == 4)))
+    try:
+        with pytest.raises(dpa_amd.DpahError):
+            ctx.batch_pass([])
+        with ctx.batch_pass(jobs) as bp:
+            with pytest.raises(dpa_amd.DpahError):
+                bp.results()   # nothing replayed yet
+            for rep in range(3):
+                eager = ctx.advect_batch(jobs)
+                bp.launch()
+                got = bp.results()
+                for k, (g, e) in enumerate(zip(got, eager)):
+                    same(result_arrays(g), result_arrays(e))
+                    if rep == 0:
+                        j = jobs[k]
+                        n = len(j["seeds"])
+                        rem = j["remaining"] if j["remaining"] is not None else j["uniform_remaining"]
+                        ora = po.run_rounds(ofld, j["seeds"], rem, j["particles_per_round"] or n,
+                                            j["step_size"], integrator=j["integrator"],
+                                            record=j["record"], use_64_bit=j["use_64_bit"])
+                        compare_advect(g, ora, n)
+                for r in got + eager:
+                    r.free()
+                say(f"  replay {rep} matches")
+    finally:
+        fld.release()
