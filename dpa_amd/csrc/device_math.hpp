@@ -57,17 +57,19 @@ struct Axis {
 // fmodf(d, h) for d >= 0 (or -0) and h > 0, given s = floorf(fl(d / h)).
 // fl(d/h) is monotone in the exact quotient, so s is the exact floor k or k+1.
 // When s == k the exact remainder d - k*h is representable (it is fmod's
-// result), so one fmaf returns it without rounding -- for s = +-0 that is d
-// itself, signed zero included; when s == k+1 the fmaf result is negative
-// (rounding keeps the sign of a non-zero value) and the fmaf with k is exact.
-// s = +-0 with d < 0 (a tiny negative d whose quotient underflowed to -0)
-// keeps d, as fmodf does.  Both candidates are computed and one selected: no
-// branch.
+// result), so one fmaf returns it without rounding; when s == k+1 the fmaf
+// result is negative (rounding keeps the sign of a non-zero value) and the
+// fmaf with k is exact.  s = +-0 with d < 0 (a tiny negative d whose quotient
+// underflowed to -0) keeps d, as fmodf does.  Both candidates are computed and
+// one selected: no branch.  fmodf's result carries d's sign, also when it is
+// zero: d = -0 gives s = -0 and an fmaf sum +0 + -0 = +0, so the sign is set
+// from d at the end (fmodf(-0, h) = -0, and the weight -0 / h = -0 matters
+// wherever it multiplies a corner value of -0).
 __device__ __forceinline__ float exact_fmod(float d, float h, float s) {
   const float r = __builtin_fmaf(-s, h, d);
   const float k = s - 1.0f;
   const float r1 = __builtin_fmaf(-k, h, d);
-  return (r < 0.0f && s != 0.0f) ? r1 : r;
+  return __builtin_copysignf((r < 0.0f && s != 0.0f) ? r1 : r, d);
 }
 
 // d / h from r = fl(1/h): q0 = fl(d*r), the residual d - q0*h is exact in one
@@ -151,7 +153,7 @@ __device__ __forceinline__ uint32_t axis_inside(float s, int32_t n) {
 template <int kDiv>
 __device__ __forceinline__ float axis_weight(float d, float s, float h, float rh) {
   if constexpr (kDiv == kDivExact) {
-    const float r = __builtin_fmaf(-s, h, d);
+    const float r = __builtin_copysignf(__builtin_fmaf(-s, h, d), d);   // as exact_fmod
     return r * rh;
   } else {
     const float r = exact_fmod(d, h, s);

@@ -7,7 +7,8 @@
 // and on a non-power-of-two grid), and on grids whose spacings are all powers
 // of two the reciprocal multiply with fmod as one fmaf (kDivExact); so every
 // mode is pinned against the correctly rounded divisions and fmodf, also at
-// boundary probes (signed zeros, denormals, faces, the last cell).
+// boundary probes (signed zeros, denormals, faces, the last cell).  Each
+// axis's floor and weight are also pinned directly to floorf and fmodf.
 // Prints "samples S shifts F diffs D" and exits non-zero on a diff.
 #include <hip/hip_runtime.h>
 #undef __device__
@@ -94,6 +95,43 @@ static long probes(const FieldView& e, long& samples) {
   return diffs;
 }
 
+// One axis's floor and weight against libm, as regular_grid.hpp:61-62 computes
+// them: s = floorf(d / h) and w = fmodf(d, h) / h, bit for bit (signed zeros
+// included), in division mode kDiv, for boundary offsets d (signed zeros,
+// denormals, tiny values, every face and its neighbours) and random ones in
+// [-h, n h).  Weights are compared where the cell is inside, as only there are
+// they used.
+template <int kDiv>
+static long axis_probes(float o, float h, int32_t n, std::mt19937& rng, long& samples) {
+  const float rh = 1.0f / h;
+  const float tiny[] = {0.0f, -0.0f, 1e-45f, -1e-45f, 1e-38f, -1e-38f, 1e-30f, -1e-30f,
+                        -1e-7f, 1e-7f};
+  std::vector<float> xs;
+  for (float d : tiny) xs.push_back(o + d);
+  for (int k = 0; k < n; ++k) {
+    const float face = o + h * (float)k;
+    xs.push_back(face);
+    xs.push_back(std::nextafter(face, -1e30f));
+    xs.push_back(std::nextafter(face, 1e30f));
+  }
+  std::uniform_real_distribution<float> u(-h, h * (float)n);
+  for (int k = 0; k < 20000; ++k) xs.push_back(o + u(rng));
+  long diffs = 0;
+  for (float x : xs) {
+    for (float d : {x - o, -0.0f, 0.0f}) {
+      const float s = axis_floor<kDiv>(d, h, rh);
+      const float ref_s = floorf(d / h);
+      ++samples;
+      if (std::memcmp(&s, &ref_s, 4) != 0) ++diffs;
+      if (!axis_inside(ref_s, n)) continue;
+      const float w = axis_weight<kDiv>(d, s, h, rh);
+      const float ref_w = fmodf(d, h) / h;
+      if (std::memcmp(&w, &ref_w, 4) != 0) ++diffs;
+    }
+  }
+  return diffs;
+}
+
 int main() {
   const int n0 = 19, n1 = 23, n2 = 29, nt = 5;
   std::vector<float> data(3ull * n0 * n1 * n2 * nt);
@@ -112,7 +150,18 @@ int main() {
   FieldView unit = f;
   unit.o0 = unit.o1 = unit.o2 = 0.0f; unit.h0 = unit.h1 = unit.h2 = 1.0f;
   unit.r0 = unit.r1 = unit.r2 = 1.0f;
-  long diffs = walk<false, kDivide>(unit, rng, samples, shifts);
+  long diffs = 0;
+  for (float h : {0.5f, 1.0f, 0.75f, 2.0f, 0.25f}) {   // spacings the walks below use
+    diffs += axis_probes<kDivide>(0.0f, h, 23, rng, samples);
+    diffs += axis_probes<kDivide>(-1.5f, h, 23, rng, samples);
+    diffs += axis_probes<kDivFast>(0.0f, h, 23, rng, samples);
+    diffs += axis_probes<kDivFast>(2.0f, h, 23, rng, samples);
+  }
+  for (float h : {0.5f, 1.0f, 2.0f, 0.25f}) {          // powers of two
+    diffs += axis_probes<kDivExact>(0.0f, h, 23, rng, samples);
+    diffs += axis_probes<kDivExact>(0.25f, h, 23, rng, samples);
+  }
+  diffs += walk<false, kDivide>(unit, rng, samples, shifts);
   diffs += walk<false, kDivide>(f, rng, samples, shifts);
   diffs += walk<false, kDivFast>(f, rng, samples, shifts);   // 0.75 is not a power of two
   FieldView g = unit;
