@@ -62,20 +62,40 @@ __host__ __device__ inline uint64_t lattice_brick_index(uint32_t d0, uint32_t d1
   return ((uint64_t)(B * b0 + i0) * d1 + (B * b1 + i1)) * (uint64_t)d2 + (B * b2 + i2);
 }
 
-// Record slots are tiled by 64 rows: slot j of row r lives at float
-// 3 * ((r / 64 * stride + j) * 64 + r % 64).  The 64 lanes of a wave that hold
-// rows 64t .. 64t+63 at the same step write one contiguous 768-byte run (six
-// whole lines) instead of 64 partial lines of 64 rows, and a row's slots are
-// kSlotPitch floats apart.  Slot buffers hold whole tiles (tiled_rows rows).
+// Record slots: rows tiled by 64, each row's slots grouped in blocks of 8.
+// Block b of row r (slots 8b .. 8b+7, 96 bytes: three whole 32-byte sectors)
+// lives at float 24 * ((r / 64 * B + b) * 64 + r % 64), B = ceil(stride / 8)
+// blocks per row, so block b of the 64 rows of a tile is one contiguous
+// 6144-byte run and slot j of a row is slot_offset(j) floats past its slot 0.
+// A lane writes a whole block at a time (staged, below), and a retired row
+// leaves whole sectors untouched rather than 12-byte holes in every line.
+// Slot buffers hold whole tiles (tiled_rows rows) of whole blocks.
 constexpr uint64_t kTileRows = 64;
-constexpr uint64_t kSlotPitch = 3 * kTileRows;
+constexpr uint64_t kSlotGroup = 8;                               // slots per block
+constexpr uint64_t kBlockFloats = 3 * kSlotGroup;                // one row's block
+constexpr uint64_t kTileBlockFloats = kBlockFloats * kTileRows;  // block b of a tile
 
 __host__ __device__ inline uint64_t tiled_rows(uint64_t rows) {
   return (rows + kTileRows - 1) / kTileRows * kTileRows;
 }
 
+__host__ __device__ inline uint64_t slot_blocks(uint64_t stride) {
+  return (stride + kSlotGroup - 1) / kSlotGroup;
+}
+
+// Floats of the slot region of `rows` rows of `stride` slots.
+__host__ __device__ inline uint64_t slot_region_floats(uint64_t rows, uint64_t stride) {
+  return tiled_rows(rows) * slot_blocks(stride) * kBlockFloats;
+}
+
+// Slot j of a row, relative to its slot 0.
+__host__ __device__ inline uint64_t slot_offset(uint64_t j) {
+  return j / kSlotGroup * kTileBlockFloats + j % kSlotGroup * 3;
+}
+
 __host__ __device__ inline uint64_t slot_float(uint64_t row, uint64_t stride, uint64_t j) {
-  return 3 * ((row / kTileRows * stride + j) * kTileRows + row % kTileRows);
+  return (row / kTileRows * slot_blocks(stride) * kTileRows + row % kTileRows) * kBlockFloats +
+         slot_offset(j);
 }
 
 // A contiguous run of particles with one slot layout: one dpa round of one job.
@@ -105,7 +125,7 @@ struct AdvectArgs {
   uint8_t*        status_out;   // [n] or null
   uint32_t*       steps_out;    // [n] or null
   uint8_t*        special_out;  // [n] or null (record): a vertex equals a sentinel
-  float*          slots;        // record: rows tiled by 64 (slot_float), slot 0 = seed
+  float*          slots;        // record: slot layout of slot_float, slot 0 = seed
   uint64_t        stride;
   uint64_t        n;
   float           t0;
@@ -116,6 +136,7 @@ struct AdvectArgs {
   uint32_t        parts;            // queue parts (one per XCD), 1..8
   uint64_t        part_lo[9];       // part p covers positions [part_lo[p], part_lo[p+1])
   uint32_t        refill;           // a wave draws new particles once this many lanes idle
+  uint32_t        stage_slots;      // record, persistent: stage slot blocks in LDS
 };
 
 __device__ __forceinline__ bool is_sentinel(float a, float b, float c) {
@@ -140,13 +161,73 @@ __device__ __forceinline__ void add_wave_steps(unsigned long long* total, uint64
 // Particle registers carried across the step loop.
 struct Lane {
   CellCache cc;    // corners of the last sampled cell
-  float* row;      // record: this particle's slot row
+  float* row;      // record: this particle's slot row (its slot 0)
+  float* stage;    // staged record: this lane's slot 0 in the workgroup's LDS stage
+  bool staged;     // launch-uniform: slots go through the stage (else straight out)
   uint64_t pid;
   uint64_t rem;
   uint32_t it;
   float p0, p1, p2;
   bool special;
 };
+
+// Slot writes.  kSlotDirect stores each slot as it comes.  The staged modes
+// (recorded persistent launches) keep a lane's current block in LDS --
+// slot-major over the wave, slot s of lane l at float (s * 64 + l) * 3 of the
+// wave's stage, so the 64 lanes' 12-byte accesses fall on distinct banks --
+// and write the block's 96 bytes at once: kSlotStaged when the lane fills its
+// block (or at the terminal vertex), kSlotDeferred leaves full blocks to the
+// caller (flush_wave_block, one contiguous run for the whole wave).  Only a
+// lane and its own wave touch its stage entries, in program order: no barrier.
+// Slots past the terminal vertex get stale values; nothing reads a row past
+// its slot k + 1, and the block lies inside the row.
+enum SlotMode : int { kSlotDirect = 0, kSlotStaged = 1, kSlotDeferred = 2 };
+constexpr uint32_t kStageFloatsPerWave = (uint32_t)(kSlotGroup * kTileRows * 3);
+
+__device__ __forceinline__ void flush_block(const Lane& L, uint32_t b) {
+  // slot by slot (three registers in flight): the eight 12-byte stores follow
+  // one another, so each 32-byte sector is whole in L2 before it is written
+  float* dst = L.row + (uint64_t)b * kTileBlockFloats;
+#pragma unroll
+  for (uint32_t q = 0; q < kSlotGroup; ++q) {
+    const float* src = L.stage + q * (3 * kTileRows);
+    put3(dst + 3 * q, src[0], src[1], src[2]);
+  }
+}
+
+// Block b of the lanes in `mask`, for a wave whose lane l holds row 64t + l of
+// one tile (tile = its rows' first block, block 0): the blocks of a tile are
+// one 6144-byte run, written as 384 pieces of 16 bytes, piece m = lane + 64 q
+// holding floats 4 (m % 6) .. +3 of row m / 6's block.  Every lane of the wave
+// takes part; pieces of rows outside `mask` are skipped (whole sectors).
+__device__ __forceinline__ void flush_wave_block(const float* wave_stage, float* tile, uint32_t b,
+                                                 uint64_t mask, uint32_t lane) {
+  float4* dst = (float4*)__builtin_assume_aligned(tile + (uint64_t)b * kTileBlockFloats, 16);
+#pragma unroll 1
+  for (uint32_t q = 0; q < kBlockFloats / 4; ++q) {   // one piece (4 registers) at a time
+    const uint32_t m = lane + (uint32_t)kTileRows * q, r = m / 6, c = 4 * (m % 6);
+    if ((mask >> r) & 1ull) {
+      float v[4];
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e) {
+        const uint32_t f = c + e;   // float f of the block: slot f / 3, component f % 3
+        v[e] = wave_stage[((f / 3) * (uint32_t)kTileRows + r) * 3 + f % 3];
+      }
+      dst[m] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+  }
+}
+
+template <int kSlots>
+__device__ __forceinline__ void put_slot(const Lane& L, uint32_t j, float x, float y, float z) {
+  if (kSlots == kSlotDirect || !L.staged) {
+    put3(L.row + slot_offset(j), x, y, z);
+  } else {
+    put3(L.stage + (j % kSlotGroup) * (3 * kTileRows), x, y, z);
+    if constexpr (kSlots == kSlotStaged)
+      if (j % kSlotGroup == kSlotGroup - 1) flush_block(L, j / (uint32_t)kSlotGroup);
+  }
+}
 
 // position = offset + stride * multi_index, each product and sum rounded
 // (uniform_seed_generator.cpp:35-36 with indexing.hpp:10-20's C-order unravel).
@@ -209,7 +290,7 @@ __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, flo
   return s.first + local;
 }
 
-template <bool kRecord>
+template <bool kRecord, int kSlots = kSlotDirect>
 __device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uint64_t q) {
   float* row = nullptr;
   const uint64_t pid = resolve(a, q, row);
@@ -222,19 +303,26 @@ __device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uin
   if constexpr (kRecord) {
     L.row = row;
     L.special = is_sentinel(L.p0, L.p1, L.p2);
-    put3(row, L.p0, L.p1, L.p2);
+    put_slot<kSlots>(L, 0, L.p0, L.p1, L.p2);   // slot 0 never ends a block
   }
 }
 
 // kCountIt: the loop counted the particle's steps in L.it; otherwise they are
 // its starting iterations minus the ones left.
-template <bool kRecord, bool kCountIt = true>
+// Staged: the caller has flushed every full block before the terminal vertex.
+template <bool kRecord, bool kCountIt = true, int kSlots = kSlotDirect>
 __device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const Lane& L, uint8_t st) {
   const uint64_t pid = L.pid;
   uint32_t it = L.it;
   if constexpr (!kCountIt) it = (uint32_t)((a.rem_in ? a.rem_in[pid] : a.uniform_rem) - L.rem);
   if constexpr (kRecord) {
-    put3(L.row + kSlotPitch * ((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
+    const uint32_t j = L.it + 1;   // the terminal vertex
+    if (kSlots == kSlotDirect || !L.staged) {
+      put_slot<kSlotDirect>(L, j, -2.0f, -2.0f, -2.0f);
+    } else {
+      put_slot<kSlotDeferred>(L, j, -2.0f, -2.0f, -2.0f);
+      flush_block(L, j / (uint32_t)kSlotGroup);   // the row's last block, full or not
+    }
     if (a.special_out) a.special_out[pid] = L.special ? 1 : 0;
   }
   put3(a.pos_out + 3 * pid, L.p0, L.p1, L.p2);
@@ -247,7 +335,8 @@ __device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const L
 // One iteration of the loop at particle_advector.cpp:336-389 on field f with
 // step coefficients c and start time t0.  Returns the retirement status, or
 // kActive while the particle keeps going.
-template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv, bool kCountIt = true>
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv, bool kCountIt = true,
+          int kSlots = kSlotDirect>
 __device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef& c, float t0,
                                              Lane& L) {
   if (L.rem == 0) return kCompleted;
@@ -267,7 +356,7 @@ __device__ __forceinline__ uint8_t step_lane(const FieldView& f, const StepCoef&
   L.rem -= 1;
   if constexpr (kCountIt) L.it += 1;
   if constexpr (kRecord) {
-    put3(L.row + kSlotPitch * (uint64_t)L.it, L.p0, L.p1, L.p2);
+    put_slot<kSlots>(L, L.it, L.p0, L.p1, L.p2);
     L.special |= is_sentinel(L.p0, L.p1, L.p2);
   }
   return L.rem == 0 ? kCompleted : kActive;
@@ -307,10 +396,22 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   // slot index (record) and time (time-variant fields, classic RK4) need the
   // step count inside the loop; steady dpa-mode endpoints only at the end
   constexpr bool kCountIt = kRecord || kUnsteady || kIntegrator == kRk4;
+  // recorded launches stage slot blocks (4 waves of at most 256 threads)
+  constexpr int kSlots = kRecord ? kSlotDeferred : kSlotDirect;
+  __shared__ float stage[kRecord ? 4 * kStageFloatsPerWave : 1];
+  float* const wave_stage = stage + (threadIdx.x >> 6) * kStageFloatsPerWave;
+  // Refill threshold 64 and no order table: a wave draws 64 aligned queue
+  // positions at once, so lane l holds row 64t + l of one tile and the busy
+  // lanes step in lockstep (same slot index): full blocks go out as one run
+  // per wave.  Otherwise each lane writes its own block.
+  const bool wave_runs = a.refill == 64 && a.order == nullptr;
+  const bool staged = kRecord && a.stage_slots != 0;
   const uint32_t lane = threadIdx.x & 63;
   Lane L;
   cache_clear(L.cc);
   L.row = nullptr;
+  L.stage = wave_stage + 3 * lane;
+  L.staged = staged;
   L.pid = ~0ull;
   L.rem = 0; L.it = 0; L.p0 = L.p1 = L.p2 = 0.0f; L.special = false;
   uint32_t part = blockIdx.x % a.parts;   // wave-uniform
@@ -331,7 +432,7 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
         if (idle) {
           const uint64_t pos = lo + base + lane_rank(need);
           const uint64_t q = pos < hi ? (a.order ? (uint64_t)a.order[pos] : pos) : a.n;
-          if (q < a.n) begin_particle<kRecord>(a, L, q);   // q < n also guards the order table
+          if (q < a.n) begin_particle<kRecord, kSlots>(a, L, q);   // q < n also guards the order table
         }
         if (lo + base + cnt >= hi) {
           part = part + 1 == a.parts ? 0 : part + 1;
@@ -342,12 +443,32 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
     const bool busy = (L.pid != ~0ull);
     // a wave whose lanes all found a drained part keeps drawing from the next
     if (!__any(busy) && !open) break;
+    uint8_t st = kActive;
+    bool full = false;   // this step's slot completed a block
     if (busy) {
-      const uint8_t st = step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt>(a.f, a.coef, a.t0, L);
-      if (st != kActive) {
-        mine += finish_particle<kRecord, kCountIt>(a, L, st);
-        L.pid = ~0ull;
+      const uint32_t it0 = L.it;
+      st = step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt, kSlots>(a.f, a.coef, a.t0, L);
+      full = staged && L.it != it0 && L.it % kSlotGroup == kSlotGroup - 1;
+    }
+    if (staged) {   // full blocks out before a terminal vertex reuses the stage
+      if (wave_runs) {
+        const uint64_t fm = __ballot(full);
+        if (fm) {
+          const int src = __ffsll((unsigned long long)fm) - 1;
+          const uint32_t b = (uint32_t)__shfl((int)(L.it / (uint32_t)kSlotGroup), src, 64);
+          const uint64_t t = (uint64_t)(L.row - kBlockFloats * lane);   // the tile's block 0
+          const uint32_t tlo = (uint32_t)__shfl((int)(uint32_t)t, src, 64);
+          const uint32_t thi = (uint32_t)__shfl((int)(uint32_t)(t >> 32), src, 64);
+          float* tile = (float*)(((uint64_t)thi << 32) | (uint64_t)tlo);
+          flush_wave_block(wave_stage, tile, b, fm, lane);
+        }
+      } else if (full) {
+        flush_block(L, L.it / (uint32_t)kSlotGroup);
       }
+    }
+    if (busy && st != kActive) {
+      mine += finish_particle<kRecord, kCountIt, kSlots>(a, L, st);
+      L.pid = ~0ull;
     }
   }
   add_wave_steps(a.total_steps, mine);

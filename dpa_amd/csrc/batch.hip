@@ -140,7 +140,7 @@ struct GroupRun {
       segs[s].stride = record ? r.stride : 0;
       segs[s].slot_base = slot_floats;
       q += tiled_rows(r.count);   // segments start on 64-ordinal boundaries (whole waves)
-      if (record) slot_floats += 3 * tiled_rows(r.count) * r.stride;
+      if (record) slot_floats += slot_region_floats(r.count, r.stride);
     }
     row_off.assign(segs.size(), 0);
     sums_off.assign(segs.size(), 0);

@@ -583,7 +583,7 @@ dpah_result* dpah::run_advect(dpah_context* ctx, const dpah_field* field, const 
         Round round;
         if (rc) {
           // chunks of whole 64-row tiles (the slot layout), within the slot budget
-          const uint64_t row_bytes = stride * 3 * sizeof(float);
+          const uint64_t row_bytes = slot_blocks(stride) * kBlockFloats * sizeof(float);
           uint64_t chunk = std::max<uint64_t>(1, ctx->slot_budget / row_bytes) / kTileRows *
                            kTileRows;
           chunk = std::min(std::max<uint64_t>(chunk, kTileRows), rc);

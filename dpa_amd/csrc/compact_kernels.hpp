@@ -3,7 +3,7 @@
 // (index_generator.cpp:10-39) and color_generator::generate_from_angular_velocities
 // (color_generator.cpp:37-58) fused over one chunk of a round's slots.
 //
-// Slot row p (stride s, rows tiled by 64: slot_float in advect_kernels.hpp)
+// Slot row p (stride s, the block layout of slot_float in advect_kernels.hpp)
 // holds particle p's seed, one vertex per step and the terminal vertex at
 // k_p + 1; rows are never pre-filled, so the kernels read only slots
 // [0, k_p + 1].  For a row with no sentinel-valued vertex the pruned curve is
@@ -18,7 +18,7 @@
 namespace dpah {
 
 struct CompactArgs {
-  const float*    slots;     // rows tiled by 64 (slot_float), whole tiles
+  const float*    slots;     // slot layout of slot_float, whole tiles
   uint64_t        stride;
   const uint32_t* steps;     // k_p
   const uint8_t*  special;   // row holds a sentinel-valued vertex
@@ -92,7 +92,7 @@ static __global__ __launch_bounds__(256) void k_compact_count(CompactArgs a) {
   uint64_t kept = 0, pairs = 0;
   bool have_prev = false, prev_term = false;
   for (uint64_t j = 0; j <= k + 1; ++j) {
-    const float* v = row + kSlotPitch * j;
+    const float* v = row + slot_offset(j);
     if (eq3(v, -1.0f)) continue;
     const bool term = eq3(v, -2.0f);
     if (have_prev && !prev_term && !term) ++pairs;
@@ -114,12 +114,12 @@ static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
   uint64_t m = 0, pairs = 0;
   int64_t prev_j = -1;
   for (uint64_t j = 0; j <= k + 1; ++j) {
-    const float* v = row + kSlotPitch * j;
+    const float* v = row + slot_offset(j);
     if (eq3(v, -1.0f)) continue;
     // next kept slot
     int64_t next_j = -1;
     for (uint64_t q = j + 1; q <= k + 1; ++q)
-      if (!eq3(row + kSlotPitch * q, -1.0f)) { next_j = (int64_t)q; break; }
+      if (!eq3(row + slot_offset(q), -1.0f)) { next_j = (int64_t)q; break; }
     const uint64_t g = a.vertex_base + vb + m;
     float* out = a.vertices + 3 * (vb + m);
     out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
@@ -127,12 +127,12 @@ static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
     const bool term = eq3(v, -2.0f);
     if (!term) {
       float prev[3] = {0.0f, 0.0f, 0.0f}, next[3] = {0.0f, 0.0f, 0.0f};
-      if (g > 1 && prev_j >= 0 && !eq3(row + kSlotPitch * prev_j, -2.0f)) {
-        const float* u = row + kSlotPitch * prev_j;
+      if (g > 1 && prev_j >= 0 && !eq3(row + slot_offset((uint64_t)prev_j), -2.0f)) {
+        const float* u = row + slot_offset((uint64_t)prev_j);
         normalized3(v[0] - u[0], v[1] - u[1], v[2] - u[2], prev);
       }
-      if (next_j >= 0 && !eq3(row + kSlotPitch * next_j, -2.0f)) {
-        const float* u = row + kSlotPitch * next_j;
+      if (next_j >= 0 && !eq3(row + slot_offset((uint64_t)next_j), -2.0f)) {
+        const float* u = row + slot_offset((uint64_t)next_j);
         normalized3(u[0] - v[0], u[1] - v[1], u[2] - v[2], next);
         put_index(a, ib + pairs, g);
         ++pairs;
@@ -199,8 +199,9 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
   uint32_t nmax = 0;
   for (uint32_t i = 0; i < kGroupRows; ++i) nmax = max(nmax, len[i]);
   nmax = (uint32_t)__builtin_amdgcn_readfirstlane(nmax);   // same in every lane
-  // slot j of group row l: float 3 * (64 j + l) past band
-  const float* band = a.slots + slot_float((uint64_t)tile * kTileRows, a.stride, 0) + 3 * grp * kGroupRows;
+  // slot j of group row l: float ((j / 8 * 64 + grp * kGroupRows + l) * 24 + j % 8 * 3)
+  // past the tile's first block
+  const float* tile_base = a.slots + slot_float((uint64_t)tile * kTileRows, a.stride, 0);
   const uint32_t h = t / kRunLanes, i = t % kRunLanes;
   const uint32_t jb = span * kCompactSpan, je = min(nmax, jb + kCompactSpan);
 
@@ -223,9 +224,10 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
     // resource sits in scalar registers (readfirstlane returns int: each half
     // goes through uint32_t before widening, or bit 31 of the low half would
     // be sign-extended into the high half)
+    const uint32_t b0 = lo / (uint32_t)kSlotGroup;   // the chunk's blocks b0 .. hi / 8
     const uint32_t nbytes = (uint32_t)__builtin_amdgcn_readfirstlane(
-        (hi - lo + 1) * (uint32_t)kSlotPitch * 4u);   // <= (kCompactChunk + 2) * 768
-    const uint64_t base = (uint64_t)(band + (uint64_t)lo * kSlotPitch);
+        (hi / (uint32_t)kSlotGroup - b0 + 1) * (uint32_t)kTileBlockFloats * 4u);   // <= 10 * 6144
+    const uint64_t base = (uint64_t)(tile_base + (uint64_t)b0 * kTileBlockFloats);
     const uint32_t base_lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
     const uint32_t base_hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
     const uint64_t ubase = ((uint64_t)base_hi << 32) | (uint64_t)base_lo;
@@ -235,7 +237,10 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
     for (uint32_t q = 0; q < kPer; ++q) {
       const uint32_t x = t + kCompactThreads * q, l = x % kGroupRows, s = lo + x / kGroupRows;
       const bool ok = s <= hi && s < len[l];
-      const uint32_t off = ok ? ((s - lo) * (uint32_t)kSlotPitch + 3 * l) * 4u : 0x80000000u;
+      const uint32_t off =
+          ok ? (((s / (uint32_t)kSlotGroup - b0) * (uint32_t)kTileRows + grp * kGroupRows + l) *
+                    (uint32_t)kBlockFloats + s % (uint32_t)kSlotGroup * 3) * 4u
+             : 0x80000000u;
       r[q] = __builtin_amdgcn_raw_buffer_load_b96(rs, (int)off, 0, 0);
     }
   };

@@ -21,7 +21,7 @@ struct TickSeg {
   uint32_t* it;             // [count] steps taken so far
   uint8_t* status;          // [count], kActive until retired
   uint8_t* special;         // [count] record: a vertex equals a sentinel
-  float* slots;             // record: rows tiled by 64 (slot_float)
+  float* slots;             // record: slot layout of slot_float
   uint64_t stride;
   float dt, t0;
   unsigned int* active_left;          // particles of the request not yet retired
@@ -119,7 +119,7 @@ static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
       s.it[i] = L.it;
       tally.seg = seg;
       if (st != kActive) {
-        if constexpr (kRecord) put3(L.row + kSlotPitch * ((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
+        if constexpr (kRecord) put3(L.row + slot_offset((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
         s.status[i] = st;
         tally.retired = 1;
       }

@@ -1,0 +1,91 @@
+"""Recorded launches that stage slot blocks in LDS.
+
+Launches of at least 2^15 recorded particles keep each lane's current block of
+eight slots in LDS and write whole 96-byte blocks: as one run per wave when
+the refill threshold is 64 (lanes in lockstep, lane l on row 64t + l), lane
+by lane otherwise.  Smaller launches store every slot directly.  Either way
+the rounds must equal the oracle's bit for bit, with particles retiring at
+every position within a block (uneven remaining counts, exits, NaN seeds).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from dpa_amd import workloads as W
+from oracle import py_oracle as po
+from test_gpu_batch import result_arrays, same
+from test_gpu_parity import compare_advect
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def vortex(ctx):
+    field = W.vortex_field(64, count=24)
+    fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
+    yield field, fld
+    fld.release()
+
+
+def uneven(n, seed):
+    rng = np.random.default_rng(seed)
+    seeds = W.uniform_seeds(n, (0, 0, 0), (63, 63, 63), seed_offset=seed)
+    seeds[3::1777] = (np.nan, 5.0, 5.0)           # retires before its first step
+    seeds[9::2111] = (63.0, 20.0, 20.0)           # on the face n-1: outside
+    rem = rng.integers(0, 150, n).astype(np.uint64)   # 0: completed before a step
+    return seeds, rem
+
+
+@pytest.mark.parametrize("refill,bpc", [(0, 0), (16, 0), (1, 2)])
+def test_staged_round_matches_oracle(ctx, vortex, refill, bpc):
+    field, fld = vortex
+    say(f"case staged recorded round, refill={refill} blocks_per_cu={bpc}")
+    seeds, rem = uneven(40000, 501)
+    dt = f32(0.25)
+    ora = po.run_rounds(po.Field(field, (0, 0, 0), (1, 1, 1)), seeds, rem, 40000, dt)
+    before = ctx.tuning()
+    ctx.set_tuning(refill=refill, blocks_per_cu=bpc)
+    try:
+        res = ctx.advect(fld, seeds, remaining=rem, particles_per_round=40000, step_size=dt)
+        try:
+            compare_advect(res, ora, len(seeds))
+        finally:
+            res.free()
+    finally:
+        ctx.set_tuning(**before)
+
+
+def test_staged_batch_equals_direct_jobs(ctx, vortex):
+    """Three jobs of 15000 rows: alone each launch stores slots directly; in
+    one batch (45000 rows) the launch stages them.  Same bits either way."""
+    field, fld = vortex
+    say("case staged batch vs direct jobs")
+    jobs = []
+    for k in range(3):
+        seeds, rem = uneven(15000, 610 + k)
+        jobs.append(dict(field=fld, seeds=seeds, remaining=rem, uniform_remaining=0,
+                         particles_per_round=None if k else 6000, step_size=f32(0.25),
+                         record=True, use_64_bit=k == 2))
+    batch = ctx.advect_batch(jobs)
+    try:
+        for k, (j, rb) in enumerate(zip(jobs, batch)):
+            alone = ctx.advect(fld, j["seeds"], remaining=j["remaining"],
+                               particles_per_round=j["particles_per_round"] or len(j["seeds"]),
+                               step_size=j["step_size"], use_64_bit=j["use_64_bit"])
+            try:
+                same(result_arrays(rb), result_arrays(alone))
+            finally:
+                alone.free()
+        ora = po.run_rounds(po.Field(field, (0, 0, 0), (1, 1, 1)), jobs[1]["seeds"],
+                            jobs[1]["remaining"], 15000, f32(0.25))
+        compare_advect(batch[1], ora, 15000)
+    finally:
+        for r in batch:
+            r.free()
