@@ -23,8 +23,13 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
     ctx->timed("advect_persistent", [&] {
-      hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv>),
-                         dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
+      if (kRecord && a.slot_shift == kStagedShift)
+        hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
+                                                kRecord ? kSlotDeferred : kSlotDirect>),
+                           dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
+      else
+        hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv>),
+                           dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
     });
   } else {
     const uint64_t grid = (a.n + 255) / 256;
@@ -57,8 +62,6 @@ void dispatch_division(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, 
     default:        dispatch_integrator<kRecord, kDivide>(ctx, T, a, integrator, unsteady, persistent); break;
   }
 }
-
-constexpr uint64_t kStageMinParticles = 1ull << 15;
 
 // Clears the step counter and the queue heads (a kernel, not a memset, so the
 // sequence stays valid inside a captured graph).
@@ -217,11 +220,9 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   // (a wave's 64 lanes then write one aligned tile of slots per step)
   a.refill = (uint32_t)(T.refill ? T.refill
                                  : (L.seed_kind == kSeedArray && !L.slots ? 8 : 64));
-  // recorded launches large enough to stream their slots to HBM stage them in
-  // LDS (whole 96-byte blocks); small ones, whose slots stay in L2 and whose
-  // few waves are latency-bound, store each slot as it comes (ABC 64^3, 4096
-  // rows x 258 slots: 0.33 ms direct, 0.39 ms staged)
-  a.stage_slots = (L.slots && L.n >= kStageMinParticles) ? 1u : 0u;
+  DPAH_REQUIRE(L.slot_shift == 0 || (L.slot_shift == kStagedShift && L.persistent),
+               "staged slot blocks need a persistent launch");
+  a.slot_shift = L.slots ? L.slot_shift : 0u;
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;

@@ -62,40 +62,47 @@ __host__ __device__ inline uint64_t lattice_brick_index(uint32_t d0, uint32_t d1
   return ((uint64_t)(B * b0 + i0) * d1 + (B * b1 + i1)) * (uint64_t)d2 + (B * b2 + i2);
 }
 
-// Record slots: rows tiled by 64, each row's slots grouped in blocks of 8.
-// Block b of row r (slots 8b .. 8b+7, 96 bytes: three whole 32-byte sectors)
-// lives at float 24 * ((r / 64 * B + b) * 64 + r % 64), B = ceil(stride / 8)
-// blocks per row, so block b of the 64 rows of a tile is one contiguous
-// 6144-byte run and slot j of a row is slot_offset(j) floats past its slot 0.
-// A lane writes a whole block at a time (staged, below), and a retired row
-// leaves whole sectors untouched rather than 12-byte holes in every line.
-// Slot buffers hold whole tiles (tiled_rows rows) of whole blocks.
+// Record slots: rows tiled by 64, each row's slots grouped in blocks of 2^g
+// (g, the slot shift, is a launch-wide choice: 0 or kStagedShift).  Block b of
+// row r (slots b 2^g .. b 2^g + 2^g - 1) lives at float
+// 3 2^g ((r / 64 * B + b) * 64 + r % 64), B = ceil(stride / 2^g) blocks per
+// row, so block b of the 64 rows of a tile is one contiguous run and slot j of
+// a row is slot_offset(j, g) floats past its slot 0.  g = 0: slot j of a
+// tile's 64 rows is one 768-byte run, which a wave stepping those rows writes
+// with one store per step.  g = 3: blocks of 96 bytes (three whole 32-byte
+// sectors), which recorded launches large enough to stream their slots to
+// HBM stage in LDS and write whole: a retired row then leaves whole sectors
+// untouched rather than 12-byte holes in every run.  Slot buffers hold whole
+// tiles (tiled_rows rows) of whole blocks.
 constexpr uint64_t kTileRows = 64;
-constexpr uint64_t kSlotGroup = 8;                               // slots per block
-constexpr uint64_t kBlockFloats = 3 * kSlotGroup;                // one row's block
-constexpr uint64_t kTileBlockFloats = kBlockFloats * kTileRows;  // block b of a tile
+constexpr uint64_t kSlotPitch = 3 * kTileRows;                   // g = 0: slot to slot
+constexpr uint32_t kStagedShift = 3;
+constexpr uint64_t kSlotGroup = 1ull << kStagedShift;            // staged: slots per block
+constexpr uint64_t kBlockFloats = 3 * kSlotGroup;                // staged: one row's block
+constexpr uint64_t kTileBlockFloats = kBlockFloats * kTileRows;  // staged: block b of a tile
 
 __host__ __device__ inline uint64_t tiled_rows(uint64_t rows) {
   return (rows + kTileRows - 1) / kTileRows * kTileRows;
 }
 
-__host__ __device__ inline uint64_t slot_blocks(uint64_t stride) {
-  return (stride + kSlotGroup - 1) / kSlotGroup;
+__host__ __device__ inline uint64_t slot_blocks(uint64_t stride, uint32_t g) {
+  return (stride + (1ull << g) - 1) >> g;
 }
 
 // Floats of the slot region of `rows` rows of `stride` slots.
-__host__ __device__ inline uint64_t slot_region_floats(uint64_t rows, uint64_t stride) {
-  return tiled_rows(rows) * slot_blocks(stride) * kBlockFloats;
+__host__ __device__ inline uint64_t slot_region_floats(uint64_t rows, uint64_t stride, uint32_t g) {
+  return tiled_rows(rows) * slot_blocks(stride, g) * (3ull << g);
 }
 
 // Slot j of a row, relative to its slot 0.
-__host__ __device__ inline uint64_t slot_offset(uint64_t j) {
-  return j / kSlotGroup * kTileBlockFloats + j % kSlotGroup * 3;
+__host__ __device__ inline uint64_t slot_offset(uint64_t j, uint32_t g) {
+  return (j >> g) * (kSlotPitch << g) + (j & ((1ull << g) - 1)) * 3;
 }
 
-__host__ __device__ inline uint64_t slot_float(uint64_t row, uint64_t stride, uint64_t j) {
-  return (row / kTileRows * slot_blocks(stride) * kTileRows + row % kTileRows) * kBlockFloats +
-         slot_offset(j);
+__host__ __device__ inline uint64_t slot_float(uint64_t row, uint64_t stride, uint64_t j,
+                                               uint32_t g) {
+  return (row / kTileRows * slot_blocks(stride, g) * kTileRows + row % kTileRows) * (3ull << g) +
+         slot_offset(j, g);
 }
 
 // A contiguous run of particles with one slot layout: one dpa round of one job.
@@ -136,7 +143,8 @@ struct AdvectArgs {
   uint32_t        parts;            // queue parts (one per XCD), 1..8
   uint64_t        part_lo[9];       // part p covers positions [part_lo[p], part_lo[p+1])
   uint32_t        refill;           // a wave draws new particles once this many lanes idle
-  uint32_t        stage_slots;      // record, persistent: stage slot blocks in LDS
+  uint32_t        slot_shift;       // record: slot blocks of 2^slot_shift (kStagedShift:
+                                    // staged through LDS, persistent launches only)
 };
 
 __device__ __forceinline__ bool is_sentinel(float a, float b, float c) {
@@ -163,7 +171,6 @@ struct Lane {
   CellCache cc;    // corners of the last sampled cell
   float* row;      // record: this particle's slot row (its slot 0)
   float* stage;    // staged record: this lane's slot 0 in the workgroup's LDS stage
-  bool staged;     // launch-uniform: slots go through the stage (else straight out)
   uint64_t pid;
   uint64_t rem;
   uint32_t it;
@@ -171,7 +178,7 @@ struct Lane {
   bool special;
 };
 
-// Slot writes.  kSlotDirect stores each slot as it comes.  The staged modes
+// Slot writes.  kSlotDirect stores each slot as it comes (slot shift 0).  The staged modes
 // (recorded persistent launches) keep a lane's current block in LDS --
 // slot-major over the wave, slot s of lane l at float (s * 64 + l) * 3 of the
 // wave's stage, so the 64 lanes' 12-byte accesses fall on distinct banks --
@@ -220,8 +227,8 @@ __device__ __forceinline__ void flush_wave_block(const float* wave_stage, float*
 
 template <int kSlots>
 __device__ __forceinline__ void put_slot(const Lane& L, uint32_t j, float x, float y, float z) {
-  if (kSlots == kSlotDirect || !L.staged) {
-    put3(L.row + slot_offset(j), x, y, z);
+  if constexpr (kSlots == kSlotDirect) {
+    put3(L.row + slot_offset(j, 0), x, y, z);   // direct writes: g = 0
   } else {
     put3(L.stage + (j % kSlotGroup) * (3 * kTileRows), x, y, z);
     if constexpr (kSlots == kSlotStaged)
@@ -263,11 +270,13 @@ __device__ __forceinline__ void seed_position(const AdvectArgs& a, uint64_t pid,
   }
 }
 
-// Queue ordinal -> particle index and slot row (binary search over segments);
-// ~0 for the padding ordinals at the end of a segment's last tile.
+// Queue ordinal -> particle index and slot row (binary search over segments;
+// slot shift kShift, which is the launch's a.slot_shift); ~0 for the padding
+// ordinals at the end of a segment's last tile.
+template <uint32_t kShift = 0>
 __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, float*& row) {
   if (a.nseg == 0) {
-    row = a.slots + slot_float(q, a.stride, 0);
+    row = a.slots + slot_float(q, a.stride, 0, kShift);
     if (a.lat.brick) {   // brick-ordered lattice (whole-lattice endpoint launches)
       const uint32_t d0 = (uint32_t)a.lat.d0, d1 = (uint32_t)a.lat.d1, d2 = (uint32_t)a.lat.d2;
       if (a.seed_kind == kSeedAux) {
@@ -286,14 +295,14 @@ __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, flo
   const Segment& s = a.segs[lo];
   const uint64_t local = q - s.qbase;
   if (local >= s.count) return ~0ull;
-  row = a.slots + s.slot_base + slot_float(local, s.stride, 0);
+  row = a.slots + s.slot_base + slot_float(local, s.stride, 0, kShift);
   return s.first + local;
 }
 
 template <bool kRecord, int kSlots = kSlotDirect>
 __device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uint64_t q) {
   float* row = nullptr;
-  const uint64_t pid = resolve(a, q, row);
+  const uint64_t pid = resolve<kSlots == kSlotDirect ? 0u : kStagedShift>(a, q, row);
   L.pid = pid;
   if (pid == ~0ull) return;   // segment padding: the lane stays idle
   cache_clear(L.cc);
@@ -317,7 +326,7 @@ __device__ __forceinline__ uint64_t finish_particle(const AdvectArgs& a, const L
   if constexpr (!kCountIt) it = (uint32_t)((a.rem_in ? a.rem_in[pid] : a.uniform_rem) - L.rem);
   if constexpr (kRecord) {
     const uint32_t j = L.it + 1;   // the terminal vertex
-    if (kSlots == kSlotDirect || !L.staged) {
+    if constexpr (kSlots == kSlotDirect) {
       put_slot<kSlotDirect>(L, j, -2.0f, -2.0f, -2.0f);
     } else {
       put_slot<kSlotDeferred>(L, j, -2.0f, -2.0f, -2.0f);
@@ -391,27 +400,28 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
 // workgroups are dispatched to the XCDs round-robin, so workgroup b starts on
 // part b % parts (its XCD's share of a spatially ordered queue) and moves on to
 // the following parts once its own is drained.
-template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv>
+// kSlots (recorded launches): kSlotDeferred stages slot blocks in LDS (slot
+// shift kStagedShift), kSlotDirect stores each slot (slot shift 0).
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv, int kSlots = kSlotDirect>
 __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   // slot index (record) and time (time-variant fields, classic RK4) need the
   // step count inside the loop; steady dpa-mode endpoints only at the end
   constexpr bool kCountIt = kRecord || kUnsteady || kIntegrator == kRk4;
-  // recorded launches stage slot blocks (4 waves of at most 256 threads)
-  constexpr int kSlots = kRecord ? kSlotDeferred : kSlotDirect;
-  __shared__ float stage[kRecord ? 4 * kStageFloatsPerWave : 1];
+  static_assert(kRecord || kSlots == kSlotDirect, "endpoint launches write no slots");
+  constexpr bool kStaged = kSlots != kSlotDirect;
+  // staged launches: one stage of slot blocks per wave (4 waves of at most 256 threads)
+  __shared__ float stage[kStaged ? 4 * kStageFloatsPerWave : 1];
   float* const wave_stage = stage + (threadIdx.x >> 6) * kStageFloatsPerWave;
   // Refill threshold 64 and no order table: a wave draws 64 aligned queue
   // positions at once, so lane l holds row 64t + l of one tile and the busy
   // lanes step in lockstep (same slot index): full blocks go out as one run
   // per wave.  Otherwise each lane writes its own block.
-  const bool wave_runs = a.refill == 64 && a.order == nullptr;
-  const bool staged = kRecord && a.stage_slots != 0;
+  const bool wave_runs = kStaged && a.refill == 64 && a.order == nullptr;
   const uint32_t lane = threadIdx.x & 63;
   Lane L;
   cache_clear(L.cc);
   L.row = nullptr;
   L.stage = wave_stage + 3 * lane;
-  L.staged = staged;
   L.pid = ~0ull;
   L.rem = 0; L.it = 0; L.p0 = L.p1 = L.p2 = 0.0f; L.special = false;
   uint32_t part = blockIdx.x % a.parts;   // wave-uniform
@@ -443,14 +453,24 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
     const bool busy = (L.pid != ~0ull);
     // a wave whose lanes all found a drained part keeps drawing from the next
     if (!__any(busy) && !open) break;
-    uint8_t st = kActive;
-    bool full = false;   // this step's slot completed a block
-    if (busy) {
-      const uint32_t it0 = L.it;
-      st = step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt, kSlots>(a.f, a.coef, a.t0, L);
-      full = staged && L.it != it0 && L.it % kSlotGroup == kSlotGroup - 1;
-    }
-    if (staged) {   // full blocks out before a terminal vertex reuses the stage
+    if constexpr (!kStaged) {
+      if (busy) {
+        const uint8_t st =
+            step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt, kSlots>(a.f, a.coef, a.t0, L);
+        if (st != kActive) {
+          mine += finish_particle<kRecord, kCountIt, kSlots>(a, L, st);
+          L.pid = ~0ull;
+        }
+      }
+    } else {
+      uint8_t st = kActive;
+      bool full = false;   // this step's slot completed a block
+      if (busy) {
+        const uint32_t it0 = L.it;
+        st = step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt, kSlots>(a.f, a.coef, a.t0, L);
+        full = L.it != it0 && L.it % kSlotGroup == kSlotGroup - 1;
+      }
+      // full blocks go out before a terminal vertex reuses the stage
       if (wave_runs) {
         const uint64_t fm = __ballot(full);
         if (fm) {
@@ -465,10 +485,10 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
       } else if (full) {
         flush_block(L, L.it / (uint32_t)kSlotGroup);
       }
-    }
-    if (busy && st != kActive) {
-      mine += finish_particle<kRecord, kCountIt, kSlots>(a, L, st);
-      L.pid = ~0ull;
+      if (busy && st != kActive) {
+        mine += finish_particle<kRecord, kCountIt, kSlots>(a, L, st);
+        L.pid = ~0ull;
+      }
     }
   }
   add_wave_steps(a.total_steps, mine);

@@ -62,6 +62,7 @@ uint32_t float_bits(float f) {
 struct GroupRun {
   bool record = false;
   uint64_t N = 0, q = 0, slot_floats = 0;
+  uint32_t slot_shift = 0;
   std::vector<JobPlan> plans;
   std::vector<Segment> segs;
   std::vector<uint64_t> row_off, sums_off;
@@ -78,7 +79,7 @@ struct GroupRun {
     return CompactRows{d_slots.as<float>() + segs[s].slot_base, segs[s].stride,
                        d_steps.as<uint32_t>() + segs[s].first,
                        d_special.as<uint8_t>() + segs[s].first, segs[s].count, 0,
-                       seg_use64[s]};
+                       seg_use64[s], slot_shift};
   }
 
   void plan(dpah_context* ctx, const dpah_job* jobs, const std::vector<uint64_t>& members) {
@@ -138,9 +139,12 @@ struct GroupRun {
       segs[s].first = plans[refs[s].plan].base + r.first;
       segs[s].count = r.count;
       segs[s].stride = record ? r.stride : 0;
-      segs[s].slot_base = slot_floats;
       q += tiled_rows(r.count);   // segments start on 64-ordinal boundaries (whole waves)
-      if (record) slot_floats += slot_region_floats(r.count, r.stride);
+    }
+    slot_shift = record ? slot_shift_for(q, true) : 0u;   // one layout for the launch
+    for (size_t s = 0; s < refs.size(); ++s) {
+      segs[s].slot_base = slot_floats;
+      if (record) slot_floats += slot_region_floats(segs[s].count, segs[s].stride, slot_shift);
     }
     row_off.assign(segs.size(), 0);
     sums_off.assign(segs.size(), 0);
@@ -195,6 +199,7 @@ struct GroupRun {
     L.steps_out = record ? d_steps.as<uint32_t>() : nullptr;
     L.special_out = record ? d_special.as<uint8_t>() : nullptr;
     L.slots = record ? d_slots.as<float>() : nullptr;
+    L.slot_shift = slot_shift;
     L.n = q;
     L.dt = j0.params.step_size;
     L.t0 = j0.params.start_time;

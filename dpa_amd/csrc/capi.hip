@@ -583,11 +583,14 @@ dpah_result* dpah::run_advect(dpah_context* ctx, const dpah_field* field, const 
         Round round;
         if (rc) {
           // chunks of whole 64-row tiles (the slot layout), within the slot budget
-          const uint64_t row_bytes = slot_blocks(stride) * kBlockFloats * sizeof(float);
+          // (sized for the larger, staged block layout)
+          const uint64_t row_bytes = slot_region_floats(1, stride, kStagedShift) / kTileRows *
+                                     sizeof(float);
           uint64_t chunk = std::max<uint64_t>(1, ctx->slot_budget / row_bytes) / kTileRows *
                            kTileRows;
           chunk = std::min(std::max<uint64_t>(chunk, kTileRows), rc);
-          DeviceBuffer slots(ctx, tiled_rows(chunk) * row_bytes);
+          const uint32_t g = slot_shift_for(chunk, persistent);
+          DeviceBuffer slots(ctx, slot_region_floats(chunk, stride, g) * sizeof(float));
           for (uint64_t c0 = 0; c0 < rc; c0 += chunk) {
             const uint64_t cn = std::min(chunk, rc - c0);
             const uint64_t pid0 = first + c0;
@@ -600,11 +603,12 @@ dpah_result* dpah::run_advect(dpah_context* ctx, const dpah_field* field, const 
             L.special_out = d_special.as<uint8_t>() + pid0;
             L.slots = slots.as<float>();
             L.stride = stride;
+            L.slot_shift = g;
             L.n = cn;
             launch_advect(ctx, L);
             read_steps();
-            Chunk ch = compact_rows(ctx, slots.as<float>(), stride, L.steps_out, L.special_out, cn,
-                                    round.V, res->use64);
+            Chunk ch = compact_rows(ctx, slots.as<float>(), stride, g, L.steps_out,
+                                    L.special_out, cn, round.V, res->use64);
             round.V += ch.V;
             round.I += ch.I;
             round.chunks.push_back(ch);

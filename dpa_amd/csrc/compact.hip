@@ -23,6 +23,7 @@ void compact_count(dpah_context* ctx, const CompactRows& r, uint64_t* vcount, ui
   a.icount = icount;
   a.use64 = r.use64;
   a.vertex_base = r.vertex_base;
+  a.slot_shift = r.slot_shift;
   const uint64_t blocks = (r.count + kScanTile - 1) / kScanTile;
   const unsigned g1 = (unsigned)((r.count + 255) / 256);
   hipLaunchKernelGGL(k_compact_count, dim3(g1), dim3(256), 0, st, a);
@@ -60,6 +61,7 @@ Chunk compact_write(dpah_context* ctx, const CompactRows& r, const uint64_t* vco
   a.count = r.count;
   a.vcount = const_cast<uint64_t*>(vcount);
   a.icount = const_cast<uint64_t*>(icount);
+  a.slot_shift = r.slot_shift;
   a.use64 = r.use64;
   a.vertex_base = r.vertex_base;
   a.vertices = ch.vertices;
@@ -82,16 +84,16 @@ Chunk compact_write(dpah_context* ctx, const CompactRows& r, const uint64_t* vco
   return ch;
 }
 
-Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, const uint32_t* steps,
-                   const uint8_t* special, uint64_t count, uint64_t vertex_base, int use64,
-                   hipStream_t stream) {
+Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, uint32_t g,
+                   const uint32_t* steps, const uint8_t* special, uint64_t count,
+                   uint64_t vertex_base, int use64, hipStream_t stream) {
   hipStream_t st = stream ? stream : ctx->stream;
   auto sync = [&] {
     DPAH_HIP(hipStreamSynchronize(st));
     ctx->resolve_timings();
   };
   if (count == 0) return Chunk{};
-  const CompactRows r{slots, stride, steps, special, count, vertex_base, use64};
+  const CompactRows r{slots, stride, steps, special, count, vertex_base, use64, g};
   DeviceBuffer vcount(ctx, count * sizeof(uint64_t));
   DeviceBuffer icount(ctx, count * sizeof(uint64_t));
   const uint64_t words = compact_sums_words(count);

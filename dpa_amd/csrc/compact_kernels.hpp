@@ -33,6 +33,7 @@ struct CompactArgs {
   uint32_t        tiles;        // ceil(count / 64)
   uint32_t        spans;        // slot spans per tile
   uint32_t        xcds;         // workgroups go to the XCDs round-robin by id
+  uint32_t        slot_shift;   // slot blocks of 2^slot_shift (slot_float)
 };
 
 __device__ __forceinline__ bool eq3(const float* v, float s) {
@@ -88,11 +89,11 @@ static __global__ __launch_bounds__(256) void k_compact_count(CompactArgs a) {
     a.icount[p] = k;
     return;
   }
-  const float* row = a.slots + slot_float(p, a.stride, 0);
+  const float* row = a.slots + slot_float(p, a.stride, 0, a.slot_shift);
   uint64_t kept = 0, pairs = 0;
   bool have_prev = false, prev_term = false;
   for (uint64_t j = 0; j <= k + 1; ++j) {
-    const float* v = row + slot_offset(j);
+    const float* v = row + slot_offset(j, a.slot_shift);
     if (eq3(v, -1.0f)) continue;
     const bool term = eq3(v, -2.0f);
     if (have_prev && !prev_term && !term) ++pairs;
@@ -108,18 +109,18 @@ static __global__ __launch_bounds__(256) void k_compact_count(CompactArgs a) {
 // index and colour rules on the row's kept vertices.
 static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
   const uint64_t k = a.steps[p];
-  const float* row = a.slots + slot_float(p, a.stride, 0);
+  const float* row = a.slots + slot_float(p, a.stride, 0, a.slot_shift);
   const uint64_t vb = a.vcount[p], ib = a.icount[p];
   // list of kept slot indices is walked twice with a small lookahead
   uint64_t m = 0, pairs = 0;
   int64_t prev_j = -1;
   for (uint64_t j = 0; j <= k + 1; ++j) {
-    const float* v = row + slot_offset(j);
+    const float* v = row + slot_offset(j, a.slot_shift);
     if (eq3(v, -1.0f)) continue;
     // next kept slot
     int64_t next_j = -1;
     for (uint64_t q = j + 1; q <= k + 1; ++q)
-      if (!eq3(row + slot_offset(q), -1.0f)) { next_j = (int64_t)q; break; }
+      if (!eq3(row + slot_offset(q, a.slot_shift), -1.0f)) { next_j = (int64_t)q; break; }
     const uint64_t g = a.vertex_base + vb + m;
     float* out = a.vertices + 3 * (vb + m);
     out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
@@ -127,12 +128,12 @@ static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
     const bool term = eq3(v, -2.0f);
     if (!term) {
       float prev[3] = {0.0f, 0.0f, 0.0f}, next[3] = {0.0f, 0.0f, 0.0f};
-      if (g > 1 && prev_j >= 0 && !eq3(row + slot_offset((uint64_t)prev_j), -2.0f)) {
-        const float* u = row + slot_offset((uint64_t)prev_j);
+      if (g > 1 && prev_j >= 0 && !eq3(row + slot_offset((uint64_t)prev_j, a.slot_shift), -2.0f)) {
+        const float* u = row + slot_offset((uint64_t)prev_j, a.slot_shift);
         normalized3(v[0] - u[0], v[1] - u[1], v[2] - u[2], prev);
       }
-      if (next_j >= 0 && !eq3(row + slot_offset((uint64_t)next_j), -2.0f)) {
-        const float* u = row + slot_offset((uint64_t)next_j);
+      if (next_j >= 0 && !eq3(row + slot_offset((uint64_t)next_j, a.slot_shift), -2.0f)) {
+        const float* u = row + slot_offset((uint64_t)next_j, a.slot_shift);
         normalized3(u[0] - v[0], u[1] - v[1], u[2] - v[2], next);
         put_index(a, ib + pairs, g);
         ++pairs;
@@ -199,9 +200,10 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
   uint32_t nmax = 0;
   for (uint32_t i = 0; i < kGroupRows; ++i) nmax = max(nmax, len[i]);
   nmax = (uint32_t)__builtin_amdgcn_readfirstlane(nmax);   // same in every lane
-  // slot j of group row l: float ((j / 8 * 64 + grp * kGroupRows + l) * 24 + j % 8 * 3)
-  // past the tile's first block
-  const float* tile_base = a.slots + slot_float((uint64_t)tile * kTileRows, a.stride, 0);
+  // slot j of group row l: float (((j >> gs) * 64 + grp * kGroupRows + l) * 3 * 2^gs +
+  // (j & (2^gs - 1)) * 3) past the tile's first block, gs = a.slot_shift
+  const uint32_t gs = a.slot_shift, gmask = (1u << gs) - 1u;
+  const float* tile_base = a.slots + slot_float((uint64_t)tile * kTileRows, a.stride, 0, gs);
   const uint32_t h = t / kRunLanes, i = t % kRunLanes;
   const uint32_t jb = span * kCompactSpan, je = min(nmax, jb + kCompactSpan);
 
@@ -224,10 +226,10 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
     // resource sits in scalar registers (readfirstlane returns int: each half
     // goes through uint32_t before widening, or bit 31 of the low half would
     // be sign-extended into the high half)
-    const uint32_t b0 = lo / (uint32_t)kSlotGroup;   // the chunk's blocks b0 .. hi / 8
+    const uint32_t b0 = lo >> gs;   // the chunk's blocks b0 .. hi >> gs
     const uint32_t nbytes = (uint32_t)__builtin_amdgcn_readfirstlane(
-        (hi / (uint32_t)kSlotGroup - b0 + 1) * (uint32_t)kTileBlockFloats * 4u);   // <= 10 * 6144
-    const uint64_t base = (uint64_t)(tile_base + (uint64_t)b0 * kTileBlockFloats);
+        ((hi >> gs) - b0 + 1) * ((uint32_t)kSlotPitch << gs) * 4u);   // <= 65 * 768 or 10 * 6144
+    const uint64_t base = (uint64_t)(tile_base + (uint64_t)b0 * (kSlotPitch << gs));
     const uint32_t base_lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
     const uint32_t base_hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
     const uint64_t ubase = ((uint64_t)base_hi << 32) | (uint64_t)base_lo;
@@ -238,8 +240,8 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
       const uint32_t x = t + kCompactThreads * q, l = x % kGroupRows, s = lo + x / kGroupRows;
       const bool ok = s <= hi && s < len[l];
       const uint32_t off =
-          ok ? (((s / (uint32_t)kSlotGroup - b0) * (uint32_t)kTileRows + grp * kGroupRows + l) *
-                    (uint32_t)kBlockFloats + s % (uint32_t)kSlotGroup * 3) * 4u
+          ok ? ((((s >> gs) - b0) * (uint32_t)kTileRows + grp * kGroupRows + l) * (3u << gs) +
+                (s & gmask) * 3) * 4u
              : 0x80000000u;
       r[q] = __builtin_amdgcn_raw_buffer_load_b96(rs, (int)off, 0, 0);
     }

@@ -174,7 +174,7 @@ uint64_t footprint(const Request& r) {
       const uint64_t rc = std::min(ppr, active), first = active - rc;
       uint64_t mx = 0;
       for (uint64_t i = 0; i < rc; ++i) mx = std::max(mx, r.rem[first + i]);
-      slots += slot_region_floats(rc, mx + 2) * sizeof(float);
+      slots += slot_region_floats(rc, mx + 2, 0) * sizeof(float);
       active = first;
     } while (active > 0);
     b += slots + verts * 16 + verts * 2 * ib;
@@ -296,7 +296,7 @@ void dpah_server::setup(Request& r, hipStream_t s) {
     for (uint64_t i = 0; i < rs.count && !ftle; ++i) mx = std::max(mx, r.rem[rs.first + i]);
     rs.stride = mx + 2;
     if (record && rs.count)
-      rs.slots = DeviceBuffer(ctx, slot_region_floats(rs.count, rs.stride) * sizeof(float));
+      rs.slots = DeviceBuffer(ctx, slot_region_floats(rs.count, rs.stride, 0) * sizeof(float));
     r.rounds.push_back(std::move(rs));
     active = r.rounds.back().first;
   } while (active > 0);
@@ -502,7 +502,7 @@ void dpah_server::finish(Request& r, hipStream_t s) {
       for (RoundState& rs : r.rounds) {
         Round round;
         if (rs.count) {
-          Chunk ch = compact_rows(ctx, rs.slots.as<float>(), rs.stride,
+          Chunk ch = compact_rows(ctx, rs.slots.as<float>(), rs.stride, 0,
                                   r.it.as<uint32_t>() + rs.first,
                                   r.special.as<uint8_t>() + rs.first, rs.count, 0, res->use64, s);
           round.V = ch.V;

@@ -21,7 +21,7 @@ struct TickSeg {
   uint32_t* it;             // [count] steps taken so far
   uint8_t* status;          // [count], kActive until retired
   uint8_t* special;         // [count] record: a vertex equals a sentinel
-  float* slots;             // record: slot layout of slot_float
+  float* slots;             // record: slot layout of slot_float, slot shift 0
   uint64_t stride;
   float dt, t0;
   unsigned int* active_left;          // particles of the request not yet retired
@@ -102,7 +102,7 @@ static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
       L.row = nullptr;
       L.special = false;
       if constexpr (kRecord) {
-        L.row = s.slots + slot_float(i, s.stride, 0);
+        L.row = s.slots + slot_float(i, s.stride, 0, 0);
         L.special = s.special[i] != 0;
       }
       const StepCoef coef = make_coef(s.dt);
@@ -119,7 +119,7 @@ static __global__ __launch_bounds__(256) void k_advect_tick(TickArgs a) {
       s.it[i] = L.it;
       tally.seg = seg;
       if (st != kActive) {
-        if constexpr (kRecord) put3(L.row + slot_offset((uint64_t)L.it + 1), -2.0f, -2.0f, -2.0f);
+        if constexpr (kRecord) put3(L.row + slot_offset((uint64_t)L.it + 1, 0), -2.0f, -2.0f, -2.0f);
         s.status[i] = st;
         tally.retired = 1;
       }
@@ -159,7 +159,7 @@ static __global__ __launch_bounds__(256) void k_tick_init(TickInitArgs a) {
   a.it[i] = 0;
   a.status[i] = kActive;
   if (a.slots) {
-    put3(a.slots + slot_float(i, a.stride, 0), p0, p1, p2);
+    put3(a.slots + slot_float(i, a.stride, 0, 0), p0, p1, p2);
     a.special[i] = is_sentinel(p0, p1, p2) ? 1 : 0;
   }
 }
