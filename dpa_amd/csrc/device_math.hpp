@@ -62,14 +62,16 @@ struct Axis {
 // fmaf with k is exact.  s = +-0 with d < 0 (a tiny negative d whose quotient
 // underflowed to -0) keeps d, as fmodf does.  Both candidates are computed and
 // one selected: no branch.  fmodf's result carries d's sign, also when it is
-// zero: d = -0 gives s = -0 and an fmaf sum +0 + -0 = +0, so the sign is set
-// from d at the end (fmodf(-0, h) = -0, and the weight -0 / h = -0 matters
-// wherever it multiplies a corner value of -0).
+// zero: for d = -0, s = -0, and -s * h + d would be +0 + -0 = +0, so the
+// product is formed as -|s| * h (= -0 * h = -0, and -0 + -0 = -0; for every
+// s >= 0 it is -s * h): fmodf(-0, h) = -0, and the weight -0 / h = -0 matters
+// wherever it multiplies a corner value of -0.  The |.| and the negation are
+// operand modifiers of the fma: no extra instruction.
 __device__ __forceinline__ float exact_fmod(float d, float h, float s) {
-  const float r = __builtin_fmaf(-s, h, d);
+  const float r = __builtin_fmaf(-__builtin_fabsf(s), h, d);
   const float k = s - 1.0f;
   const float r1 = __builtin_fmaf(-k, h, d);
-  return __builtin_copysignf((r < 0.0f && s != 0.0f) ? r1 : r, d);
+  return (r < 0.0f && s != 0.0f) ? r1 : r;
 }
 
 // d / h from r = fl(1/h): q0 = fl(d*r), the residual d - q0*h is exact in one
@@ -153,7 +155,7 @@ __device__ __forceinline__ uint32_t axis_inside(float s, int32_t n) {
 template <int kDiv>
 __device__ __forceinline__ float axis_weight(float d, float s, float h, float rh) {
   if constexpr (kDiv == kDivExact) {
-    const float r = __builtin_copysignf(__builtin_fmaf(-s, h, d), d);   // as exact_fmod
+    const float r = __builtin_fmaf(-__builtin_fabsf(s), h, d);   // -0 for d = -0, as exact_fmod
     return r * rh;
   } else {
     const float r = exact_fmod(d, h, s);
