@@ -68,14 +68,25 @@ Chunk compact_write(dpah_context* ctx, const CompactRows& r, const uint64_t* vco
   a.colors = ch.colors;
   a.indices = ch.indices;
   // one workgroup per group of rows of a 64-row tile and span of slots (a
-  // row has at most `stride` slots); tiles padded to a multiple of the XCD count
+  // row has at most `stride` slots); tiles padded to a multiple of the XCD
+  // count.  Spans are whole rows when the row groups alone give 32 workgroups
+  // per CU; fewer groups split their rows into spans of whole chunks, at most
+  // as many as make up that count (a span past all its rows' ends costs a
+  // workgroup that only reads the rows' lengths)
   const uint64_t tiles = (r.count + kTileRows - 1) / kTileRows;
-  const uint64_t spans = (r.stride + kCompactSpan - 1) / kCompactSpan;
   const uint64_t xcds = (uint64_t)std::max(1, ctx->xcds);
-  const uint64_t groups = (tiles + xcds - 1) / xcds * xcds * spans * kGroupsPerTile;
+  const uint64_t row_groups = (tiles + xcds - 1) / xcds * xcds * kGroupsPerTile;
+  const uint64_t want = (uint64_t)std::max(1, ctx->cu_count) * 32;
+  const uint64_t chunks = (r.stride + kCompactChunk - 1) / kCompactChunk;
+  const uint64_t split = std::min(chunks, (want + row_groups - 1) / row_groups);
+  const uint64_t span_slots = (chunks + split - 1) / split * kCompactChunk;
+  const uint64_t spans = (r.stride + span_slots - 1) / span_slots;
+  const uint64_t groups = row_groups * spans;
+  DPAH_REQUIRE(span_slots < (1ull << 31), "rows too long for one compaction");
   DPAH_REQUIRE(groups < (1ull << 31), "too many rows or slots in one compaction");
   a.tiles = (uint32_t)tiles;
   a.spans = (uint32_t)spans;
+  a.span_slots = (uint32_t)span_slots;
   a.xcds = (uint32_t)xcds;
   ctx->timed("compact_write", st, [&] {
     hipLaunchKernelGGL(k_compact_write, dim3((unsigned)groups), dim3(kCompactThreads), 0, st, a);

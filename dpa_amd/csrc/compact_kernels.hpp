@@ -32,6 +32,7 @@ struct CompactArgs {
   uint64_t        vertex_base;  // round-level index of this chunk's first vertex
   uint32_t        tiles;        // ceil(count / 64)
   uint32_t        spans;        // slot spans per tile
+  uint32_t        span_slots;   // slots per span, a multiple of kCompactChunk
   uint32_t        xcds;         // workgroups go to the XCDs round-robin by id
   uint32_t        slot_shift;   // slot blocks of 2^slot_shift (slot_float)
 };
@@ -147,8 +148,10 @@ static __device__ void compact_special_row(const CompactArgs& a, uint64_t p) {
 }
 
 // Writes: one workgroup (four waves) per group of kGroupRows = 4 rows of a
-// 64-row tile and span of kCompactSpan slots, walked in chunks of
-// kCompactChunk = 63 vertices; wave h owns row h of the group.  Few rows per
+// 64-row tile and span of span_slots slots (whole rows when the launch has
+// enough groups to fill the chip, so that no workgroup starts on slots past
+// its rows' ends), walked in chunks of kCompactChunk = 63 vertices; wave h
+// owns row h of the group.  Few rows per
 // workgroup keep few output runs open at once (3 streams per row): on
 // 2^18 x 1000 recorded rows, 64 rows per workgroup left so many lines partly
 // written that they outgrew L2 (stores at 2.6 TB/s, compaction 2.2 ms); 8
@@ -170,7 +173,6 @@ constexpr uint32_t kRunLanes = 64;                     // lanes per row (a wave)
 constexpr uint32_t kCompactThreads = 256;              // workgroup size
 constexpr uint32_t kGroupRows = kCompactThreads / kRunLanes;   // rows per workgroup
 constexpr uint32_t kCompactChunk = kRunLanes - 1;
-constexpr uint32_t kCompactSpan = 4 * kCompactChunk;
 constexpr uint32_t kGroupsPerTile = kTileRows / kGroupRows;
 constexpr uint32_t kBandRow = 3 * (kCompactChunk + 2);   // staged slots x 3 floats, odd
 typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
@@ -205,7 +207,8 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
   const uint32_t gs = a.slot_shift, gmask = (1u << gs) - 1u;
   const float* tile_base = a.slots + slot_float((uint64_t)tile * kTileRows, a.stride, 0, gs);
   const uint32_t h = t / kRunLanes, i = t % kRunLanes;
-  const uint32_t jb = span * kCompactSpan, je = min(nmax, jb + kCompactSpan);
+  const uint32_t jb = span * a.span_slots;   // < stride
+  const uint32_t je = (uint32_t)min((uint64_t)nmax, (uint64_t)jb + a.span_slots);
 
   // Staging: item x = t, t + kCompactThreads of the chunk's (slot, row)
   // pairs, slot lo + x / kGroupRows of row x % kGroupRows, for slots
