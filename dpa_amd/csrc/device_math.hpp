@@ -83,11 +83,35 @@ __device__ __forceinline__ float exact_fmod(float d, float h, float s) {
 // (verify_division_kernel); outside that range the quotient only decides
 // "outside the grid", and both forms agree on that (same sign, magnitude
 // beyond n-1).
-__device__ __forceinline__ float div_by(float d, float h, float r) {
-  if (!(__builtin_fabsf(d) >= 0x1p-90f)) return d / h;
+__device__ __forceinline__ bool div_tiny(float d) { return !(__builtin_fabsf(d) >= 0x1p-90f); }
+
+__device__ __forceinline__ float div_by_fast(float d, float h, float r) {
   const float q0 = d * r;
   const float e = __builtin_fmaf(-q0, h, d);
   return __builtin_fmaf(e, r, q0);
+}
+
+__device__ __forceinline__ float div_by(float d, float h, float r) {
+  if (div_tiny(d)) return d / h;
+  return div_by_fast(d, h, r);
+}
+
+// div_by on three axes with one test for the rare operands: the three fast
+// quotients, then one branch (taken by no lane in practice) that divides the
+// axes whose operand is tiny, zero or NaN -- the same bits as three div_by
+// calls, with one divergent branch instead of three.
+__device__ __forceinline__ void div_by3(float d0, float h0, float r0, float d1, float h1, float r1,
+                                        float d2, float h2, float r2, float& q0, float& q1,
+                                        float& q2) {
+  q0 = div_by_fast(d0, h0, r0);
+  q1 = div_by_fast(d1, h1, r1);
+  q2 = div_by_fast(d2, h2, r2);
+  const bool t0 = div_tiny(d0), t1 = div_tiny(d1), t2 = div_tiny(d2);
+  if (t0 || t1 || t2) {
+    if (t0) q0 = d0 / h0;
+    if (t1) q1 = d1 / h1;
+    if (t2) q2 = d2 / h2;
+  }
 }
 
 // Counts the floats x in [-B, B] (bit patterns 0..hi of |x|, both signs) where
@@ -304,9 +328,15 @@ __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, f
                                        V3& v, CellCache& cc) {
   const float d0 = p0 - f.o0, d1 = p1 - f.o1, d2 = p2 - f.o2;
   Axis a0, a1, a2;
-  a0.s = axis_floor<kDiv>(d0, f.h0, f.r0);
-  a1.s = axis_floor<kDiv>(d1, f.h1, f.r1);
-  a2.s = axis_floor<kDiv>(d2, f.h2, f.r2);
+  if constexpr (kDiv == kDivFast) {   // axis_floor on three axes, one rare-operand branch
+    float q0, q1, q2;
+    div_by3(d0, f.h0, f.r0, d1, f.h1, f.r1, d2, f.h2, f.r2, q0, q1, q2);
+    a0.s = floorf(q0); a1.s = floorf(q1); a2.s = floorf(q2);
+  } else {
+    a0.s = axis_floor<kDiv>(d0, f.h0, f.r0);
+    a1.s = axis_floor<kDiv>(d1, f.h1, f.r1);
+    a2.s = axis_floor<kDiv>(d2, f.h2, f.r2);
+  }
   // every axis tested together: one branch instead of one per axis
   uint32_t in = axis_inside(a0.s, f.n0) & axis_inside(a1.s, f.n1) & axis_inside(a2.s, f.n2);
   if constexpr (kUnsteady) {
@@ -323,9 +353,14 @@ __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, f
   } else {
     (void)t;
     if (!in) return false;
-    a0.w = axis_weight<kDiv>(d0, a0.s, f.h0, f.r0);
-    a1.w = axis_weight<kDiv>(d1, a1.s, f.h1, f.r1);
-    a2.w = axis_weight<kDiv>(d2, a2.s, f.h2, f.r2);
+    if constexpr (kDiv == kDivFast) {   // axis_weight on three axes, one rare-operand branch
+      div_by3(exact_fmod(d0, f.h0, a0.s), f.h0, f.r0, exact_fmod(d1, f.h1, a1.s), f.h1, f.r1,
+              exact_fmod(d2, f.h2, a2.s), f.h2, f.r2, a0.w, a1.w, a2.w);
+    } else {
+      a0.w = axis_weight<kDiv>(d0, a0.s, f.h0, f.r0);
+      a1.w = axis_weight<kDiv>(d1, a1.s, f.h1, f.r1);
+      a2.w = axis_weight<kDiv>(d2, a2.s, f.h2, f.r2);
+    }
     v = trilinear(f, f.data, a0, a1, a2, cc);
   }
   return true;
