@@ -105,6 +105,14 @@ __host__ __device__ inline uint64_t slot_float(uint64_t row, uint64_t stride, ui
          slot_offset(j, g);
 }
 
+// Slot s (block s >> g >= b0) of row r of a tile, as a float offset from the
+// start of the tile's block b0: the compaction reads a chunk of slots relative
+// to its first block.
+__host__ __device__ inline uint32_t tile_slot_float(uint32_t s, uint32_t b0, uint32_t r,
+                                                    uint32_t g) {
+  return (((s >> g) - b0) * (uint32_t)kTileRows + r) * (3u << g) + (s & ((1u << g) - 1u)) * 3u;
+}
+
 // A contiguous run of particles with one slot layout: one dpa round of one job.
 // The queue hands out ordinals q; segment s owns [qbase, qbase + tiled_rows(count))
 // (qbase a multiple of 64, so a wave never spans two segments) and maps ordinal

@@ -202,9 +202,9 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
   uint32_t nmax = 0;
   for (uint32_t i = 0; i < kGroupRows; ++i) nmax = max(nmax, len[i]);
   nmax = (uint32_t)__builtin_amdgcn_readfirstlane(nmax);   // same in every lane
-  // slot j of group row l: float (((j >> gs) * 64 + grp * kGroupRows + l) * 3 * 2^gs +
-  // (j & (2^gs - 1)) * 3) past the tile's first block, gs = a.slot_shift
-  const uint32_t gs = a.slot_shift, gmask = (1u << gs) - 1u;
+  // slot j of group row l: tile_slot_float(j, b0, grp * kGroupRows + l, gs) past the
+  // tile's block b0, gs = a.slot_shift
+  const uint32_t gs = a.slot_shift;
   const float* tile_base = a.slots + slot_float((uint64_t)tile * kTileRows, a.stride, 0, gs);
   const uint32_t h = t / kRunLanes, i = t % kRunLanes;
   const uint32_t jb = span * a.span_slots;   // < stride
@@ -242,10 +242,7 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
     for (uint32_t q = 0; q < kPer; ++q) {
       const uint32_t x = t + kCompactThreads * q, l = x % kGroupRows, s = lo + x / kGroupRows;
       const bool ok = s <= hi && s < len[l];
-      const uint32_t off =
-          ok ? ((((s >> gs) - b0) * (uint32_t)kTileRows + grp * kGroupRows + l) * (3u << gs) +
-                (s & gmask) * 3) * 4u
-             : 0x80000000u;
+      const uint32_t off = ok ? tile_slot_float(s, b0, grp * kGroupRows + l, gs) * 4u : 0x80000000u;
       r[q] = __builtin_amdgcn_raw_buffer_load_b96(rs, (int)off, 0, 0);
     }
   };

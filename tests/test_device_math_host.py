@@ -8,6 +8,10 @@
   launches (queue position -> lattice index) is a bijection that visits
   bricks in C order, one contiguous run per brick, points in C order inside,
   also on lattices whose sides are not multiples of the brick.
+* tests/native/slot_layout_check.cpp: the record-slot layout for slot shifts
+  0 and 3 -- shift 0 is the one-slot-run formula, every slot of a region is
+  distinct and inside it, staged blocks start 16-byte aligned, and the
+  compaction's chunk offsets land on the same floats as slot_float.
 * tests/native/div_check.cpp: the reciprocal division used on verified axes
   must equal the correctly rounded x / h bit for bit (all patterns around the
   switch to plain division and over [h/2, 2h], plus a sample of the whole
@@ -55,3 +59,9 @@ def test_reciprocal_division_is_correctly_rounded(tmp_path):
 def test_lattice_brick_order_is_a_bijection(tmp_path):
     words = build_and_run(tmp_path, "brick_order_check").split()
     assert int(words[1]) == 27 and int(words[3]) > 100000000 and int(words[5]) == 0, words
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
+def test_slot_layout_is_consistent(tmp_path):
+    words = build_and_run(tmp_path, "slot_layout_check").split()
+    assert int(words[1]) == 80 and int(words[3]) > 400000 and int(words[5]) == 0, words
