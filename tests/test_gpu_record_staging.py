@@ -89,3 +89,28 @@ def test_staged_batch_equals_direct_jobs(ctx, vortex):
     finally:
         for r in batch:
             r.free()
+
+
+def test_staged_batch_pass_replays_equal_eager(ctx, vortex):
+    """The staged kernel instance inside a captured batch pass (45000 rows):
+    every replay equals the eager batch bit for bit."""
+    field, fld = vortex
+    say("case staged batch pass, 2 replays")
+    jobs = []
+    for k in range(3):
+        seeds, rem = uneven(15000, 640 + k)
+        jobs.append(dict(field=fld, seeds=seeds, remaining=rem, uniform_remaining=0,
+                         particles_per_round=None, step_size=f32(0.25), record=True,
+                         use_64_bit=False))
+    with ctx.batch_pass(jobs) as bp:
+        for rep in range(2):
+            eager = ctx.advect_batch(jobs)
+            bp.launch()
+            got = bp.results()
+            try:
+                for g, e in zip(got, eager):
+                    same(result_arrays(g), result_arrays(e))
+            finally:
+                for r in got + eager:
+                    r.free()
+            say(f"  replay {rep} matches")
