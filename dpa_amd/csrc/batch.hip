@@ -28,9 +28,16 @@ struct JobPlan {
   uint64_t base = 0;    // first particle in the group's concatenated arrays
   uint64_t count = 0;
   int use64 = 0;
-  std::vector<uint64_t> rem;
+  uint64_t rem_sum = 0;   // the job's starting iterations, summed
   std::vector<RoundPlan> rounds;
 };
+
+// The uniform starting iterations of jobs without a per-particle array.
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
 
 void validate(const dpah_context* ctx, const dpah_job& j) {
   DPAH_REQUIRE(j.field, "job field is null");
@@ -69,9 +76,6 @@ struct GroupRun {
   std::vector<int> seg_use64;
   uint64_t rows_total = 0, sums_total = 0;
   AdvectLaunch L{};
-  // host staging of the uploads (alive until the stream has read them)
-  std::vector<float> h_seeds;
-  std::vector<uint64_t> h_rem;
   DeviceBuffer d_seeds, d_rem, d_pos, d_rem_out, d_status, d_steps, d_special, d_segs, d_slots;
   DeviceBuffer d_vcount, d_icount, d_sums;
 
@@ -92,28 +96,38 @@ struct GroupRun {
       p.base = N;
       p.count = j.count;
       p.use64 = j.params.use_64_bit_indices ? 1 : 0;
-      p.rem.resize(j.count);
-      for (uint64_t i = 0; i < j.count; ++i) {
-        p.rem[i] = j.remaining ? j.remaining[i] : j.uniform_remaining;
-        DPAH_REQUIRE(p.rem[i] <= (1ull << 32) - 3,
+      // the caller's arrays are read here and by the uploads below only
+      const uint64_t* rin = j.remaining;
+      if (rin) {
+        for (uint64_t i = 0; i < j.count; ++i) {
+          DPAH_REQUIRE(rin[i] <= (1ull << 32) - 3, "remaining iterations must be below 2^32 - 2");
+          p.rem_sum += rin[i];
+        }
+      } else if (j.count) {
+        DPAH_REQUIRE(j.uniform_remaining <= (1ull << 32) - 3,
                      "remaining iterations must be below 2^32 - 2");
+        p.rem_sum = j.uniform_remaining * j.count;
       }
+      auto max_rem = [&](uint64_t first, uint64_t count) -> uint64_t {
+        if (!rin) return count ? j.uniform_remaining : 0;
+        uint64_t mx = 0;
+        for (uint64_t i = 0; i < count; ++i) mx = std::max(mx, rin[first + i]);
+        return mx;
+      };
       if (record) {
         uint64_t active = j.count;
         do {   // pipeline.cpp:100-146 round order, tail first
           RoundPlan r;
           r.count = std::min(j.params.particles_per_round, active);
           r.first = active - r.count;
-          uint64_t mx = 0;
-          for (uint64_t i = 0; i < r.count; ++i) mx = std::max(mx, p.rem[r.first + i]);
-          r.stride = mx + 2;
+          r.stride = max_rem(r.first, r.count) + 2;
           p.rounds.push_back(r);
           active = r.first;
         } while (active > 0);
       } else if (j.count) {
         RoundPlan r;
         r.count = j.count;
-        for (uint64_t i = 0; i < j.count; ++i) r.stride = std::max(r.stride, p.rem[i]);
+        r.stride = max_rem(0, j.count);
         p.rounds.push_back(r);   // ordering key only; no slots
       }
       N += j.count;
@@ -158,13 +172,6 @@ struct GroupRun {
     }
 
     const uint64_t n1 = std::max<uint64_t>(N, 1);
-    h_seeds.assign(3 * n1, 0.0f);
-    h_rem.assign(n1, 0);
-    for (const JobPlan& p : plans) {
-      const dpah_job& j = jobs[p.job];
-      if (j.count) std::memcpy(h_seeds.data() + 3 * p.base, j.seeds, 3 * j.count * sizeof(float));
-      std::copy(p.rem.begin(), p.rem.end(), h_rem.begin() + p.base);
-    }
     d_seeds = DeviceBuffer(ctx, 3 * n1 * sizeof(float));
     d_rem = DeviceBuffer(ctx, n1 * sizeof(uint64_t));
     d_pos = DeviceBuffer(ctx, 3 * n1 * sizeof(float));
@@ -179,10 +186,22 @@ struct GroupRun {
       d_icount = DeviceBuffer(ctx, rows_total * sizeof(uint64_t));
       d_sums = DeviceBuffer(ctx, sums_total * sizeof(uint64_t));
     }
-    DPAH_HIP(hipMemcpyAsync(d_seeds.ptr, h_seeds.data(), 3 * N * sizeof(float),
-                            hipMemcpyHostToDevice, ctx->stream));
-    DPAH_HIP(hipMemcpyAsync(d_rem.ptr, h_rem.data(), N * sizeof(uint64_t),
-                            hipMemcpyHostToDevice, ctx->stream));
+    // each job's seeds and iterations straight from the caller's arrays
+    for (const JobPlan& p : plans) {
+      const dpah_job& j = jobs[p.job];
+      if (!j.count) continue;
+      DPAH_HIP(hipMemcpyAsync(d_seeds.as<float>() + 3 * p.base, j.seeds,
+                              3 * j.count * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+      if (j.remaining) {
+        DPAH_HIP(hipMemcpyAsync(d_rem.as<uint64_t>() + p.base, j.remaining,
+                                j.count * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+      } else {
+        const unsigned grid = (unsigned)std::min<uint64_t>((j.count + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(256), 0, ctx->stream,
+                           d_rem.as<uint64_t>() + p.base, j.count, j.uniform_remaining);
+        DPAH_HIP(hipGetLastError());
+      }
+    }
     if (!segs.empty())
       DPAH_HIP(hipMemcpyAsync(d_segs.ptr, segs.data(), segs.size() * sizeof(Segment),
                               hipMemcpyHostToDevice, ctx->stream));
@@ -222,14 +241,15 @@ struct GroupRun {
     if (record && rows_total)
       DPAH_HIP(hipMemcpyAsync(h_sums.data(), d_sums.ptr, sums_total * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<float> h_pos(3 * n1);
-    std::vector<uint64_t> h_rem_out(n1);
-    std::vector<uint8_t> h_status(n1);
-    DPAH_HIP(hipMemcpyAsync(h_pos.data(), d_pos.ptr, 3 * N * sizeof(float),
+    // staging without zero-filling (every element is overwritten)
+    std::unique_ptr<float[]> h_pos(new float[3 * n1]);
+    std::unique_ptr<uint64_t[]> h_rem_out(new uint64_t[n1]);
+    std::unique_ptr<uint8_t[]> h_status(new uint8_t[n1]);
+    DPAH_HIP(hipMemcpyAsync(h_pos.get(), d_pos.ptr, 3 * N * sizeof(float),
                             hipMemcpyDeviceToHost, ctx->stream));
-    DPAH_HIP(hipMemcpyAsync(h_rem_out.data(), d_rem_out.ptr, N * sizeof(uint64_t),
+    DPAH_HIP(hipMemcpyAsync(h_rem_out.get(), d_rem_out.ptr, N * sizeof(uint64_t),
                             hipMemcpyDeviceToHost, ctx->stream));
-    DPAH_HIP(hipMemcpyAsync(h_status.data(), d_status.ptr, N, hipMemcpyDeviceToHost,
+    DPAH_HIP(hipMemcpyAsync(h_status.get(), d_status.ptr, N, hipMemcpyDeviceToHost,
                             ctx->stream));
     ctx->sync();
 
@@ -237,12 +257,12 @@ struct GroupRun {
       auto res = std::make_unique<dpah_result>();
       res->ctx = ctx;
       res->use64 = p.use64;
-      res->final_pos.assign(h_pos.begin() + 3 * p.base, h_pos.begin() + 3 * (p.base + p.count));
-      res->final_rem.assign(h_rem_out.begin() + p.base, h_rem_out.begin() + p.base + p.count);
-      res->final_status.assign(h_status.begin() + p.base, h_status.begin() + p.base + p.count);
-      uint64_t steps = 0;
-      for (uint64_t i = 0; i < p.count; ++i) steps += p.rem[i] - res->final_rem[i];
-      res->total_steps = steps;
+      res->final_pos.assign(h_pos.get() + 3 * p.base, h_pos.get() + 3 * (p.base + p.count));
+      res->final_rem.assign(h_rem_out.get() + p.base, h_rem_out.get() + p.base + p.count);
+      res->final_status.assign(h_status.get() + p.base, h_status.get() + p.base + p.count);
+      uint64_t left = 0;
+      for (uint64_t i = 0; i < p.count; ++i) left += res->final_rem[i];
+      res->total_steps = p.rem_sum - left;
       if (record) {
         for (const RoundPlan& r : p.rounds) {
           Round round;
