@@ -64,20 +64,26 @@ def host_threads() -> int:
     return max(1, min(16, len(os.sched_getaffinity(0))))
 
 
-def _timed_sample(fn, fld, seeds, iters, dt, threads, target_s):
-    """Runs fn on the first seeds, sized from a 2048-seed probe so that the
-    timed run takes about target_s seconds."""
+def _timed_sample(fn, fld, seeds, iters, dt, threads, target_s, runs=3):
+    """Runs fn on the first seeds, sized from a 2048-seed probe (which also
+    warms the caches) so that each of `runs` timed runs takes about
+    target_s / runs seconds; returns the median run's rate."""
     probe = seeds[: 2048]
     t = time.perf_counter()
     r = fn(fld, probe, iters, dt, threads=threads)
     el = time.perf_counter() - t
     per_seed = max(r["total_steps"] / len(probe), 1e-9)
     rate = r["total_steps"] / max(el, 1e-9)
-    count = int(min(len(seeds), max(2048, rate * target_s / per_seed)))
-    t = time.perf_counter()
-    r = fn(fld, seeds[: count], iters, dt, threads=threads)
-    el = time.perf_counter() - t
-    return r["total_steps"] / el, count, r["total_steps"], el
+    count = int(min(len(seeds), max(2048, rate * target_s / runs / per_seed)))
+    timed = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        r = fn(fld, seeds[: count], iters, dt, threads=threads)
+        el = time.perf_counter() - t
+        timed.append((r["total_steps"] / el, r["total_steps"], el))
+    timed.sort()
+    v, steps, el = timed[len(timed) // 2]
+    return v, count, steps, el
 
 
 def cpu_baseline_endpoints(field, offset, spacing, seeds, iters, dt, target_s: float,
@@ -97,9 +103,10 @@ def cpu_baseline_endpoints(field, offset, spacing, seeds, iters, dt, target_s: f
     v1, c1, _, _ = _timed_sample(po.advect_endpoints_dpa_structured, fld, seeds, iters, dt, 1,
                                  target_s * 0.25)
     return {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"first {count} of the {what} seeds, all {iters} iterations, same field "
-                      f"({steps} particle-steps in {el:.2f} s; host {cpu_model()}); "
-                      "dpa-structured interpolation (heap corner list, recursive corner walk)",
+            "sample": f"first {count} of the {what} seeds, all {iters} iterations, same field, "
+                      f"median of 3 warm runs ({steps} particle-steps in {el:.2f} s; host "
+                      f"{cpu_model()}); dpa-structured interpolation (heap corner list, "
+                      "recursive corner walk)",
             "plain_port": {"value": vp, "cores": threads, "seeds": cp},
             "one_thread": {"value": v1, "cores": 1, "seeds": c1}}
 
