@@ -108,3 +108,46 @@ def test_batch_pass_replays_match_eager_batch_and_oracle(ctx):
                 say(f"  replay {rep} matches")
     finally:
         fld.release()
+
+
+def test_batch_pass_time_variant_both_integrators(ctx):
+    """A captured batch over a time-variant field (pathlines): recorded and
+    endpoint jobs, dpa runge_kutta_4 and classic RK4, start times apart, so
+    the graph holds several groups of unsteady launches.  Two replays, each
+    equal to the eager batch, and the oracle's rounds."""
+    from test_gpu_batch import result_arrays, same
+    from test_gpu_parity import compare_advect
+    say("case batch pass, time-variant field, 6 jobs")
+    field = W.unsteady_vortex_field(32, time_steps=4, count=8)
+    fld = ctx.field_unsteady(field, (0, 0, 0), (1, 1, 1), 0.0, 1.0)
+    ofld = po.Field(field, (0, 0, 0), (1, 1, 1), 0.0, 1.0)
+    jobs = []
+    for k in range(6):
+        seeds = W.uniform_seeds(150 + 70 * k, (0, 0, 0), (31, 31, 31), seed_offset=900 + k)
+        jobs.append(dict(field=fld, seeds=seeds, remaining=None, uniform_remaining=60 + 15 * k,
+                         particles_per_round=None if k % 2 else 100, step_size=f32(0.02),
+                         start_time=float(f32(0.1 * k)), record=k != 3,
+                         integrator=po.RK4 if k >= 4 else po.DPA_RK4, use_64_bit=False))
+    try:
+        with ctx.batch_pass(jobs) as bp:
+            for rep in range(2):
+                eager = ctx.advect_batch(jobs)
+                bp.launch()
+                got = bp.results()
+                try:
+                    for k, (g, e) in enumerate(zip(got, eager)):
+                        same(result_arrays(g), result_arrays(e))
+                        if rep == 0:
+                            j = jobs[k]
+                            n = len(j["seeds"])
+                            ora = po.run_rounds(ofld, j["seeds"], j["uniform_remaining"],
+                                                j["particles_per_round"] or n, j["step_size"],
+                                                t0=j["start_time"], integrator=j["integrator"],
+                                                record=j["record"])
+                            compare_advect(g, ora, n)
+                finally:
+                    for r in got + eager:
+                        r.free()
+                say(f"  replay {rep} matches")
+    finally:
+        fld.release()
