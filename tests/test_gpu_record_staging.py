@@ -114,3 +114,27 @@ def test_staged_batch_pass_replays_equal_eager(ctx, vortex):
                 for r in got + eager:
                     r.free()
             say(f"  replay {rep} matches")
+
+
+@pytest.mark.parametrize("integrator", [po.DPA_RK4, po.RK4])
+def test_staged_pathlines_match_oracle(ctx, integrator):
+    """Staged slot blocks on a time-variant field (pathlines; the sample
+    lerps time first), one round of 36000 rows, both integrators."""
+    say(f"case staged pathlines, integrator {integrator}")
+    field = W.unsteady_vortex_field(32, time_steps=4, count=8)
+    fld = ctx.field_unsteady(field, (0, 0, 0), (1, 1, 1), 0.0, 1.0)
+    try:
+        rng = np.random.default_rng(720 + integrator)
+        seeds = W.uniform_seeds(36000, (0, 0, 0), (31, 31, 31), seed_offset=720 + integrator)
+        rem = rng.integers(0, 45, len(seeds)).astype(np.uint64)
+        dt, t0 = f32(0.02), 0.25
+        ora = po.run_rounds(po.Field(field, (0, 0, 0), (1, 1, 1), 0.0, 1.0), seeds, rem, 36000,
+                            dt, t0, integrator)
+        res = ctx.advect(fld, seeds, remaining=rem, particles_per_round=36000, step_size=dt,
+                         start_time=t0, integrator=integrator)
+        try:
+            compare_advect(res, ora, len(seeds))
+        finally:
+            res.free()
+    finally:
+        fld.release()
