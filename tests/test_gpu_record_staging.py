@@ -138,3 +138,33 @@ def test_staged_pathlines_match_oracle(ctx, integrator):
             res.free()
     finally:
         fld.release()
+
+
+@pytest.mark.parametrize("fastdiv", [1, 0])
+def test_staged_round_other_division_modes(ctx, fastdiv):
+    """Staged slot blocks in the verified-reciprocal (fastdiv 1) and the
+    dividing (fastdiv 0) kernel instances: an anisotropic, offset grid whose
+    spacings are not all powers of two."""
+    say(f"case staged round, spacing (0.75, 0.1, 0.5), fastdiv={fastdiv}")
+    offset, spacing = (0.25, -0.5, 1.0), (0.75, float(f32(0.1)), 0.5)
+    data = W.vortex_field(36, count=10)
+    before = ctx.tuning()
+    ctx.set_tuning(fastdiv=fastdiv)
+    try:
+        fld = ctx.field(data, offset, spacing)
+        try:
+            lo = np.array(offset, f32)
+            hi = lo + np.array(spacing, f32) * 35
+            seeds = W.uniform_seeds(36000, lo, hi, seed_offset=730 + fastdiv)
+            rem = np.random.default_rng(731).integers(0, 80, len(seeds)).astype(np.uint64)
+            dt = f32(0.002)
+            ora = po.run_rounds(po.Field(data, offset, spacing), seeds, rem, 36000, dt)
+            res = ctx.advect(fld, seeds, remaining=rem, particles_per_round=36000, step_size=dt)
+            try:
+                compare_advect(res, ora, len(seeds))
+            finally:
+                res.free()
+        finally:
+            fld.release()
+    finally:
+        ctx.set_tuning(**before)
