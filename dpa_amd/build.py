@@ -103,29 +103,9 @@ def build_oracle() -> Path:
     return ROOT / "oracle" / "libdpa_oracle.so"
 
 
-def build_copy_probe() -> Path:
-    """The copy kernel that calibrates the HBM byte counters (FETCH_SIZE,
-    WRITE_SIZE) for the traffic figures of the roofline."""
-    out = CLI.parent / "copy_probe"
-    CLI.parent.mkdir(exist_ok=True)
-    _run([HIPCC, *HIP_FLAGS, str(CSRC / "copy_probe.hip"), "-o", str(out)])
-    return out
-
-
-def build_gather_probe() -> Path:
-    """Random cell-record reads on their own (128-B padded and 96-B packed
-    records): the practical ceiling of the advect kernel's cell reloads."""
-    out = CLI.parent / "gather_probe"
-    CLI.parent.mkdir(exist_ok=True)
-    _run([HIPCC, *HIP_FLAGS, str(CSRC / "gather_probe.hip"), "-o", str(out)])
-    return out
-
-
 def build_all() -> None:
     build_library()
     build_cli()
-    build_copy_probe()
-    build_gather_probe()
     build_oracle()
 
 

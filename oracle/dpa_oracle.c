@@ -23,6 +23,13 @@
  *                            colours 0; right-angle zig-zag pi/2 with the
  *                            vertex-1 quirk; dpa's remove semantics on planted
  *                            sentinels.
+ *   time-variant sampling    (tests/test_oracle_unsteady.py) fields multilinear
+ *                            in (x0, x1, x2, t); a slice-index field; the 4-D
+ *                            sample equals the 3-D sample of the nodes blended
+ *                            in time first, bit for bit; time-axis contains;
+ *                            exit iteration by brute force; a linear time ramp
+ *                            against its closed-form sum / integral.  Each
+ *                            planted slip (DPO_PLANT) fails its pins.
  *   lattice seeds            count and C order closed form.
  *   ftle                     diagonal linear field closed form, general linear
  *                            field vs numpy float64 SVD, eigen special cases vs
@@ -35,6 +42,20 @@
  * not reproducible without Eigen.
  */
 #include "dpa_oracle.h"
+
+/* DPO_PLANT: test builds only (tests/test_oracle_unsteady.py compiles this
+ * file with -DDPO_PLANT=k into a temporary directory).  Each value plants one
+ * known mistake in the time-variant (4-D) path, so that the pins of that path
+ * can be shown to fail on it.  The default build (oracle/Makefile) defines
+ * nothing and is 0: no plant.
+ *   1  time lerped last (time the outermost corner bit, not the innermost)
+ *   2  time slice off by one (the next interval, clamped to the last)
+ *   3  time weight swapped (1 - w instead of w)
+ *   4  time of iteration i taken as (i + 1) * dt
+ *   5  contains admits the last time node (s <= nt - 1) */
+#ifndef DPO_PLANT
+#define DPO_PLANT 0
+#endif
 
 #include <math.h>
 #include <stdlib.h>
@@ -63,6 +84,9 @@ int dpo_contains(const dpo_grid* g, const float p[3], float t) {
     const float d = x - o;
     const float s = floorf(d / h);
     if (!(s >= 0.0f)) return 0;             /* negative, or NaN */
+#if DPO_PLANT == 5
+    if (i == 3) n += 1;
+#endif
     if (s >= (float)(n - 1)) return 0;      /* also +inf */
   }
   return 1;
@@ -88,17 +112,26 @@ void dpo_interpolate(const dpo_grid* g, const float p[3], float t, float out[3])
     w[i] = fmodf(d, h) / h;
     c[i] = (int64_t)floorf(d / h);
   }
+  /* axis[j]: the grid axis of corner bit j (bit dims-1-j of the list index) */
+  int axis[4] = {0, 1, 2, 3};
+#if DPO_PLANT == 1
+  if (dims == 4) { axis[0] = 3; axis[1] = 0; axis[2] = 1; axis[3] = 2; }
+#elif DPO_PLANT == 2
+  if (dims == 4 && c[3] + 2 < g->nt) c[3] += 1;
+#elif DPO_PLANT == 3
+  if (dims == 4) w[3] = 1.0f - w[3];
+#endif
   float L[16][3];
   const int corners = 1 << dims;
   for (int k = 0; k < corners; ++k) {
     int64_t idx[4] = {c[0], c[1], c[2], c[3]};
-    for (int i = 0; i < dims; ++i)
-      idx[i] += (k >> (dims - 1 - i)) & 1;
+    for (int j = 0; j < dims; ++j)
+      idx[axis[j]] += (k >> (dims - 1 - j)) & 1;
     const float* v = grid_node(g, idx[0], idx[1], idx[2], dims == 4 ? idx[3] : 0);
     L[k][0] = v[0]; L[k][1] = v[1]; L[k][2] = v[2];
   }
   for (int i = dims - 1; i >= 0; --i) {
-    const float wi = w[i];
+    const float wi = w[axis[i]];
     const float om = 1.0f - wi;
     for (int j = 0; j < (1 << i); ++j)
       for (int e = 0; e < 3; ++e) {
@@ -188,7 +221,11 @@ uint64_t dpo_advect_particle(const dpo_grid* g, float p[3], uint64_t* remaining,
   uint8_t st = DPO_ACTIVE;
   if (rec) { rec[0] = p[0]; rec[1] = p[1]; rec[2] = p[2]; }
   for (; *remaining > 0; ++it, --*remaining) {
+#if DPO_PLANT == 4
+    const float ft = (float)(it + 1) * dt;
+#else
     const float ft = (float)it * dt;     /* iteration_index * step_size_ (375) */
+#endif
     const float t  = t0 + ft;
     if (!dpo_contains(g, p, t)) { st = DPO_EXITED; break; }
     float v[3];

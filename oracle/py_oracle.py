@@ -5,6 +5,7 @@ cpu_baseline leg of bench.py, as the checker.  The product never imports it.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import subprocess
 from pathlib import Path
@@ -35,7 +36,25 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
-    lib = C.CDLL(str(LIB_PATH))
+    _lib = bind(LIB_PATH)
+    return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Routes every function of this module through another build of the
+    oracle (a planted-mistake variant of a pin test) while the block runs."""
+    global _lib
+    saved = load()
+    _lib = bind(path)
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def bind(path) -> C.CDLL:
+    lib = C.CDLL(str(path))
     G = C.POINTER(Grid)
     sig = {
         "dpo_contains": (C.c_int, [G, _vp, C.c_float]),
@@ -74,7 +93,6 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
     return lib
 
 
