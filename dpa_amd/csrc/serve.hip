@@ -365,6 +365,10 @@ bool dpah_server::launch_ready() {
       try {
         if (!r.set_up) setup(r, s);
       } catch (const Error& e) {
+        // setup may have queued copies into the request's buffers on s before
+        // it failed: they finish before the buffers go back to the shared
+        // cache, which the other streams draw from
+        (void)hipStreamSynchronize(s);
         release_request(rp, kFailed, e.code, e.what());
         continue;
       }
