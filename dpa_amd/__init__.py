@@ -85,6 +85,7 @@ class _ServerStats(C.Structure):
 
 REQUEST_ADVECT, REQUEST_FTLE_AUX = 0, 1
 SERVE_BATCHED, SERVE_PER_REQUEST, SERVE_SERIAL = 0, 1, 2
+MAX_NOALLOC_PARTICLES = 1 << 21     # DPAH_NOALLOC_MAX_PARTICLES
 TICKET_PENDING, TICKET_RUNNING, TICKET_DONE, TICKET_FAILED, TICKET_CANCELLED = range(5)
 
 _lib = None
@@ -126,6 +127,9 @@ def load_library() -> C.CDLL:
         "dpah_advect_endpoints_device": (C.c_int, [vp, vp, vp, vp, C.c_uint64, C.c_uint64,
                                                    C.c_float, C.c_float, C.c_int, vp, vp, vp,
                                                    _u64p]),
+        "dpah_advect_endpoints_device_noalloc": (C.c_int, [vp, vp, vp, vp, C.c_uint64,
+                                                           C.c_uint64, C.c_float, C.c_float,
+                                                           C.c_int, vp, vp, vp, _u64p]),
         "dpah_ftle_estimate_device": (C.c_int, [vp, _u64p, _f32p, _f32p, _f32p, C.c_uint64,
                                                 C.c_float, vp, vp, C.c_uint64, vp, _u64p]),
         "dpah_ftle_lattice": (C.c_int, [vp, vp, _f32p, C.c_uint64, C.c_float, C.c_int, vp,
@@ -322,6 +326,25 @@ class Context:
                                 wait: bool = True) -> int | None:
         steps = C.c_uint64()
         _check(self.lib.dpah_advect_endpoints_device(
+            self.h, field.h, C.c_void_p(d_seeds), C.c_void_p(d_remaining) if d_remaining else None,
+            int(uniform_remaining), int(count), float(step_size), float(start_time), int(integrator),
+            C.c_void_p(d_pos_out), C.c_void_p(d_rem_out) if d_rem_out else None,
+            C.c_void_p(d_status_out) if d_status_out else None,
+            C.byref(steps) if wait else None))
+        return steps.value if wait else None
+
+    def advect_endpoints_device_noalloc(self, field: "Field", d_seeds: int, count: int,
+                                        d_pos_out: int, d_remaining: int | None = None,
+                                        uniform_remaining: int = 0, step_size: float = 0.1,
+                                        start_time: float = 0.0,
+                                        integrator: int = INTEGRATOR_DPA_RK4,
+                                        d_rem_out: int | None = None,
+                                        d_status_out: int | None = None,
+                                        wait: bool = True) -> int | None:
+        """advect_endpoints_device with no device allocation: its scratch is the
+        library's fixed per-GPU array (at most MAX_NOALLOC_PARTICLES particles)."""
+        steps = C.c_uint64()
+        _check(self.lib.dpah_advect_endpoints_device_noalloc(
             self.h, field.h, C.c_void_p(d_seeds), C.c_void_p(d_remaining) if d_remaining else None,
             int(uniform_remaining), int(count), float(step_size), float(start_time), int(integrator),
             C.c_void_p(d_pos_out), C.c_void_p(d_rem_out) if d_rem_out else None,

@@ -20,7 +20,7 @@ BUILD = HERE / "_build"
 LIB = HERE / "libdpa_hip.so"
 
 HIP_SOURCES = ["advect.hip", "compact.hip", "ftle.hip", "batch.hip", "serve.hip", "tune.hip",
-               "graph.hip", "capi.hip"]
+               "graph.hip", "noalloc.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DPAH_OFFLOAD_ARCH", "gfx950")

@@ -211,8 +211,9 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   a.n = L.n;
   a.t0 = L.t0;
   a.coef = make_coef(L.dt);
-  a.queue = ctx->d_counters + kQueueHeads;
-  a.total_steps = ctx->d_counters + 1;
+  unsigned long long* const counters = L.counters ? L.counters : ctx->d_counters;
+  a.queue = counters + kQueueHeads;
+  a.total_steps = counters + 1;
   a.order = nullptr;
   set_parts(a, 1);
   // auto: 8 for endpoint launches of seed arrays (keeps lanes busy); 64 for
@@ -223,7 +224,7 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   DPAH_REQUIRE(L.slot_shift == 0 || (L.slot_shift == kStagedShift && L.persistent),
                "staged slot blocks need a persistent launch");
   a.slot_shift = L.slots ? L.slot_shift : 0u;
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;
   if (ordered_launch(ctx, T, L)) {

@@ -176,6 +176,7 @@ DPAH_EXPORT int dpah_create(int device, const dpah_options* options, dpah_contex
     DPAH_HIP(hipGetDeviceCount(&count));
     DPAH_REQUIRE(device >= 0 && device < count, "device index out of range");
     DPAH_HIP(hipSetDevice(device));
+    dpah::noalloc_setup(device);
     auto* ctx = new dpah_context();
     try {
       ctx->device = device;

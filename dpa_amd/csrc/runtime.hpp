@@ -269,6 +269,8 @@ struct AdvectLaunch {
   int integrator;
   bool persistent;
   const OrderScratch* order_scratch = nullptr;   // null: temporaries from the allocator
+  // step counter and queue heads (kCounterCount words); null: the context's
+  unsigned long long* counters = nullptr;
 };
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
 // Brick shape of the queue ordering of a field (order_kernels.hpp).
@@ -286,6 +288,9 @@ bool use_cells(const Tuning& T, bool scattered);
 std::string launch_key(const FieldView& f, int integrator, int seed_kind, bool record);
 // Loads the context's autotune cache file (entries of this device) into ctx->tuned.
 void load_autotune_cache(dpah_context* ctx);
+// Looks up, once per process and device, the address of the allocation-free
+// entry point's scratch array on `device` (which must be current).
+void noalloc_setup(int device);
 
 // Copies a result's device arrays to host memory and releases them (the
 // per-request serving form returns results this way).

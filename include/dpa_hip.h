@@ -162,6 +162,26 @@ int dpah_advect_endpoints_device(dpah_context* ctx, const dpah_field* field, con
                                  int integrator, float* positions_out, uint64_t* remaining_out,
                                  uint8_t* status_out, uint64_t* total_steps);
 
+/* Allocation-free form of dpah_advect_endpoints_device.  No counterpart in dpa
+ * (whose CPU loop allocates freely): it is for callers that cannot allocate
+ * device memory -- code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Same arguments and results, bit for bit; it calls no
+ * hipMalloc / hipFree and launches untimed (kernel_timing records nothing for
+ * it).  Its scratch (step counter, queue heads, queue-ordering keys, order
+ * table and histogram) is one fixed device array of DPAH_NOALLOC_SCRATCH_BYTES
+ * per GPU, held from dpah_create until the process ends, so count is limited
+ * to DPAH_NOALLOC_MAX_PARTICLES (larger: DPAH_ERR_INVALID).  Every call on a
+ * GPU shares that GPU's array: calls from one context are stream-ordered, but
+ * calls that overlap on one GPU from two contexts race. */
+#define DPAH_NOALLOC_MAX_PARTICLES (1ull << 21)
+#define DPAH_NOALLOC_SCRATCH_BYTES (18ull << 20)
+int dpah_advect_endpoints_device_noalloc(dpah_context* ctx, const dpah_field* field,
+                                         const float* seeds, const uint64_t* remaining,
+                                         uint64_t uniform_remaining, uint64_t count,
+                                         float step_size, float start_time, int integrator,
+                                         float* positions_out, uint64_t* remaining_out,
+                                         uint8_t* status_out, uint64_t* total_steps);
+
 /* ftle_estimator::estimate (ftle_estimator.cpp:7-65) on device data: the final
  * positions are scattered onto the strided lattice of the original positions
  * (shape size_t(n_i / stride_i), spacing h_i * stride_i, cell of
