@@ -32,10 +32,17 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "RK4 particle-steps/s on 1 GPU + p50/p99 query latency vs per-request launch"
-BYTES_PER_STEP = 96          # 8 corners x 12 B gathered per dpa-mode step
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = ROOT / "profiles" / "advect_persistent_vortex256_traffic.txt"
-
+# Roofline bytes of an endpoint launch: the register cell cache serves every
+# step that stays in its cell, so what the launch must bring past the L2 is one
+# 128-byte cell record per crossing into a new cell (96 bytes, a cell's eight
+# nodes, for launches that read the node layout), counted on the device, plus
+# each particle's seed read and final position, remaining count and status.
+RECORD_BYTES = 128
+NODE_GROUP_BYTES = 96
+PARTICLE_IO_BYTES = 12 + 12 + 8 + 1
+CORNER_BYTES_PER_STEP = 96      # eight 12-byte corners per dpa-mode step, cache-served included
+COPY_BYTES = 1 << 30            # device-to-device copy that calibrates the byte counters
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -51,17 +58,20 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def read_traffic():
-    if not TRAFFIC_FILE.exists():
-        return None
-    for line in TRAFFIC_FILE.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
-
-
 def host_threads() -> int:
-    return max(1, min(16, len(os.sched_getaffinity(0))))
+    """Threads of the CPU baseline: OMP_NUM_THREADS when it is set (the GPU
+    boxes allot each GPU's process its share of the host's cores through it),
+    otherwise every CPU this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def host_info() -> dict:
+    return {"cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def _timed_sample(fn, fld, seeds, iters, dt, threads, target_s, runs=3):
@@ -103,6 +113,7 @@ def cpu_baseline_endpoints(field, offset, spacing, seeds, iters, dt, target_s: f
     v1, c1, _, _ = _timed_sample(po.advect_endpoints_dpa_structured, fld, seeds, iters, dt, 1,
                                  target_s * 0.25)
     return {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "host": host_info(),
             "sample": f"first {count} of the {what} seeds, all {iters} iterations, same field, "
                       f"median of 3 warm runs ({steps} particle-steps in {el:.2f} s; host "
                       f"{cpu_model()}); dpa-structured interpolation (heap corner list, "
@@ -111,18 +122,162 @@ def cpu_baseline_endpoints(field, offset, spacing, seeds, iters, dt, target_s: f
             "one_thread": {"value": v1, "cores": 1, "seeds": c1}}
 
 
-def roofline(steps_per_launch, avg_ms, bytes_per_step=BYTES_PER_STEP, traffic=None):
-    """Algorithmic gather bytes (bytes_per_step x steps) over the launch time.
-    Corners served from registers or the L2/MALL count too, so `frac` can exceed
-    1; `traffic_gbs` is the measured HBM-side traffic over the same time."""
-    achieved = steps_per_launch * bytes_per_step / (avg_ms * 1e-3) / 1e9
+def roofline(avg_ms, algorithmic_bytes, steps, traffic=None):
+    """HBM roofline of one launch: the bytes it must bring past the L2 (one
+    cell record per cell crossing plus the particles' own I/O, counted on the
+    device) over its average duration.  `traffic` is what the byte counters
+    measured past the L2 for the same launch (HBM and the Infinity Cache
+    behind it); `corner_gbs` is the per-step corner figure (96 B per step,
+    corners the register cache serves included), which is not a roofline."""
+    achieved = algorithmic_bytes / (avg_ms * 1e-3) / 1e9
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_avg_ms": avg_ms,
-           "bytes_per_step": bytes_per_step}
+           "algorithmic_bytes": algorithmic_bytes, "bytes_per_step": algorithmic_bytes / steps,
+           "corner_gbs": steps * CORNER_BYTES_PER_STEP / (avg_ms * 1e-3) / 1e9}
     if traffic:
         out["traffic_gbs"] = traffic / (avg_ms * 1e-3) / 1e9
         out["traffic_frac"] = out["traffic_gbs"] / HBM_PEAK_GBS
     return out
+
+
+def counted_bytes(ctx, one_pass, n, steps):
+    """One more pass with the cell-load counting instance: the launch's
+    algorithmic bytes (its results must not change)."""
+    ctx.set_count_cell_loads(True)
+    try:
+        s = one_pass(True)
+        rec, node = ctx.cell_loads()
+    finally:
+        ctx.set_count_cell_loads(False)
+    if s != steps:
+        raise RuntimeError(f"the load-counting pass took {s} steps, not {steps}")
+    return rec * RECORD_BYTES + node * NODE_GROUP_BYTES + n * PARTICLE_IO_BYTES, rec, node
+
+
+def endpoint_loads(ctx, fld, sets, steps):
+    """Cell loads of endpoint passes over (seeds, iterations, dt) sets, run by
+    the counting instance; their particle-steps must add up to `steps`."""
+    total = 0
+    ctx.set_count_cell_loads(True)
+    try:
+        for seeds, iters, dt in sets:
+            n = len(seeds)
+            d_s = ctx.to_device(np.ascontiguousarray(seeds, np.float32))
+            d_p = ctx.alloc(12 * n)
+            try:
+                total += ctx.advect_endpoints_device(fld, d_s, n, d_p, uniform_remaining=iters,
+                                                     step_size=dt, wait=True)
+            finally:
+                ctx.free(d_s)
+                ctx.free(d_p)
+        rec, node = ctx.cell_loads()
+    finally:
+        ctx.set_count_cell_loads(False)
+    if total != steps:
+        raise RuntimeError(f"counted endpoint passes took {total} steps, not {steps}")
+    return rec, node
+
+
+# ------------------------------------------------ traffic (rocprofv3 --pmc)
+
+def _counter_rows(outdir: Path):
+    import csv
+    rows = []
+    for f in sorted(outdir.rglob("*counter_collection.csv")):
+        with open(f, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _per_dispatch(rows, counter):
+    """[(dispatch id, kernel name, value)] of one counter, in dispatch order."""
+    acc = {}
+    for r in rows:
+        if r.get("Counter_Name") != counter:
+            continue
+        d = int(r.get("Dispatch_Id", 0))
+        name, v = r.get("Kernel_Name", ""), float(r.get("Counter_Value", 0.0))
+        prev = acc.get(d, (name, 0.0))
+        acc[d] = (name, prev[1] + v)
+    return [(d, acc[d][0], acc[d][1]) for d in sorted(acc)]
+
+
+def measure_traffic(args, kernel="k_advect_persistent"):
+    """Bytes past the L2 per launch of the advect kernel, measured in this run:
+    two rocprofv3 --pmc runs of this script in --traffic-child mode (the same
+    workload: a warm pass, then two measured passes, then a 1 GiB
+    device-to-device copy), one for FETCH_SIZE and one for WRITE_SIZE, each
+    scaled by what the same counter reads for the copy.  None if rocprofv3 is
+    missing or fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not Path(prof).exists():
+        return None, "rocprofv3 not found"
+    out = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = Path(tempfile.mkdtemp(prefix="dpah_pmc_", dir="/tmp"))
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", str(d), "-o", "run", "--",
+               sys.executable, "-u", str(ROOT / "bench.py"), "--traffic-child",
+               "--seeds", str(args.seeds), "--iterations", str(args.iterations)]
+        log(f"traffic: {counter} (rocprofv3 --pmc run of the same workload)")
+        try:
+            p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True, timeout=240)
+            rows = _counter_rows(d)
+        except subprocess.TimeoutExpired:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"rocprofv3 --pmc {counter} timed out"
+        shutil.rmtree(d, ignore_errors=True)
+        disp = _per_dispatch(rows, counter)
+        ours = [v for _, name, v in disp if kernel in name]
+        if len(ours) < 3:
+            tail = " | ".join(p.stdout.strip().splitlines()[-3:])
+            return None, f"rocprofv3 --pmc {counter}: {len(ours)} advect dispatches (rc {p.returncode}: {tail})"
+        last_ours = max(i for i, (_, name, _) in enumerate(disp) if kernel in name)
+        copies = [v for _, name, v in disp[last_ours + 1:] if "dpah" not in name and v > 0]
+        if not copies:
+            return None, f"rocprofv3 --pmc {counter}: no copy dispatch after the advect passes"
+        scale = COPY_BYTES / (copies[-1] * 1024.0)
+        kib = sum(ours[1:3]) / 2.0                       # the two measured passes
+        out[counter] = {"kib_per_dispatch": kib, "scale": scale, "bytes": kib * 1024.0 * scale,
+                        "copy_kib": copies[-1]}
+    total = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return total, out
+
+
+def traffic_child(args) -> int:
+    """The workload of run_vortex256 without timing, for a rocprofv3 --pmc run:
+    a warm pass, two measured passes, then a device-to-device copy of
+    COPY_BYTES that calibrates the byte counters."""
+    import ctypes as C
+    import dpa_amd
+    from dpa_amd import workloads as W
+    field = W.vortex_field(256)
+    seeds = W.vortex_seeds(args.seeds)
+    n = len(seeds)
+    ctx = dpa_amd.Context(0)
+    fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
+    d_seeds = ctx.to_device(seeds)
+    d_pos, d_rem, d_st = ctx.alloc(12 * n), ctx.alloc(8 * n), ctx.alloc(n)
+    for _ in range(3):
+        ctx.advect_endpoints_device(fld, d_seeds, n, d_pos, uniform_remaining=args.iterations,
+                                    step_size=np.float32(0.25), d_rem_out=d_rem,
+                                    d_status_out=d_st, wait=True)
+    src, dst = ctx.alloc(COPY_BYTES), ctx.alloc(COPY_BYTES)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    if hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), COPY_BYTES, 3) != 0:   # device to device
+        return 1
+    if hip.hipDeviceSynchronize() != 0:
+        return 1
+    for p in (src, dst, d_seeds, d_pos, d_rem, d_st):
+        ctx.free(p)
+    fld.release()
+    ctx.close()
+    return 0
 
 
 # ----------------------------------------------------------------- serving
@@ -253,6 +408,10 @@ def run_vortex256(args):
     if last != steps_per_pass:
         raise RuntimeError(f"step count changed between passes: {last} != {steps_per_pass}")
     avg_ms = total_ms / max(launches, 1)
+    alg_bytes, rec_loads, node_loads = counted_bytes(ctx, one_pass, n, steps_per_pass)
+    log(f"cell loads per pass: {rec_loads} records, {node_loads} node groups "
+        f"({rec_loads / steps_per_pass:.3f} per step): {alg_bytes / 1e9:.2f} GB "
+        f"= {alg_bytes / (avg_ms * 1e-3) / 1e9:.0f} GB/s")
     value = args.steps * steps_per_pass / elapsed
     log(f"timed {args.steps} passes in {elapsed:.3f} s: {value:.4e} particle-steps/s; "
         f"kernel avg {avg_ms:.3f} ms over {launches} launches")
@@ -306,8 +465,19 @@ def run_vortex256(args):
         log(f"cpu: dpa-structured {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads, "
             f"plain {cpu['plain_port']['value']:.4e}, one thread {cpu['one_thread']['value']:.4e}")
 
-    rf = roofline(steps_per_pass, avg_ms, traffic=read_traffic())
+    traffic, tsrc = (None, "not measured (--no-traffic)")
+    if args.traffic:
+        traffic, tsrc = measure_traffic(args)
+        if traffic is None:
+            log(f"traffic not measured: {tsrc}")
+        else:
+            log(f"traffic past the L2: {traffic / 1e9:.2f} GB per launch "
+                f"({traffic / (avg_ms * 1e-3) / 1e9:.0f} GB/s)")
+    rf = roofline(avg_ms, alg_bytes, steps_per_pass, traffic=traffic)
     rf["kernel"] = "k_advect_persistent"
+    rf["cell_record_loads"] = rec_loads
+    rf["node_group_loads"] = node_loads
+    rf["traffic_source"] = tsrc
     out = base_line(args, value, elapsed, "f32")
     out["config"] = {
         "workload": "large steady endpoint advection: 256^3 f32 Gaussian-vortex field "
@@ -372,17 +542,23 @@ def run_abc64(args):
     elapsed = time.perf_counter() - t0
     na, ta = ctx.kernel_timing("advect_persistent")
     nc, tc = ctx.kernel_timing("compact_write")
+    n = len(c["seeds"])
+    # cell loads: the recorded particles follow the same trajectories as an
+    # endpoint pass over the same seeds, which the counting instance runs
+    rec, node = endpoint_loads(ctx, fld, [(c["seeds"], c["iterations"], c["step_size"])], steps)
     fld.release()
     ctx.close()
     value = args.steps * steps / elapsed
     adv_ms = ta / max(na, 1)
+    # + every recorded vertex (steps, seeds, terminals) written once, 12 bytes
+    alg = rec * RECORD_BYTES + node * NODE_GROUP_BYTES + n * PARTICLE_IO_BYTES + 12 * (steps + 2 * n)
     out = base_line(args, value, elapsed, "f32")
     out["config"] = {"workload": "ABC 64^3, 4096 seeds x 256 steps dt 0.05 h, record, u32; "
                                  "value includes host upload / download and allocation",
                      "particle_steps_per_pass": steps}
     out["kernels"] = {"advect_persistent_ms": adv_ms, "compact_write_ms": tc / max(nc, 1),
                       "device_steps_per_s": steps / (adv_ms * 1e-3)}
-    out["roofline"] = roofline(steps, adv_ms, bytes_per_step=108)
+    out["roofline"] = roofline(adv_ms, alg, steps)
     if not args.no_cpu:
         from oracle import py_oracle as po
         th = host_threads()
@@ -425,6 +601,18 @@ def run_ftle(args, dpa_mode=False):
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     na, ta = ctx.kernel_timing("advect_persistent")
+    ctx.set_count_cell_loads(True)
+    try:
+        if one() != steps:
+            raise RuntimeError("the load-counting pass took a different number of steps")
+        rec, node = ctx.cell_loads()
+    finally:
+        ctx.set_count_cell_loads(False)
+    particles = points * (1 if dpa_mode else 6)
+    # loads + each particle's final position (lattice seeds are computed, not read)
+    alg = rec * RECORD_BYTES + node * NODE_GROUP_BYTES + 12 * particles
+    log(f"cell loads per pass: {rec} records, {node} node groups ({(rec + node) / steps:.3f} "
+        "per step)")
     ctx.free(d_out)
     fld.release()
     ctx.close()
@@ -435,8 +623,9 @@ def run_ftle(args, dpa_mode=False):
                                  ", ABC 512x512x128, T=4 at dt 0.02",
                      "lattice_points": points, "particle_steps_per_pass": steps,
                      "points_per_s": args.steps * points / elapsed}
-    out["roofline"] = roofline(steps, adv_ms)
-    out["roofline"]["kernel"] = "k_advect_persistent"
+    out["roofline"] = roofline(adv_ms, alg, steps)
+    out["roofline"].update(kernel="k_advect_persistent", cell_record_loads=rec,
+                           node_group_loads=node)
     return out
 
 
@@ -496,6 +685,11 @@ def run_batch(args):
             server.wait(t, -1).free()
     per_req = time.perf_counter() - t0
     log(f"per-request: {steps / per_req:.3e} steps/s")
+    rec, node = endpoint_loads(ctx, fld, [(j["seeds"], j["uniform_remaining"], j["step_size"])
+                                          for j in jobs], steps)
+    nseeds = int(sum(len(j["seeds"]) for j in jobs))
+    alg = (rec * RECORD_BYTES + node * NODE_GROUP_BYTES + nseeds * PARTICLE_IO_BYTES
+           + 12 * (steps + 2 * nseeds))
     fld.release()
     ctx.close()
     value = args.steps * steps / elapsed
@@ -511,7 +705,9 @@ def run_batch(args):
         out["graph_replay"] = graph
     out["per_request_launch"] = {"value": steps / per_req, "seconds": per_req}
     out["vs_per_request"] = value / (steps / per_req)
-    out["roofline"] = roofline(steps, ta / max(na, 1), bytes_per_step=108)
+    out["roofline"] = roofline(ta / max(na, 1), alg, steps)
+    out["roofline"].update(kernel="k_advect_persistent", cell_record_loads=rec,
+                           node_group_loads=node)
     return out
 
 
@@ -559,6 +755,10 @@ def main() -> int:
     ap.add_argument("--iterations", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", dest="traffic", action="store_false",
+                    help="skip the two rocprofv3 --pmc runs that measure the headline "
+                         "kernel's bytes past the L2")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=100.0)
     ap.add_argument("--serve-requests", type=int, default=40)
@@ -583,6 +783,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="the default: time the library's launch shapes as they are")
     args = ap.parse_args()
+    if args.traffic_child:
+        return traffic_child(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path here (with the field "
               "replicated, seeds would shard across GPUs with no exchange)", file=sys.stderr)

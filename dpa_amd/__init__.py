@@ -148,6 +148,8 @@ def load_library() -> C.CDLL:
         "dpah_result_ftle": (C.c_int, [vp, vp, _u64p]),
         "dpah_kernel_timing": (C.c_int, [vp, C.c_char_p, _u64p, C.POINTER(C.c_double)]),
         "dpah_kernel_timing_reset": (C.c_int, [vp]),
+        "dpah_set_count_cell_loads": (C.c_int, [vp, C.c_int]),
+        "dpah_cell_loads": (C.c_int, [vp, _u64p, _u64p]),
         "dpah_get_tuning": (C.c_int, [vp, C.POINTER(_Tuning)]),
         "dpah_set_tuning": (C.c_int, [vp, C.POINTER(_Tuning)]),
         "dpah_verify_division": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, _u64p]),
@@ -393,6 +395,17 @@ class Context:
 
     def kernel_timing_reset(self) -> None:
         _check(self.lib.dpah_kernel_timing_reset(self.h))
+
+    def set_count_cell_loads(self, on: bool) -> None:
+        """Counting of the cell cache's reloads by steady dpa-mode endpoint
+        launches (roofline instrumentation); turning it on zeroes the counts."""
+        _check(self.lib.dpah_set_count_cell_loads(self.h, 1 if on else 0))
+
+    def cell_loads(self) -> tuple[int, int]:
+        """(cell-record loads, node-group loads) since counting was turned on."""
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(self.lib.dpah_cell_loads(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def verify_division(self, h: float, r: float, bound: float) -> int:
         """Floats in [-bound, bound] where the reciprocal division differs from x / h."""

@@ -23,9 +23,15 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
     ctx->timed("advect_persistent", [&] {
+      // the load-counting instance exists for steady dpa-mode endpoint launches
+      constexpr bool kCountable = !kRecord && kIntegrator == kDpaRk4 && !kUnsteady;
       if (kRecord && a.slot_shift == kStagedShift)
         hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
                                                 kRecord ? kSlotDeferred : kSlotDirect>),
+                           dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
+      else if (kCountable && a.loads_out)
+        hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
+                                                kSlotDirect, kCountable>),
                            dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
       else
         hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv>),
@@ -69,6 +75,10 @@ __global__ void k_reset_counters(unsigned long long* c) {
   const unsigned t = threadIdx.x;
   if (t == 1 || (t >= (unsigned)kQueueHeads && t < (unsigned)(kQueueHeads + kMaxParts)))
     c[t] = 0ull;
+}
+
+__global__ void k_clear_loads(unsigned long long* c) {
+  if (threadIdx.x < 2) c[kLoadsCells + threadIdx.x] = 0ull;
 }
 
 // Device scratch of one ordering; released when the launch is enqueued, which
@@ -172,6 +182,11 @@ void order_queue(dpah_context* ctx, const Tuning& T, AdvectArgs& a, QueueOrder& 
 
 }  // namespace
 
+void clear_load_counters(dpah_context* ctx) {
+  hipLaunchKernelGGL(k_clear_loads, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters);
+  DPAH_HIP(hipGetLastError());
+}
+
 bool use_cells(const Tuning& T, bool scattered) {
   return T.cells == 2 || (T.cells == 1 && scattered);
 }
@@ -224,6 +239,10 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   DPAH_REQUIRE(L.slot_shift == 0 || (L.slot_shift == kStagedShift && L.persistent),
                "staged slot blocks need a persistent launch");
   a.slot_shift = L.slots ? L.slot_shift : 0u;
+  // always the context's counters (also for the allocation-free form, whose
+  // own counters are the scratch array's): loads accumulate across launches
+  a.loads_out = ctx->count_loads ? ctx->d_counters + (a.f.cells ? kLoadsCells : kLoadsNodes)
+                                 : nullptr;
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;

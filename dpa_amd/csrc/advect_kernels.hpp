@@ -153,6 +153,8 @@ struct AdvectArgs {
   uint32_t        refill;           // a wave draws new particles once this many lanes idle
   uint32_t        slot_shift;       // record: slot blocks of 2^slot_shift (kStagedShift:
                                     // staged through LDS, persistent launches only)
+  unsigned long long* loads_out;    // load-counting instance: cell cache reloads, one
+                                    // atomic add per wave at exit
 };
 
 __device__ __forceinline__ bool is_sentinel(float a, float b, float c) {
@@ -410,7 +412,12 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
 // the following parts once its own is drained.
 // kSlots (recorded launches): kSlotDeferred stages slot blocks in LDS (slot
 // shift kStagedShift), kSlotDirect stores each slot (slot shift 0).
-template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv, int kSlots = kSlotDirect>
+// kCountLoads: also counts the register cell cache's reloads into
+// a.loads_out (the roofline's byte count; a measurement instance, for steady
+// dpa-mode endpoint launches: one sample per step, so a step reloaded exactly
+// when it changed the cache's cell key).
+template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv, int kSlots = kSlotDirect,
+          bool kCountLoads = false>
 __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   // slot index (record) and time (time-variant fields, classic RK4) need the
   // step count inside the loop; steady dpa-mode endpoints only at the end
@@ -435,7 +442,10 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   uint32_t part = blockIdx.x % a.parts;   // wave-uniform
   uint32_t parts_left = a.parts;          // parts this wave has not found drained
   bool open = true;                       // wave-uniform: some part may have particles
+  static_assert(!kCountLoads || (!kRecord && !kUnsteady && kIntegrator == kDpaRk4),
+                "load counting: steady dpa-mode endpoint launches");
   uint64_t mine = 0;
+  uint64_t loads = 0;
   while (true) {
     const bool idle = (L.pid == ~0ull);
     if (open) {
@@ -463,8 +473,12 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
     if (!__any(busy) && !open) break;
     if constexpr (!kStaged) {
       if (busy) {
+        float k0 = 0.0f, k1 = 0.0f, k2 = 0.0f;
+        if constexpr (kCountLoads) { k0 = L.cc.s0; k1 = L.cc.s1; k2 = L.cc.s2; }
         const uint8_t st =
             step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt, kSlots>(a.f, a.coef, a.t0, L);
+        if constexpr (kCountLoads)
+          loads += (L.cc.s0 != k0 || L.cc.s1 != k1 || L.cc.s2 != k2) ? 1u : 0u;
         if (st != kActive) {
           mine += finish_particle<kRecord, kCountIt, kSlots>(a, L, st);
           L.pid = ~0ull;
@@ -500,6 +514,7 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
     }
   }
   add_wave_steps(a.total_steps, mine);
+  if constexpr (kCountLoads) add_wave_steps(a.loads_out, loads);
 }
 
 }  // namespace dpah

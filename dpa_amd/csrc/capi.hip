@@ -780,6 +780,30 @@ DPAH_EXPORT int dpah_kernel_timing(dpah_context* ctx, const char* kernel, uint64
   });
 }
 
+DPAH_EXPORT int dpah_set_count_cell_loads(dpah_context* ctx, int on) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx, "ctx is null");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    if (on && !ctx->count_loads) dpah::clear_load_counters(ctx);
+    ctx->count_loads = on != 0;
+  });
+}
+
+DPAH_EXPORT int dpah_cell_loads(dpah_context* ctx, uint64_t* record_loads, uint64_t* node_loads) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx, "ctx is null");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    unsigned long long c[2] = {0, 0};
+    DPAH_HIP(hipMemcpyAsync(c, ctx->d_counters + dpah::kLoadsCells, sizeof(c),
+                            hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    if (record_loads) *record_loads = c[0];
+    if (node_loads) *node_loads = c[1];
+  });
+}
+
 DPAH_EXPORT int dpah_kernel_timing_reset(dpah_context* ctx) {
   return guarded([&] {
     DPAH_REQUIRE(ctx, "null argument");

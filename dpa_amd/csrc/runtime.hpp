@@ -101,9 +101,12 @@ struct Tuning {
 };
 
 // Device counters of a context: [1] particle-steps, [2] FTLE misses,
-// [3] division-check mismatches,
+// [3] division-check mismatches, [kLoadsCells] / [kLoadsNodes] cell cache
+// reloads of load-counting launches that read cell records / nodes,
 // [kQueueHeads, kQueueHeads + kMaxParts) the persistent queue's part heads.
 constexpr int kCounterCount = 16;
+constexpr int kLoadsCells = 4;
+constexpr int kLoadsNodes = 5;
 constexpr int kQueueHeads = 8;
 constexpr int kMaxParts = 8;             // AdvectArgs::part_lo holds kMaxParts + 1
 
@@ -129,6 +132,7 @@ struct dpah_context {
   uint64_t cached_bytes = 0;
   void trim_cache();
   unsigned long long* d_counters = nullptr;   // kCounterCount counters (see Tuning)
+  bool count_loads = false;   // endpoint launches count their cell cache reloads
   bool timing = false;
   std::map<std::string, dpah::TimedKernel> timings;
   std::vector<dpah::PendingTiming> pending;
@@ -282,6 +286,8 @@ bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch
 bool brick_launch(const Tuning& T, const AdvectLaunch& L);
 // The tuning a launch of this key uses (an autotuned entry or the context's).
 const Tuning& tuning_for(const dpah_context* ctx, const std::string& key);
+// Zeroes the cell-load counters (a kernel on the context stream).
+void clear_load_counters(dpah_context* ctx);
 // Whether a launch reads the cell layout (when its field has one).
 bool use_cells(const Tuning& T, bool scattered);
 // Autotuning key of a launch: field shape, integrator, seed kind, record flag.

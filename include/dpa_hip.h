@@ -354,6 +354,17 @@ int dpah_batch_pass_destroy(dpah_batch_pass* pass);
 int dpah_kernel_timing(dpah_context* ctx, const char* kernel, uint64_t* launches, double* total_ms);
 int dpah_kernel_timing_reset(dpah_context* ctx);
 
+/* Roofline instrumentation (no counterpart in dpa).  While on, endpoint
+ * launches of DPAH_INTEGRATOR_DPA_RK4 on steady fields run a kernel instance
+ * that also counts the register cell cache's reloads -- one per step that
+ * crosses into a new cell (and one per particle's first sample): a 128-byte
+ * cell record for launches that read the cell layout, the cell's eight nodes
+ * (96 bytes) for launches that read the nodes.  These are the bytes the
+ * launch must bring past the L2.  Results are unchanged.  dpah_cell_loads
+ * returns the counts accumulated since counting was last turned on. */
+int dpah_set_count_cell_loads(dpah_context* ctx, int on);
+int dpah_cell_loads(dpah_context* ctx, uint64_t* record_loads, uint64_t* node_loads);
+
 /* Launch shape of the persistent advect kernel and the order of its work
  * queue.  No counterpart in dpa, whose CPU loop has no launch shape;
  * dpah_autotune_endpoints picks these by timing a sample of a workload. */
