@@ -52,7 +52,8 @@ class _Job(C.Structure):
 
 class _ServerOptions(C.Structure):
     _fields_ = [("memory_budget_bytes", C.c_uint64), ("max_active", C.c_uint32),
-                ("steps_per_tick", C.c_uint32), ("mode", C.c_int)]
+                ("steps_per_tick", C.c_uint32), ("mode", C.c_int),
+                ("results_on_host", C.c_int)]
 
 
 class _Request(C.Structure):
@@ -569,13 +570,15 @@ class Result:
 class Server:
     """A dpah_server: continuous batching (mode SERVE_BATCHED) or one of the
     comparators that run requests one at a time (SERVE_PER_REQUEST: allocate,
-    launch, download and free per request; SERVE_SERIAL: the library's
-    single-request path)."""
+    launch and free per request, one thread per particle; SERVE_SERIAL: the
+    library's single-request path).  results_on_host: every mode copies each
+    result to host memory before its ticket is ready."""
 
     def __init__(self, ctx: Context, mode: int = SERVE_BATCHED, memory_budget: int = 0,
-                 max_active: int = 0, steps_per_tick: int = 0):
+                 max_active: int = 0, steps_per_tick: int = 0, results_on_host: bool = False):
         self.ctx, self.lib = ctx, ctx.lib
-        o = _ServerOptions(memory_budget, max_active, steps_per_tick, mode)
+        o = _ServerOptions(memory_budget, max_active, steps_per_tick, mode,
+                           1 if results_on_host else 0)
         h = C.c_void_p()
         _check(self.lib.dpah_server_create(ctx.h, C.byref(o), C.byref(h)))
         self.h = h

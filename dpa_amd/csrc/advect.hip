@@ -11,7 +11,8 @@ namespace dpah {
 namespace {
 
 template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv>
-void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool persistent) {
+void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool persistent,
+              hipStream_t s) {
   if (a.n == 0) return;
   if (persistent) {
     const int block = T.block;
@@ -22,26 +23,26 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
     uint64_t grid = (uint64_t)ctx->cu_count * bpc;
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
-    ctx->timed("advect_persistent", [&] {
+    ctx->timed("advect_persistent", s, [&] {
       // the load-counting instance exists for steady dpa-mode endpoint launches
       constexpr bool kCountable = !kRecord && kIntegrator == kDpaRk4 && !kUnsteady;
       if (kRecord && a.slot_shift == kStagedShift)
         hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
                                                 kRecord ? kSlotDeferred : kSlotDirect>),
-                           dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
+                           dim3((unsigned)grid), dim3(block), 0, s, a);
       else if (kCountable && a.loads_out)
         hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
                                                 kSlotDirect, kCountable>),
-                           dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
+                           dim3((unsigned)grid), dim3(block), 0, s, a);
       else
         hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv>),
-                           dim3((unsigned)grid), dim3(block), 0, ctx->stream, a);
+                           dim3((unsigned)grid), dim3(block), 0, s, a);
     });
   } else {
     const uint64_t grid = (a.n + 255) / 256;
-    ctx->timed("advect_simple", [&] {
+    ctx->timed("advect_simple", s, [&] {
       hipLaunchKernelGGL((k_advect_simple<kRecord, kIntegrator, kUnsteady, kDiv>), dim3((unsigned)grid),
-                         dim3(256), 0, ctx->stream, a);
+                         dim3(256), 0, s, a);
     });
   }
   DPAH_HIP(hipGetLastError());
@@ -49,23 +50,23 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
 
 template <bool kRecord, int kDiv>
 void dispatch_integrator(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, int integrator,
-                         bool unsteady, bool persistent) {
+                         bool unsteady, bool persistent, hipStream_t s) {
   if (integrator == kRk4) {
-    if (unsteady) launch_t<kRecord, kRk4, true, kDiv>(ctx, T, a, persistent);
-    else          launch_t<kRecord, kRk4, false, kDiv>(ctx, T, a, persistent);
+    if (unsteady) launch_t<kRecord, kRk4, true, kDiv>(ctx, T, a, persistent, s);
+    else          launch_t<kRecord, kRk4, false, kDiv>(ctx, T, a, persistent, s);
   } else {
-    if (unsteady) launch_t<kRecord, kDpaRk4, true, kDiv>(ctx, T, a, persistent);
-    else          launch_t<kRecord, kDpaRk4, false, kDiv>(ctx, T, a, persistent);
+    if (unsteady) launch_t<kRecord, kDpaRk4, true, kDiv>(ctx, T, a, persistent, s);
+    else          launch_t<kRecord, kDpaRk4, false, kDiv>(ctx, T, a, persistent, s);
   }
 }
 
 template <bool kRecord>
 void dispatch_division(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, int integrator,
-                       bool unsteady, bool persistent) {
+                       bool unsteady, bool persistent, hipStream_t s) {
   switch (div_mode(a.f.exactdiv, a.f.fastdiv, unsteady)) {
-    case kDivExact: dispatch_integrator<kRecord, kDivExact>(ctx, T, a, integrator, unsteady, persistent); break;
-    case kDivFast:  dispatch_integrator<kRecord, kDivFast>(ctx, T, a, integrator, unsteady, persistent); break;
-    default:        dispatch_integrator<kRecord, kDivide>(ctx, T, a, integrator, unsteady, persistent); break;
+    case kDivExact: dispatch_integrator<kRecord, kDivExact>(ctx, T, a, integrator, unsteady, persistent, s); break;
+    case kDivFast:  dispatch_integrator<kRecord, kDivFast>(ctx, T, a, integrator, unsteady, persistent, s); break;
+    default:        dispatch_integrator<kRecord, kDivide>(ctx, T, a, integrator, unsteady, persistent, s); break;
   }
 }
 
@@ -120,7 +121,8 @@ void order_shape(const FieldView& f, uint32_t min_shift, uint32_t& shift, uint32
 // the queue in seed order a wave's rows are neighbours in memory (2^18 seeds of
 // the 256^3 vortex field x 1000 recorded steps: 4.0 ms in seed order, 5.1 ms ordered).
 bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch& L) {
-  return L.persistent && T.order && (ctx->caching || L.order_scratch) && L.nseg == 0 &&
+  return L.persistent && T.order && (ctx->caching || L.order_scratch) &&
+         (L.stream == nullptr || L.order_scratch) && L.nseg == 0 &&
          L.slots == nullptr && L.seed_kind == kSeedArray && L.n >= kOrderMinParticles &&
          L.n < (1ull << 32);
 }
@@ -146,7 +148,7 @@ bool brick_launch(const Tuning& T, const AdvectLaunch& L) {
 namespace {
 
 void order_queue(dpah_context* ctx, const Tuning& T, AdvectArgs& a, QueueOrder& q,
-                 const OrderScratch* keep) {
+                 const OrderScratch* keep, hipStream_t s) {
   OrderArgs o{};
   uint32_t tiles = 0;
   order_shape(a.f, (uint32_t)T.order_brick, o.shift, o.bits, o.keys, tiles);
@@ -166,14 +168,14 @@ void order_queue(dpah_context* ctx, const Tuning& T, AdvectArgs& a, QueueOrder& 
   }
   const unsigned clear_grid = std::min<unsigned>((o.keys + 255) / 256, 1024u);
   const unsigned grid = (unsigned)((a.n + 255) / 256);
-  ctx->timed("order_queue", [&] {
-    hipLaunchKernelGGL(k_order_clear, dim3(clear_grid), dim3(256), 0, ctx->stream, o);
-    hipLaunchKernelGGL(k_order_key, dim3(grid), dim3(256), 0, ctx->stream, a, o);
+  ctx->timed("order_queue", s, [&] {
+    hipLaunchKernelGGL(k_order_clear, dim3(clear_grid), dim3(256), 0, s, o);
+    hipLaunchKernelGGL(k_order_key, dim3(grid), dim3(256), 0, s, a, o);
     uint32_t* sums = o.count + o.keys;
-    hipLaunchKernelGGL(k_order_tile_sums, dim3(tiles), dim3(256), 0, ctx->stream, o, sums);
-    hipLaunchKernelGGL(k_order_scan_sums, dim3(1), dim3(256), 0, ctx->stream, sums, tiles);
-    hipLaunchKernelGGL(k_order_scan_apply, dim3(tiles), dim3(256), 0, ctx->stream, o, sums);
-    hipLaunchKernelGGL(k_order_scatter, dim3(grid), dim3(256), 0, ctx->stream, a.n, o);
+    hipLaunchKernelGGL(k_order_tile_sums, dim3(tiles), dim3(256), 0, s, o, sums);
+    hipLaunchKernelGGL(k_order_scan_sums, dim3(1), dim3(256), 0, s, sums, tiles);
+    hipLaunchKernelGGL(k_order_scan_apply, dim3(tiles), dim3(256), 0, s, o, sums);
+    hipLaunchKernelGGL(k_order_scatter, dim3(grid), dim3(256), 0, s, a.n, o);
   });
   DPAH_HIP(hipGetLastError());
   a.order = o.order;
@@ -243,18 +245,19 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   // own counters are the scratch array's): loads accumulate across launches
   a.loads_out = ctx->count_loads ? ctx->d_counters + (a.f.cells ? kLoadsCells : kLoadsNodes)
                                  : nullptr;
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, ctx->stream, counters);
+  hipStream_t const s = L.stream ? L.stream : ctx->stream;
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters);
   DPAH_HIP(hipGetLastError());
   QueueOrder q;
   if (ordered_launch(ctx, T, L)) {
-    order_queue(ctx, T, a, q, L.order_scratch);
+    order_queue(ctx, T, a, q, L.order_scratch, s);
   } else if (brick_launch(T, L)) {
     a.lat.brick = kLatticeBrick;
     set_parts(a, (uint32_t)std::max(1, std::min(T.parts, kMaxParts)));
   }
   const bool unsteady = L.f->nt > 1;
-  if (L.slots) dispatch_division<true>(ctx, T, a, L.integrator, unsteady, L.persistent);
-  else         dispatch_division<false>(ctx, T, a, L.integrator, unsteady, L.persistent);
+  if (L.slots) dispatch_division<true>(ctx, T, a, L.integrator, unsteady, L.persistent, s);
+  else         dispatch_division<false>(ctx, T, a, L.integrator, unsteady, L.persistent, s);
 }
 
 }  // namespace dpah

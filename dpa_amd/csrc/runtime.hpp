@@ -275,6 +275,10 @@ struct AdvectLaunch {
   const OrderScratch* order_scratch = nullptr;   // null: temporaries from the allocator
   // step counter and queue heads (kCounterCount words); null: the context's
   unsigned long long* counters = nullptr;
+  // stream of the launch; null: the context's.  On another stream a launch
+  // sorts its queue only into caller-owned scratch (order_scratch): the
+  // caching allocator reuses blocks in the context stream's order only.
+  hipStream_t stream = nullptr;
 };
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L);
 // Brick shape of the queue ordering of a field (order_kernels.hpp).

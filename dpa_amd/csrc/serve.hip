@@ -6,11 +6,15 @@
 // their particle state, and retires finished ones (compaction / FTLE reduce,
 // then the ticket becomes ready).  Each tick launches one kernel per
 // (field, integrator, record) group that advances every admitted particle by
-// up to `steps_per_tick` steps.  Two comparators run each request alone, in
-// arrival order: the per-request mode allocates, uploads, launches one thread
-// per particle, compacts, downloads and frees for every request; the serial
-// mode goes through the library's single-request path (dpah_advect /
-// dpah_ftle_aux_device on the context's one stream).
+// up to `steps_per_tick` steps; with no step budget an FTLE request instead
+// runs as its own persistent lattice launch (the brick-walked queue of
+// dpah_ftle_aux_device).  Up to four launches run at once, one per stream.
+// Two comparators run each request alone, in arrival order: the per-request
+// mode allocates, uploads, launches one thread per particle, compacts and
+// frees for every request; the serial mode goes through the library's
+// single-request path (dpah_advect / dpah_ftle_aux_device on the context's one
+// stream).  Results stay on the device, or are copied to host memory in every
+// mode alike (results_on_host).
 #include <algorithm>
 #include <cmath>
 #include <chrono>
@@ -115,9 +119,11 @@ void launch_tick(dpah_context* ctx, const TickArgs& a, int integrator, bool reco
 
 struct Launch {
   bool busy = false;
+  bool ftle = false;   // one FTLE request as a persistent lattice launch (else a tick)
   std::vector<std::shared_ptr<Request>> reqs;
   DeviceBuffer segs;
   hipEvent_t done = nullptr;
+  unsigned long long* counters = nullptr;   // this stream's step counter and queue heads
 };
 
 struct dpah_server {
@@ -147,9 +153,12 @@ struct dpah_server {
   void run_per_request();
   bool admit_locked();
   void setup(Request& r, hipStream_t s);
+  bool persistent_ftle(const Request& r) const;
+  void launch_ftle(const std::shared_ptr<Request>& rp, int sid);
   bool launch_ready();
   bool poll_launches();
   void finish(Request& r, hipStream_t s);
+  void finish_ftle(Launch& L, hipStream_t s);
   void release_request(const std::shared_ptr<Request>& r, int state, int err,
                        const std::string& msg);
   void complete(const std::shared_ptr<Request>& r, int state, int err, const std::string& msg);
@@ -334,15 +343,24 @@ bool dpah_server::launch_ready() {
     for (int i = 0; i < kStreams; ++i)
       if (!launches[i].busy) { sid = i; break; }
     if (sid < 0) break;
+    if (persistent_ftle(*ready.front())) {
+      auto rp = ready.front();
+      ready.erase(ready.begin());
+      launch_ftle(rp, sid);
+      launched = true;
+      continue;
+    }
     const Request& head = *ready.front();
     const bool rec0 = head.spec.kind == DPAH_REQUEST_ADVECT && head.spec.params.record;
     const auto key = std::make_tuple(head.spec.field, head.spec.params.integrator, rec0);
     Launch& L = launches[sid];
+    L.ftle = false;
     L.reqs.clear();
     std::vector<std::shared_ptr<Request>> rest;
     for (auto& r : ready) {
       const bool rec = r->spec.kind == DPAH_REQUEST_ADVECT && r->spec.params.record;
-      if (std::make_tuple(r->spec.field, r->spec.params.integrator, rec) == key)
+      if (!persistent_ftle(*r) &&
+          std::make_tuple(r->spec.field, r->spec.params.integrator, rec) == key)
         L.reqs.push_back(r);
       else
         rest.push_back(r);
@@ -432,6 +450,13 @@ bool dpah_server::poll_launches() {
     if (q == hipErrorNotReady) continue;
     DPAH_HIP(q);
     hipStream_t s = streams[sid];
+    if (L.ftle) {
+      finish_ftle(L, s);
+      L.reqs.clear();
+      L.busy = false;
+      progressed = true;
+      continue;
+    }
     for (auto& rp : L.reqs)
       DPAH_HIP(hipMemcpyAsync(&rp->left_h, d_left + rp->slot, sizeof(unsigned int),
                               hipMemcpyDeviceToHost, s));
@@ -522,7 +547,115 @@ void dpah_server::finish(Request& r, hipStream_t s) {
   DPAH_HIP(hipStreamSynchronize(s));
   res->total_steps = r.steps_h;
   r.steps = r.steps_h;
+  if (opt.results_on_host) materialize_host(res.get(), s);
   r.result = res.release();
+}
+
+// FTLE requests of a batched server that runs particles to retirement: each
+// is one persistent launch of the auxiliary lattice (its queue walking 4x4x4
+// bricks of points, as dpah_ftle_aux_device's does) on one of the server's
+// streams, with that stream's own counters, then the per-point reduction.
+bool dpah_server::persistent_ftle(const Request& r) const {
+  if (r.spec.kind != DPAH_REQUEST_FTLE_AUX || opt.steps_per_tick != 0) return false;
+  const uint64_t P = ftle_points(r.spec);
+  return P > 0 && P <= (1ull << 25);
+}
+
+void dpah_server::launch_ftle(const std::shared_ptr<Request>& rp, int sid) {
+  Request& r = *rp;
+  bool cancelled;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    cancelled = r.cancel;
+  }
+  if (cancelled) {
+    release_request(rp, kCancelled, DPAH_ERR_CANCELLED, "");
+    return;
+  }
+  Launch& L = launches[sid];
+  hipStream_t s = streams[sid];
+  const uint64_t P = ftle_points(r.spec);
+  try {
+    r.pos = DeviceBuffer(ctx, P * 18 * sizeof(float));   // the six particles' final positions
+    auto res = std::make_unique<dpah_result>();
+    res->ctx = ctx;
+    res->ftle_count = P;
+    res->d_ftle = static_cast<float*>(ctx->alloc(P * sizeof(float)));
+    AdvectLaunch A{};
+    A.f = &r.spec.field->view;
+    A.seed_kind = kSeedAux;
+    SeedLattice& lat = A.lat;
+    lat.lo0 = r.spec.origin[0]; lat.lo1 = r.spec.origin[1]; lat.lo2 = r.spec.origin[2];
+    lat.st0 = r.spec.lattice_spacing[0]; lat.st1 = r.spec.lattice_spacing[1];
+    lat.st2 = r.spec.lattice_spacing[2];
+    lat.d0 = r.spec.lattice_shape[0]; lat.d1 = r.spec.lattice_shape[1];
+    lat.d2 = r.spec.lattice_shape[2];
+    lat.dl0 = r.spec.delta[0]; lat.dl1 = r.spec.delta[1]; lat.dl2 = r.spec.delta[2];
+    A.uniform_rem = r.spec.iterations;
+    A.pos_out = r.pos.as<float>();
+    A.n = 6 * P;
+    A.dt = r.spec.params.step_size;
+    A.t0 = r.spec.params.start_time;
+    A.integrator = r.spec.params.integrator;
+    A.persistent = true;
+    A.counters = L.counters;
+    A.stream = s;
+    launch_advect(ctx, A);
+    FtleAuxArgs a{};
+    a.finals = r.pos.as<float>();
+    a.points = P;
+    a.twod0 = 2.0f * r.spec.delta[0]; a.twod1 = 2.0f * r.spec.delta[1];
+    a.twod2 = 2.0f * r.spec.delta[2];
+    a.T_abs = std::fabs(r.spec.params.step_size * (float)r.spec.iterations);
+    a.out = res->d_ftle;
+    hipLaunchKernelGGL(k_ftle_aux_reduce, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, a);
+    DPAH_HIP(hipGetLastError());
+    DPAH_HIP(hipEventRecord(L.done, s));
+    r.result = res.release();
+  } catch (const Error& e) {
+    (void)hipStreamSynchronize(s);
+    delete r.result;
+    r.result = nullptr;
+    release_request(rp, kFailed, e.code, e.what());
+    return;
+  }
+  if (r.t_launch == 0) r.t_launch = now_s();
+  L.ftle = true;
+  L.reqs.assign(1, rp);
+  L.busy = true;
+  std::lock_guard<std::mutex> lk(mu);
+  stats.ticks += 1;
+}
+
+void dpah_server::finish_ftle(Launch& L, hipStream_t s) {
+  auto rp = L.reqs.front();
+  Request& r = *rp;
+  DPAH_HIP(hipMemcpyAsync(&r.steps_h, L.counters + 1, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+  DPAH_HIP(hipStreamSynchronize(s));
+  ctx->resolve_timings();
+  r.t_device = now_s();
+  bool cancelled;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    cancelled = r.cancel;
+  }
+  if (cancelled) {
+    delete r.result;
+    r.result = nullptr;
+    release_request(rp, kCancelled, DPAH_ERR_CANCELLED, "");
+    return;
+  }
+  r.steps = r.steps_h;
+  r.result->total_steps = r.steps_h;
+  try {
+    if (opt.results_on_host) materialize_host(r.result, s);
+    release_request(rp, kDone, DPAH_OK, "");
+  } catch (const Error& e) {
+    delete r.result;
+    r.result = nullptr;
+    release_request(rp, kFailed, e.code, e.what());
+  }
 }
 
 void dpah_server::run_batched() {
@@ -606,7 +739,7 @@ void dpah_server::run_per_request() {
         r->result = run_advect(ctx, r->spec.field, r->seeds.data(), r->rem.data(), 0, r->count,
                                &r->spec.params, !naive);
       }
-      if (naive) materialize_host(r->result, ctx->stream);
+      if (opt.results_on_host) materialize_host(r->result, ctx->stream);
       r->steps = r->result->total_steps;
       r->t_device = now_s();
       complete(r, kDone, DPAH_OK, "");
@@ -648,6 +781,10 @@ DPAH_EXPORT int dpah_server_create(dpah_context* ctx, const dpah_server_options*
       for (int i = 0; i < dpah_server::kStreams; ++i) {
         DPAH_HIP(hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking));
         DPAH_HIP(hipEventCreateWithFlags(&s->launches[i].done, hipEventDisableTiming));
+        s->launches[i].counters = static_cast<unsigned long long*>(
+            ctx->alloc(kCounterCount * sizeof(unsigned long long)));
+        DPAH_HIP(hipMemsetAsync(s->launches[i].counters, 0,
+                                kCounterCount * sizeof(unsigned long long), ctx->stream));
       }
       ctx->sync();
     }
@@ -828,6 +965,8 @@ DPAH_EXPORT int dpah_server_destroy(dpah_server* s) {
     for (int i = 0; i < dpah_server::kStreams; ++i) {
       if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
       s->launches[i].segs.reset();
+      if (s->launches[i].counters)
+        s->ctx->release(s->launches[i].counters, kCounterCount * sizeof(unsigned long long));
       if (s->launches[i].done) (void)hipEventDestroy(s->launches[i].done);
       if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
     }

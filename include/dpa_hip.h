@@ -230,12 +230,15 @@ int dpah_ftle_aux_device(dpah_context* ctx, const dpah_field* field, const float
  * (field, integrator, record) go into ONE launch, which advances each particle
  * by up to steps_per_tick steps (0: to retirement) and puts unfinished
  * requests back in line; results are compacted on the launch's stream.
- * Two comparators run each request alone in arrival order:
+ * With steps_per_tick 0 an FTLE request runs as its own persistent launch of
+ * the auxiliary lattice (the kernel dpah_ftle_aux_device runs) on one of the
+ * streams.  Two comparators run each request alone in arrival order:
  * DPAH_SERVE_PER_REQUEST allocates (hipMalloc), uploads, launches one thread
- * per particle, compacts, downloads the result to host memory and frees
- * (hipFree) per request; DPAH_SERVE_SERIAL goes through the single-request
- * path (dpah_advect / dpah_ftle_aux_device, caching allocator, persistent
- * kernel).  A request's result is the same in every mode and mix, bit for bit. */
+ * per particle, compacts and frees (hipFree) per request; DPAH_SERVE_SERIAL
+ * goes through the single-request path (dpah_advect / dpah_ftle_aux_device,
+ * caching allocator, persistent kernel).  results_on_host applies to every
+ * mode alike.  A request's result is the same in every mode and mix, bit for
+ * bit. */
 #define DPAH_REQUEST_ADVECT   0   /* dpah_advect arguments                    */
 #define DPAH_REQUEST_FTLE_AUX 1   /* dpah_ftle_aux_device arguments           */
 
@@ -256,6 +259,8 @@ typedef struct dpah_server_options {
   uint32_t max_active;           /* requests admitted at once; 0: 256                        */
   uint32_t steps_per_tick;       /* per particle per launch; 0: run to retirement            */
   int      mode;                 /* DPAH_SERVE_*                                              */
+  int      results_on_host;      /* 1: each result is copied to host memory before its ticket
+                                    is ready; 0: results stay in device memory (every mode)   */
 } dpah_server_options;
 
 typedef struct dpah_request {

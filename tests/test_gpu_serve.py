@@ -97,13 +97,17 @@ def alone(ctx, r):
                       step_size=r["step_size"], start_time=r["start_time"], record=r["record"])
 
 
-@pytest.mark.parametrize("mode,tick", [(dpa_amd.SERVE_BATCHED, 16), (dpa_amd.SERVE_BATCHED, 1000),
-                                       (dpa_amd.SERVE_BATCHED, 0), (dpa_amd.SERVE_PER_REQUEST, 0),
-                                       (dpa_amd.SERVE_SERIAL, 0)])
-def test_mixed_requests_match_alone(ctx, fields, mode, tick):
-    say(f"case serve mode={mode} steps_per_tick={tick}")
+@pytest.mark.parametrize("mode,tick,host", [(dpa_amd.SERVE_BATCHED, 16, False),
+                                            (dpa_amd.SERVE_BATCHED, 1000, False),
+                                            (dpa_amd.SERVE_BATCHED, 0, False),
+                                            (dpa_amd.SERVE_BATCHED, 0, True),
+                                            (dpa_amd.SERVE_PER_REQUEST, 0, False),
+                                            (dpa_amd.SERVE_PER_REQUEST, 0, True),
+                                            (dpa_amd.SERVE_SERIAL, 0, False)])
+def test_mixed_requests_match_alone(ctx, fields, mode, tick, host):
+    say(f"case serve mode={mode} steps_per_tick={tick} results_on_host={host}")
     reqs = make_requests(fields)
-    with dpa_amd.Server(ctx, mode=mode, steps_per_tick=tick) as server:
+    with dpa_amd.Server(ctx, mode=mode, steps_per_tick=tick, results_on_host=host) as server:
         tickets = [submit(server, r) for r in reqs]
         results = [server.wait(t, 120000, n=len(r.get("seeds", []))) for t, r in zip(tickets, reqs)]
         infos = [server.info(t) for t in tickets]
