@@ -282,34 +282,43 @@ def traffic_child(args) -> int:
 
 # ----------------------------------------------------------------- serving
 
-def serve_trace(ctx, fields, trace, mode, steps_per_tick=0):
-    """Replays a trace against a server; returns latency / throughput figures."""
+def serve_trace(ctx, fields, trace, mode, steps_per_tick=0, saturate=False):
+    """Replays a trace against a server (results left in device memory, freed
+    as they are collected); returns latency / throughput figures.  saturate:
+    every request is submitted at once (a backlog the server drains at its
+    capacity) instead of at its Poisson arrival time."""
     import dpa_amd
     lat, steps = [], 0
     with dpa_amd.Server(ctx, mode=mode, steps_per_tick=steps_per_tick) as server:
         tickets = []
         done = threading.Event()
         results = {}
+        failure = []
 
         def collector():
-            i = 0
-            while i < len(trace):
-                while i >= len(tickets):
-                    time.sleep(0.0005)
-                t = tickets[i]
-                server.wait(t, -1).free()
-                results[i] = server.info(t)
-                server.forget(t)
-                i += 1
-            done.set()
+            try:
+                i = 0
+                while i < len(trace):
+                    while i >= len(tickets):
+                        time.sleep(0.0005)
+                    t = tickets[i]
+                    server.wait(t, -1).free()
+                    results[i] = server.info(t)
+                    server.forget(t)
+                    i += 1
+            except Exception as e:   # surfaces in the caller
+                failure.append(e)
+            finally:
+                done.set()
 
         th = threading.Thread(target=collector)
         th.start()
         t0 = time.perf_counter()
         for r in trace:
-            delay = r["at"] - (time.perf_counter() - t0)
-            if delay > 0:
-                time.sleep(delay)
+            if not saturate:
+                delay = r["at"] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
             f = fields[r["field"]]
             if r["kind"] == "ftle":
                 tickets.append(server.submit_ftle(f, r["origin"], r["spacing"], r["shape"],
@@ -324,6 +333,8 @@ def serve_trace(ctx, fields, trace, mode, steps_per_tick=0):
         done.wait()
         th.join()
         t_end = time.perf_counter()
+        if failure:
+            raise failure[0]
     for i in range(len(trace)):
         lat.append(results[i]["latency_seconds"])
         steps += results[i]["particle_steps"]
@@ -341,21 +352,52 @@ def serve_trace(ctx, fields, trace, mode, steps_per_tick=0):
             "p50_device_done_ms": p50("device_seconds")}
 
 
-def serving_leg(ctx, fields, trace, steps_per_tick=0):
+def serving_leg(ctx, fields, trace, saturate=False):
+    """The trace through the batched server and both comparators, results left
+    in device memory in all three."""
     import dpa_amd
-    b = serve_trace(ctx, fields, trace, dpa_amd.SERVE_BATCHED, steps_per_tick)
-    log(f"served batched: {b['throughput']:.3e} steps/s p50 {b['p50_ms']:.1f} ms "
-        f"p99 {b['p99_ms']:.1f} ms (p50 admitted {b['p50_admitted_ms']:.1f}, launched "
-        f"{b['p50_launched_ms']:.1f}, device done {b['p50_device_done_ms']:.1f} ms)")
-    p = serve_trace(ctx, fields, trace, dpa_amd.SERVE_PER_REQUEST)
-    log(f"per-request:    {p['throughput']:.3e} steps/s p50 {p['p50_ms']:.1f} ms "
-        f"p99 {p['p99_ms']:.1f} ms")
-    s = serve_trace(ctx, fields, trace, dpa_amd.SERVE_SERIAL)
-    log(f"serial library: {s['throughput']:.3e} steps/s p50 {s['p50_ms']:.1f} ms "
-        f"p99 {s['p99_ms']:.1f} ms")
-    return {"batched": b, "per_request_launch": p, "serial_library_calls": s,
-            "throughput_ratio": b["throughput"] / max(p["throughput"], 1e-9),
-            "throughput_ratio_vs_serial": b["throughput"] / max(s["throughput"], 1e-9)}
+    out = {}
+    for key, mode in (("batched", dpa_amd.SERVE_BATCHED),
+                      ("serial_library_calls", dpa_amd.SERVE_SERIAL),
+                      ("per_request_launch", dpa_amd.SERVE_PER_REQUEST)):
+        r = serve_trace(ctx, fields, trace, mode, saturate=saturate)
+        out[key] = r
+        log(f"{'saturated' if saturate else 'poisson'} {key:22s} {r['throughput']:.3e} steps/s "
+            f"makespan {r['makespan_s']:.2f} s p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms")
+    b = out["batched"]["throughput"]
+    out["batched_vs_per_request"] = b / max(out["per_request_launch"]["throughput"], 1e-9)
+    out["batched_vs_serial"] = b / max(out["serial_library_calls"]["throughput"], 1e-9)
+    return out
+
+
+def serving_benchmark(ctx, base, args):
+    """Four resident 256^3 x 16-slice time-variant fields (12.9 GB); a Poisson
+    trace at args.serve_rate and the same number of requests submitted at once
+    (saturation), each through the batched server, the serial library calls
+    and the per-request launch form."""
+    from dpa_amd import workloads as W
+    t = time.perf_counter()
+    hosts = W.serving_fields(base)
+    fields = [ctx.field_unsteady(h, (0, 0, 0), (1, 1, 1), 0.0, 1.0) for h in hosts]
+    del hosts
+    log(f"served fields resident in {time.perf_counter() - t:.1f} s")
+    try:
+        t = time.perf_counter()
+        trace = W.serving_trace(args.serve_rate, args.serve_requests)
+        log(f"trace of {len(trace)} requests ready in {time.perf_counter() - t:.1f} s")
+        out = {"trace": (f"{len(trace)} requests on 4 resident 256^3 x 16-slice time-variant "
+                         "vortex fields: 80% recorded pathlines (1k-262k seeds log-uniform, "
+                         "128-2048 steps, dt 0.005), 20% FTLE slabs (256x256x16 lattice, "
+                         "+-0.5 cell, 200 steps, dt 0.02); results left in device memory "
+                         "in every mode"),
+               "ftle_requests": sum(r["kind"] == "ftle" for r in trace),
+               "poisson": dict(rate=args.serve_rate, **serving_leg(ctx, fields, trace))}
+        if args.serve_saturate:
+            out["saturated"] = serving_leg(ctx, fields, trace, saturate=True)
+        return out
+    finally:
+        for f in fields:
+            f.release()
 
 
 # --------------------------------------------------------------- workloads
@@ -442,19 +484,10 @@ def run_vortex256(args):
         ctx.free(p)
     ctx_tuning = ctx.tuning()
 
+    fld.release()
     serving = None
     if not args.no_serve:
-        # short served-query leg on a time-variant version of the same field
-        fu = ctx.field_unsteady(W.moving_field(field, 16), (0, 0, 0), (1, 1, 1), 0.0, 1.0)
-        trace = W.serving_trace(args.serve_rate, args.serve_requests, fields=1,
-                                ftle_shape=(16, 64, 64))
-        serving = serving_leg(ctx, [fu], trace)
-        serving["trace"] = (f"{args.serve_requests} Poisson arrivals at {args.serve_rate} req/s on "
-                            "a 256^3 x 16-slice time-variant field: 80% recorded pathlines "
-                            "(1k-262k seeds, 128-2048 steps, dt 0.005), 20% FTLE slabs "
-                            "(16x64x64 lattice, +-0.5 cell, 200 steps, dt 0.02)")
-        fu.release()
-    fld.release()
+        serving = serving_benchmark(ctx, field, args)
     ctx.close()
 
     cpu = None
@@ -718,18 +751,19 @@ def run_serve(args):
     from dpa_amd import workloads as W
     base = W.vortex_field(256)
     ctx = make_context(args)
-    fields = []
-    for k in range(4):
-        shifted = np.roll(base, shift=(37 * k, 71 * k, 13 * k), axis=(0, 1, 2))
-        fields.append(ctx.field_unsteady(W.moving_field(shifted, 16, phase=k), (0, 0, 0),
-                                         (1, 1, 1), 0.0, 1.0))
-        log(f"field {k} resident")
+    fields = [ctx.field_unsteady(h, (0, 0, 0), (1, 1, 1), 0.0, 1.0)
+              for h in W.serving_fields(base)]
     rates = [float(r) for r in args.rates.split(",")]
     sweep = []
     for rate in rates:
         trace = W.serving_trace(rate, args.serve_requests)
         leg = serving_leg(ctx, fields, trace)
         leg["rate"] = rate
+        sweep.append(leg)
+    if args.serve_saturate:
+        leg = serving_leg(ctx, fields, W.serving_trace(rates[-1], args.serve_requests),
+                          saturate=True)
+        leg["rate"] = "saturated"
         sweep.append(leg)
     for f in fields:
         f.release()
@@ -760,8 +794,10 @@ def main() -> int:
                          "kernel's bytes past the L2")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=100.0)
-    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=500.0)
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--no-serve-saturate", dest="serve_saturate", action="store_false",
+                    help="skip the saturation trace (every request submitted at once)")
     ap.add_argument("--rates", default="50,100,200,500")
     ap.add_argument("--order", type=int, choices=[0, 1], default=None,
                     help="spatially ordered, XCD-split work queue (default: library's)")

@@ -43,7 +43,17 @@ void dpah_context::trim_cache() {
   cached_bytes = 0;
 }
 
+void dpah_context::drain_deferred() {
+  std::vector<dpah_result*> take;
+  {
+    std::lock_guard<std::mutex> lk(deferred_mu);
+    take.swap(deferred);
+  }
+  for (dpah_result* r : take) delete r;
+}
+
 void* dpah_context::alloc(uint64_t bytes) {
+  drain_deferred();
   const uint64_t want = round_block(bytes);
   if (caching) {
     auto it = cached.lower_bound(want);
@@ -220,6 +230,7 @@ DPAH_EXPORT int dpah_destroy(dpah_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->resolve_timings();
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    ctx->drain_deferred();
     for (auto& a : ctx->user_allocs) ctx->release(a.first, a.second);
     if (ctx->d_counters) ctx->release(ctx->d_counters, dpah::kCounterCount * sizeof(unsigned long long));
     ctx->trim_cache();
@@ -722,7 +733,12 @@ DPAH_EXPORT int dpah_result_free(dpah_result* r) {
   return guarded([&] {
     if (!r) return;
     dpah_context* ctx = r->ctx;
-    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    std::unique_lock<std::recursive_mutex> lk(ctx->mu, std::try_to_lock);
+    if (!lk.owns_lock()) {   // another thread's call holds the context: defer
+      std::lock_guard<std::mutex> d(ctx->deferred_mu);
+      ctx->deferred.push_back(r);
+      return;
+    }
     DPAH_HIP(hipSetDevice(ctx->device));
     delete r;
   });

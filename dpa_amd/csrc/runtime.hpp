@@ -112,6 +112,8 @@ constexpr int kMaxParts = 8;             // AdvectArgs::part_lo holds kMaxParts 
 
 }  // namespace dpah
 
+struct dpah_result;
+
 struct dpah_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -144,6 +146,12 @@ struct dpah_context {
   std::map<uint64_t, bool> division_checked;   // (spacing bits, range bits) -> exact
   std::recursive_mutex mu;
 
+  // Results freed by a thread that found the context busy (a long call holding
+  // mu): deleted, and their blocks returned to the cache, at the next alloc
+  // or drain_deferred, so a free never waits for another thread's GPU work.
+  std::mutex deferred_mu;
+  std::vector<dpah_result*> deferred;
+  void drain_deferred();   // call with mu held
   void* alloc(uint64_t bytes);
   void release(void* p, uint64_t bytes);
   hipEvent_t take_event();

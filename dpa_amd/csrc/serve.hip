@@ -711,6 +711,32 @@ void dpah_server::run_per_request() {
         cv_done.notify_all();
         continue;
       }
+    }
+    // memory admission of the one-at-a-time modes: wait, without the
+    // context's lock, until the request fits beside the results the caller
+    // still holds (whose frees need that lock)
+    // (a request larger than the whole budget goes ahead and fails on its own)
+    while (r->footprint <= ctx->memory_budget &&
+           ctx->bytes_in_use.load() + r->footprint > ctx->memory_budget) {
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (stop || r->cancel) break;
+      }
+      {
+        std::lock_guard<std::recursive_mutex> ctx_lk(ctx->mu);
+        ctx->drain_deferred();
+      }
+      std::this_thread::sleep_for(std::chrono::microseconds(200));
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (r->cancel || stop) {
+        r->state = kCancelled;
+        r->t_done = now_s();
+        stats.cancelled += 1;
+        cv_done.notify_all();
+        continue;
+      }
       r->state = kRunning;
       r->t_admit = now_s();
       r->t_launch = r->t_admit;

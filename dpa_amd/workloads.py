@@ -186,16 +186,36 @@ def batch_jobs(count=64, n=256, seed_offset=4):
     return jobs
 
 
-def moving_field(base: np.ndarray, time_steps: int, shift=(0, 0, 1), phase=0.0) -> np.ndarray:
+def moving_field(base: np.ndarray, time_steps: int, shift=(0, 0, 1), phase=0.0,
+                 offset=(0, 0, 0), threads=None) -> np.ndarray:
     """Time-variant field from a steady one: slice t is the base pattern moved by
-    t * shift nodes (periodic roll) with its amplitude pulsing as
-    1 + 0.25 sin(0.7 t + phase).  [nt][n0][n1][n2][3] float32."""
+    offset + t * shift nodes (periodic roll) with its amplitude pulsing as
+    1 + 0.25 sin(0.7 t + phase).  [nt][n0][n1][n2][3] float32.  Slabs of
+    axis 0 of every slice are filled in parallel."""
+    n0 = base.shape[0]
     out = np.empty((time_steps,) + base.shape, np.float32)
-    for t in range(time_steps):
+    threads = threads or min(16, os.cpu_count() or 1)
+    slab = max(1, n0 // (2 * threads))
+
+    def work(job):
+        t, z0 = job
+        z1 = min(n0, z0 + slab)
+        s0, s1, s2 = (offset[a] + t * shift[a] for a in range(3))
         amp = np.float32(1.0 + 0.25 * math.sin(0.7 * t + phase))
-        rolled = np.roll(base, shift=(t * shift[0], t * shift[1], t * shift[2]), axis=(0, 1, 2))
-        np.multiply(rolled, amp, out=out[t])
+        rows = (np.arange(z0, z1) - s0) % n0          # out row z holds base row z - s0
+        np.multiply(np.roll(base[rows], shift=(s1, s2), axis=(1, 2)), amp, out=out[t, z0:z1])
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(work, [(t, z0) for t in range(time_steps) for z0 in range(0, n0, slab)]))
     return out
+
+
+def serving_fields(base: np.ndarray, count=4, time_steps=16):
+    """The served field set: `count` time-variant fields of `time_steps`
+    slices, field k moving the base pattern from a roll of (37k, 71k, 13k)
+    nodes with its own pulse phase."""
+    return [moving_field(base, time_steps, phase=k, offset=(37 * k, 71 * k, 13 * k))
+            for k in range(count)]
 
 
 def serving_trace(rate, count, seed_offset=5, fields=4, ftle_fraction=0.2,
