@@ -616,6 +616,18 @@ def run_ftle(args, dpa_mode=False):
     shape = c["lattice_shape"]
     points = int(np.prod(shape))
     d_out = ctx.alloc(4 * points)
+    tuned = None
+    if args.autotune:
+        t = time.perf_counter()
+        if dpa_mode:
+            tuned, tms = ctx.autotune_lattice(fld, c["offset"], c["spacing"],
+                                              c["field"].shape[:3], c["iterations"],
+                                              c["step_size"])
+        else:
+            tuned, tms = ctx.autotune_lattice(fld, c["lattice_origin"], c["lattice_spacing"],
+                                              shape, c["iterations"], c["step_size"],
+                                              delta=c["delta"])
+        log(f"autotuned in {time.perf_counter() - t:.1f} s: {tuned} ({tms:.3f} ms)")
 
     def one():
         if dpa_mode:
@@ -655,7 +667,8 @@ def run_ftle(args, dpa_mode=False):
                                   "FTLE auxiliary +-0.5 cell neighbours") +
                                  ", ABC 512x512x128, T=4 at dt 0.02",
                      "lattice_points": points, "particle_steps_per_pass": steps,
-                     "points_per_s": args.steps * points / elapsed}
+                     "points_per_s": args.steps * points / elapsed,
+                     "tuning": tuned or "library defaults"}
     out["roofline"] = roofline(adv_ms, alg, steps)
     out["roofline"].update(kernel="k_advect_persistent", cell_record_loads=rec,
                            node_group_loads=node)
@@ -673,6 +686,11 @@ def run_batch(args):
     fld = ctx.field(field, (0, 0, 0), (1, 1, 1))
     for j in jobs:
         j["field"] = fld
+    tuned = None
+    if args.autotune:
+        t = time.perf_counter()
+        tuned, tms = ctx.autotune_batch(jobs)
+        log(f"autotuned in {time.perf_counter() - t:.1f} s: {tuned} ({tms:.3f} ms)")
 
     def batched():
         res = ctx.advect_batch(jobs)
@@ -731,7 +749,7 @@ def run_batch(args):
                                  "the 256^3 vortex field, one persistent launch + compaction; "
                                  "host upload / download included",
                      "jobs": len(jobs), "seeds": int(sum(len(j['seeds']) for j in jobs)),
-                     "particle_steps_per_pass": steps}
+                     "particle_steps_per_pass": steps, "tuning": tuned or "library defaults"}
     out["kernels"] = {"advect_persistent_ms": ta / max(na, 1),
                       "compact_write_ms_total_per_pass": tc / max(args.steps, 1)}
     if graph:

@@ -67,7 +67,12 @@ class _Request(C.Structure):
 class _Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("order", C.c_int),
                 ("parts", C.c_int), ("refill", C.c_int), ("cells", C.c_int),
-                ("fastdiv", C.c_int), ("cell_brick", C.c_int), ("order_brick", C.c_int)]
+                ("fastdiv", C.c_int), ("cell_brick", C.c_int), ("order_brick", C.c_int),
+                ("lattice_brick", C.c_int), ("slot_stage", C.c_int)]
+
+
+def _tuning_dict(t) -> dict:
+    return {name: getattr(t, name) for name, _ in _Tuning._fields_}
 
 
 class _RequestInfo(C.Structure):
@@ -167,6 +172,11 @@ def load_library() -> C.CDLL:
         "dpah_autotune_endpoints": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, C.c_float,
                                               C.c_float, C.c_int, C.POINTER(_Tuning),
                                               C.POINTER(C.c_double)]),
+        "dpah_autotune_batch": (C.c_int, [vp, C.POINTER(_Job), C.c_uint64, C.POINTER(_Tuning),
+                                          C.POINTER(C.c_double)]),
+        "dpah_autotune_lattice": (C.c_int, [vp, vp, _f32p, _f32p, _u64p, _f32p, C.c_uint64,
+                                            C.c_float, C.c_float, C.c_int, C.POINTER(_Tuning),
+                                            C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -440,22 +450,40 @@ class Context:
                                                 int(uniform_remaining), float(step_size),
                                                 float(start_time), int(integrator), C.byref(t),
                                                 C.byref(ms)))
-        return (dict(block=t.block, blocks_per_cu=t.blocks_per_cu, order=t.order, parts=t.parts,
-                     refill=t.refill, cells=t.cells, fastdiv=t.fastdiv, cell_brick=t.cell_brick,
-                     order_brick=t.order_brick), ms.value)
+        return _tuning_dict(t), ms.value
+
+    def autotune_batch(self, jobs: list[dict]) -> tuple[dict, float]:
+        """Times candidate launch shapes of each group of advect_batch's jobs on
+        its own persistent launch; returns the tuning of the largest group and
+        its milliseconds."""
+        arr, _keep = self._job_array(jobs)
+        t = _Tuning()
+        ms = C.c_double()
+        _check(self.lib.dpah_autotune_batch(self.h, arr, len(jobs), C.byref(t), C.byref(ms)))
+        return _tuning_dict(t), ms.value
+
+    def autotune_lattice(self, field: "Field", origin, spacing, shape, iterations: int,
+                         step_size: float, delta=None, start_time: float = 0.0,
+                         integrator: int = INTEGRATOR_DPA_RK4) -> tuple[dict, float]:
+        """Times candidate launch shapes of a whole-lattice launch (auxiliary
+        FTLE particles when delta is given, else the lattice points)."""
+        t = _Tuning()
+        ms = C.c_double()
+        _check(self.lib.dpah_autotune_lattice(self.h, field.h, _f3(origin), _f3(spacing),
+                                              _u3(shape), _f3(delta) if delta is not None else None,
+                                              int(iterations), float(step_size), float(start_time),
+                                              int(integrator), C.byref(t), C.byref(ms)))
+        return _tuning_dict(t), ms.value
 
     def tuning(self) -> dict:
         t = _Tuning()
         _check(self.lib.dpah_get_tuning(self.h, C.byref(t)))
-        return dict(block=t.block, blocks_per_cu=t.blocks_per_cu, order=t.order, parts=t.parts,
-                    refill=t.refill, cells=t.cells, fastdiv=t.fastdiv, cell_brick=t.cell_brick,
-                    order_brick=t.order_brick)
+        return _tuning_dict(t)
 
     def set_tuning(self, **kw) -> None:
         cur = self.tuning()
         cur.update(kw)
-        t = _Tuning(cur["block"], cur["blocks_per_cu"], cur["order"], cur["parts"], cur["refill"],
-                    cur["cells"], cur["fastdiv"], cur["cell_brick"], cur["order_brick"])
+        t = _Tuning(*[cur[name] for name, _ in _Tuning._fields_])
         _check(self.lib.dpah_set_tuning(self.h, C.byref(t)))
 
 

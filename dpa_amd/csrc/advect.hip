@@ -134,12 +134,16 @@ bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch
 // advection, 39.7 ms in C order).  Whole lattices in one launch only, since
 // the walk is over the whole lattice and the outputs are indexed by lattice
 // point.
+uint32_t lattice_brick(const Tuning& T) {
+  return T.lattice_brick ? (uint32_t)T.lattice_brick : kLatticeBrick;
+}
+
 bool brick_launch(const Tuning& T, const AdvectLaunch& L) {
   if (!(L.persistent && T.order && L.nseg == 0 && L.slots == nullptr && L.lat.first == 0))
     return false;
   const uint64_t points = L.lat.d0 * L.lat.d1 * L.lat.d2;
   // the walk's 32-bit arithmetic: positions and a whole slab of bricks
-  if ((uint64_t)kLatticeBrick * L.lat.d1 * L.lat.d2 >= (1ull << 32)) return false;
+  if ((uint64_t)lattice_brick(T) * L.lat.d1 * L.lat.d2 >= (1ull << 32)) return false;
   if (L.seed_kind == kSeedLattice) return L.n == points && L.n < (1ull << 32);
   if (L.seed_kind == kSeedAux) return L.n == 6 * points && L.n < (1ull << 32);
   return false;
@@ -204,6 +208,12 @@ const Tuning& tuning_for(const dpah_context* ctx, const std::string& key) {
   return it == ctx->tuned.end() ? ctx->tune : it->second;
 }
 
+uint32_t record_slot_shift(const dpah_context* ctx, const FieldView& f, int integrator,
+                           uint64_t rows, bool persistent) {
+  const Tuning& T = tuning_for(ctx, launch_key(f, integrator, kSeedArray, true));
+  return slot_shift_for(rows, persistent, T.slot_stage);
+}
+
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   const Tuning& T =
       tuning_for(ctx, launch_key(*L.f, L.integrator, L.seed_kind, L.slots != nullptr));
@@ -252,7 +262,7 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   if (ordered_launch(ctx, T, L)) {
     order_queue(ctx, T, a, q, L.order_scratch, s);
   } else if (brick_launch(T, L)) {
-    a.lat.brick = kLatticeBrick;
+    a.lat.brick = lattice_brick(T);
     set_parts(a, (uint32_t)std::max(1, std::min(T.parts, kMaxParts)));
   }
   const bool unsteady = L.f->nt > 1;

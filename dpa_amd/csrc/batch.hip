@@ -155,7 +155,9 @@ struct GroupRun {
       segs[s].stride = record ? r.stride : 0;
       q += tiled_rows(r.count);   // segments start on 64-ordinal boundaries (whole waves)
     }
-    slot_shift = record ? slot_shift_for(q, true) : 0u;   // one layout for the launch
+    // one slot layout for the launch
+    slot_shift = record ? record_slot_shift(ctx, j0.field->view, j0.params.integrator, q, true)
+                        : 0u;
     for (size_t s = 0; s < refs.size(); ++s) {
       segs[s].slot_base = slot_floats;
       if (record) slot_floats += slot_region_floats(segs[s].count, segs[s].stride, slot_shift);
@@ -412,5 +414,57 @@ DPAH_EXPORT int dpah_batch_pass_destroy(dpah_batch_pass* p) {
     std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
     (void)hipSetDevice(p->ctx->device);
     delete p;
+  });
+}
+
+// Launch shapes of a batch's recorded launches: per group of jobs, workgroups
+// per CU x refill x slot layout (one-slot runs, or 8-slot blocks staged in
+// LDS), each timed on the group's own persistent launch (the slot layout is
+// planned again for each candidate, since it decides the slot buffer).
+DPAH_EXPORT int dpah_autotune_batch(dpah_context* ctx, const dpah_job* jobs, uint64_t job_count,
+                                    dpah_tuning* best_out, double* best_ms) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && jobs && job_count > 0, "null argument or empty batch");
+    for (uint64_t i = 0; i < job_count; ++i) validate(ctx, jobs[i]);
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    uint64_t most = 0;
+    for (const auto& members : group_jobs(jobs, job_count)) {
+      const dpah_job& j0 = jobs[members[0]];
+      const std::string key = launch_key(j0.field->view, j0.params.integrator, kSeedArray,
+                                         j0.params.record != 0);
+      uint64_t work = 0;
+      for (uint64_t m : members) work += jobs[m].count;
+      std::vector<Tuning> cands;
+      for (int bpc : {3, 4, 5, 6, 8})
+        for (int refill : {16, 64})
+          for (int stage : {1, 2}) {
+            if (!j0.params.record && stage == 2) continue;
+            Tuning t = ctx->tune;
+            t.block = 256;
+            t.blocks_per_cu = bpc;
+            t.refill = refill;
+            t.slot_stage = j0.params.record ? stage : 0;
+            cands.push_back(t);
+          }
+      std::unique_ptr<GroupRun> g;
+      auto prepare = [&](const Tuning&) {
+        g.reset();
+        ctx->sync();   // the previous candidate's buffers are idle before reuse
+        g = std::make_unique<GroupRun>();
+        g->plan(ctx, jobs, members);
+      };
+      double ms = 0;
+      const Tuning best = tune_key(ctx, key, cands, prepare, [&] {
+        if (g->L.nseg) launch_advect(ctx, g->L);
+      }, &ms);
+      g.reset();
+      ctx->sync();
+      if (work >= most) {   // report the group with the most particles
+        most = work;
+        if (best_out) tuning_to_abi(best, best_out);
+        if (best_ms) *best_ms = ms;
+      }
+    }
   });
 }

@@ -601,7 +601,8 @@ dpah_result* dpah::run_advect(dpah_context* ctx, const dpah_field* field, const 
           uint64_t chunk = std::max<uint64_t>(1, ctx->slot_budget / row_bytes) / kTileRows *
                            kTileRows;
           chunk = std::min(std::max<uint64_t>(chunk, kTileRows), rc);
-          const uint32_t g = slot_shift_for(chunk, persistent);
+          const uint32_t g = record_slot_shift(ctx, field->view, params->integrator, chunk,
+                                               persistent);
           DeviceBuffer slots(ctx, slot_region_floats(chunk, stride, g) * sizeof(float));
           for (uint64_t c0 = 0; c0 < rc; c0 += chunk) {
             const uint64_t cn = std::min(chunk, rc - c0);
@@ -843,6 +844,8 @@ DPAH_EXPORT int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out) {
     out->fastdiv = ctx->tune.fastdiv;
     out->cell_brick = ctx->tune.cell_brick;
     out->order_brick = ctx->tune.order_brick;
+    out->lattice_brick = ctx->tune.lattice_brick;
+    out->slot_stage = ctx->tune.slot_stage;
   });
 }
 
@@ -861,6 +864,10 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(t->cell_brick == 0 || (t->cell_brick >= 2 && t->cell_brick <= 5),
                  "cell_brick must be 0 or 2..5");
     DPAH_REQUIRE(t->order_brick >= 0 && t->order_brick <= 5, "order_brick must be 0..5");
+    DPAH_REQUIRE(t->lattice_brick == 0 || t->lattice_brick == 2 || t->lattice_brick == 4 ||
+                     t->lattice_brick == 8,
+                 "lattice_brick must be 0, 2, 4 or 8");
+    DPAH_REQUIRE(t->slot_stage >= 0 && t->slot_stage <= 2, "slot_stage must be 0, 1 or 2");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     ctx->tune.block = t->block;
     ctx->tune.blocks_per_cu = t->blocks_per_cu;
@@ -871,6 +878,8 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     ctx->tune.fastdiv = t->fastdiv;
     ctx->tune.cell_brick = t->cell_brick;
     ctx->tune.order_brick = t->order_brick;
+    ctx->tune.lattice_brick = t->lattice_brick;
+    ctx->tune.slot_stage = t->slot_stage;
     ctx->tuned.clear();   // explicit settings replace autotuned ones
   });
 }

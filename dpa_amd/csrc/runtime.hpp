@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <functional>
 #include <map>
 #include <memory>
 #include <set>
@@ -98,6 +99,12 @@ struct Tuning {
   // queue ordering of seed arrays: bricks of at least 2^order_brick cells per
   // axis (0: 4-cell bricks; larger when the grid needs more than 64 per axis)
   int order_brick = 0;
+  // whole-lattice launches walk their points in bricks of this many points per
+  // axis (2, 4 or 8); 0: kLatticeBrick
+  int lattice_brick = 0;
+  // record slots of persistent recorded launches: 0 by launch size (8-slot
+  // blocks staged in LDS from kStageMinRows rows), 1 one-slot runs, 2 staged
+  int slot_stage = 0;
 };
 
 // Device counters of a context: [1] particle-steps, [2] FTLE misses,
@@ -306,6 +313,13 @@ bool use_cells(const Tuning& T, bool scattered);
 std::string launch_key(const FieldView& f, int integrator, int seed_kind, bool record);
 // Loads the context's autotune cache file (entries of this device) into ctx->tuned.
 void load_autotune_cache(dpah_context* ctx);
+void tuning_to_abi(const Tuning& t, dpah_tuning* out);
+// Times launch() under every candidate tuning of `key` (prepare(t) runs untimed
+// before each candidate's two timed launches), keeps the fastest in ctx->tuned
+// and the autotune cache file, and returns it.
+Tuning tune_key(dpah_context* ctx, const std::string& key, const std::vector<Tuning>& cands,
+                const std::function<void(const Tuning&)>& prepare,
+                const std::function<void()>& launch, double* best_ms);
 // Looks up, once per process and device, the address of the allocation-free
 // entry point's scratch array on `device` (which must be current).
 void noalloc_setup(int device);
@@ -337,9 +351,14 @@ dpah_result* run_ftle_aux(dpah_context* ctx, const dpah_field* field, const floa
 // slot as it comes (ABC 64^3, 4096 rows x 258 slots: 0.317 ms with slot
 // shift 0, 0.356 ms with blocks of eight stored directly, 0.39 ms staged).
 constexpr uint64_t kStageMinRows = 1ull << 15;
-inline uint32_t slot_shift_for(uint64_t rows, bool persistent) {
-  return persistent && rows >= kStageMinRows ? kStagedShift : 0u;
+inline uint32_t slot_shift_for(uint64_t rows, bool persistent, int slot_stage = 0) {
+  if (!persistent || slot_stage == 1) return 0u;
+  return slot_stage == 2 || rows >= kStageMinRows ? kStagedShift : 0u;
 }
+// The same under the tuning of recorded launches on field f with this
+// integrator (an autotuned entry or the context's).
+uint32_t record_slot_shift(const dpah_context* ctx, const FieldView& f, int integrator,
+                           uint64_t rows, bool persistent);
 
 // Prunes, indexes and colours `count` slot rows (slot shift g, slot_float);
 // allocates the chunk's outputs.  Synchronises the stream twice (the output

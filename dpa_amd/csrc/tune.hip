@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <sstream>
 
 #include "runtime.hpp"
@@ -17,14 +18,11 @@ using namespace dpah;
 
 namespace {
 
-struct Candidate {
-  int block, refill, blocks_per_cu, order, cells, order_brick;
-};
-
 // Cache file: one line per launch key,
 // "<arch> <cus> <key> <block> <blocks_per_cu> <order> <parts> <refill> <cells> <fastdiv>
-// <cell_brick> <order_brick> <ms>" (cell_brick is a field-creation setting, kept for the
-// record).
+// <cell_brick> <order_brick> <ms> [<lattice_brick> <slot_stage>]" (cell_brick is a
+// field-creation setting, kept for the record; lines without the last two fields
+// read them as 0).
 void save_cache(const dpah_context* ctx, const std::string& key, const Tuning& t, double ms) {
   if (ctx->autotune_path.empty()) return;
   std::vector<std::string> keep;
@@ -43,7 +41,8 @@ void save_cache(const dpah_context* ctx, const std::string& key, const Tuning& t
   for (const auto& l : keep) out << l << "\n";
   out << ctx->arch << " " << ctx->cu_count << " " << key << " " << t.block << " "
       << t.blocks_per_cu << " " << t.order << " " << t.parts << " " << t.refill << " " << t.cells
-      << " " << t.fastdiv << " " << t.cell_brick << " " << t.order_brick << " " << ms << "\n";
+      << " " << t.fastdiv << " " << t.cell_brick << " " << t.order_brick << " " << ms << " "
+      << t.lattice_brick << " " << t.slot_stage << "\n";
   if (!out) throw Error(DPAH_ERR_INVALID, "cannot write the autotune cache " + ctx->autotune_path);
 }
 
@@ -65,15 +64,76 @@ void load_autotune_cache(dpah_context* ctx) {
     if (!(ss >> arch >> cus >> key >> t.block >> t.blocks_per_cu >> t.order >> t.parts >>
           t.refill >> t.cells >> t.fastdiv >> t.cell_brick >> t.order_brick >> ms))
       continue;
+    int lb = 0, st = 0;
+    if (ss >> lb >> st) { t.lattice_brick = lb; t.slot_stage = st; }
     if (arch != ctx->arch || cus != ctx->cu_count) continue;
     const bool ok = t.block >= 64 && t.block <= 256 && t.block % 64 == 0 &&
                     t.blocks_per_cu >= 0 && t.blocks_per_cu <= 32 && (t.order == 0 || t.order == 1) &&
                     t.parts >= 1 && t.parts <= kMaxParts && t.refill >= 0 && t.refill <= 64 &&
                     t.cells >= 0 && t.cells <= 2 && (t.fastdiv == 0 || t.fastdiv == 1) &&
                     (t.cell_brick == 0 || (t.cell_brick >= 2 && t.cell_brick <= 5)) &&
-                    t.order_brick >= 0 && t.order_brick <= 5;
+                    t.order_brick >= 0 && t.order_brick <= 5 &&
+                    (t.lattice_brick == 0 || t.lattice_brick == 2 || t.lattice_brick == 4 ||
+                     t.lattice_brick == 8) &&
+                    t.slot_stage >= 0 && t.slot_stage <= 2;
     if (ok) ctx->tuned[key] = t;
   }
+}
+
+void tuning_to_abi(const Tuning& t, dpah_tuning* out) {
+  out->block = t.block;
+  out->blocks_per_cu = t.blocks_per_cu;
+  out->order = t.order;
+  out->parts = t.parts;
+  out->refill = t.refill;
+  out->cells = t.cells;
+  out->fastdiv = t.fastdiv;
+  out->cell_brick = t.cell_brick;
+  out->order_brick = t.order_brick;
+  out->lattice_brick = t.lattice_brick;
+  out->slot_stage = t.slot_stage;
+}
+
+// Times `launch` (one launch on the context's stream) under every candidate
+// tuning of `key`, twice each after a warm-up, keeps the fastest in ctx->tuned
+// and the cache file, and returns it.  On failure the key's entry is dropped.
+Tuning tune_key(dpah_context* ctx, const std::string& key, const std::vector<Tuning>& cands,
+                const std::function<void(const Tuning&)>& prepare,
+                const std::function<void()>& launch, double* best_ms) {
+  hipEvent_t e0 = ctx->take_event(), e1 = ctx->take_event();
+  auto time_once = [&]() {
+    DPAH_HIP(hipEventRecord(e0, ctx->stream));
+    launch();
+    DPAH_HIP(hipEventRecord(e1, ctx->stream));
+    DPAH_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    DPAH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return (double)ms;
+  };
+  Tuning best = ctx->tune;
+  double best_t = 1e300;
+  try {
+    ctx->tuned[key] = ctx->tune;
+    prepare(ctx->tune);
+    time_once();   // warm-up: code objects, caches, allocator blocks
+    for (const Tuning& t : cands) {
+      ctx->tuned[key] = t;
+      prepare(t);
+      const double ms = std::min(time_once(), time_once());
+      if (ms < best_t) { best_t = ms; best = t; }
+    }
+  } catch (...) {
+    ctx->tuned.erase(key);
+    ctx->event_pool.push_back(e0);
+    ctx->event_pool.push_back(e1);
+    throw;
+  }
+  ctx->event_pool.push_back(e0);
+  ctx->event_pool.push_back(e1);
+  ctx->tuned[key] = best;
+  save_cache(ctx, key, best, best_t);
+  if (best_ms) *best_ms = best_t;
+  return best;
 }
 
 }  // namespace dpah
@@ -104,21 +164,9 @@ DPAH_EXPORT int dpah_autotune_endpoints(dpah_context* ctx, const dpah_field* fie
     L.t0 = start_time;
     L.integrator = integrator;
     L.persistent = true;
-    hipEvent_t e0 = ctx->take_event(), e1 = ctx->take_event();
-    auto time_once = [&](const Tuning& t) {
-      ctx->tuned[key] = t;
-      DPAH_HIP(hipEventRecord(e0, ctx->stream));
-      launch_advect(ctx, L);
-      DPAH_HIP(hipEventRecord(e1, ctx->stream));
-      DPAH_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      DPAH_HIP(hipEventElapsedTime(&ms, e0, e1));
-      return (double)ms;
-    };
-    const Tuning base = ctx->tune;
-    std::vector<Candidate> cands;
     // workgroup size x workgroups per CU as waves per CU (12..32), refill,
     // queue order with its brick size, field layout
+    std::vector<Tuning> cands;
     for (int block : {128, 256})
       for (int waves : {12, 16, 20, 24, 32})
         for (int refill : {4, 8, 16, 64})
@@ -127,44 +175,71 @@ DPAH_EXPORT int dpah_autotune_endpoints(dpah_context* ctx, const dpah_field* fie
               for (int cells : {0, 1}) {
                 if (cells && !field->view.cells) continue;
                 if (!order && obrick != 2) continue;   // the brick only shapes the sort
-                cands.push_back({block, refill, waves * 64 / block, order, cells, obrick});
+                Tuning t = ctx->tune;
+                t.block = block;
+                t.blocks_per_cu = waves * 64 / block;
+                t.refill = refill;
+                t.order = order;
+                t.order_brick = obrick;
+                t.cells = cells;
+                cands.push_back(t);
               }
-    Tuning best = base;
-    double best_t = 1e300;
-    try {
-      time_once(base);   // warm-up: code objects, caches, allocator blocks
-      for (const Candidate& c : cands) {
-        Tuning t = base;
-        t.block = c.block;
-        t.order_brick = c.order_brick;
-        t.refill = c.refill;
-        t.blocks_per_cu = c.blocks_per_cu;
-        t.order = c.order;
-        t.cells = c.cells;
-        const double ms = std::min(time_once(t), time_once(t));
-        if (ms < best_t) { best_t = ms; best = t; }
-      }
-    } catch (...) {
-      ctx->tuned.erase(key);
-      ctx->event_pool.push_back(e0);
-      ctx->event_pool.push_back(e1);
-      throw;
-    }
-    ctx->event_pool.push_back(e0);
-    ctx->event_pool.push_back(e1);
-    ctx->tuned[key] = best;
-    save_cache(ctx, key, best, best_t);
-    if (best_out) {
-      best_out->block = best.block;
-      best_out->blocks_per_cu = best.blocks_per_cu;
-      best_out->order = best.order;
-      best_out->parts = best.parts;
-      best_out->refill = best.refill;
-      best_out->cells = best.cells;
-      best_out->fastdiv = best.fastdiv;
-      best_out->cell_brick = best.cell_brick;
-      best_out->order_brick = best.order_brick;
-    }
-    if (best_ms) *best_ms = best_t;
+    const Tuning best = tune_key(ctx, key, cands, [](const Tuning&) {},
+                                 [&] { launch_advect(ctx, L); }, best_ms);
+    if (best_out) tuning_to_abi(best, best_out);
+  });
+}
+
+DPAH_EXPORT int dpah_autotune_lattice(dpah_context* ctx, const dpah_field* field,
+                                      const float origin[3], const float spacing[3],
+                                      const uint64_t shape[3], const float* delta,
+                                      uint64_t iterations, float step_size, float start_time,
+                                      int integrator, dpah_tuning* best_out, double* best_ms) {
+  return guarded([&] {
+    DPAH_REQUIRE(ctx && field && origin && spacing && shape, "null argument");
+    DPAH_REQUIRE(field->ctx == ctx, "field belongs to another context");
+    DPAH_REQUIRE(integrator == DPAH_INTEGRATOR_DPA_RK4 || integrator == DPAH_INTEGRATOR_RK4,
+                 "unknown integrator");
+    DPAH_REQUIRE(iterations <= (1ull << 32) - 3, "iterations must be below 2^32 - 2");
+    const uint64_t points = shape[0] * shape[1] * shape[2];
+    DPAH_REQUIRE(points > 0, "empty lattice");
+    const int kind = delta ? kSeedAux : kSeedLattice;
+    const uint64_t n = delta ? 6 * points : points;
+    DPAH_REQUIRE(n < (1ull << 32), "lattice too large to tune in one launch");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DPAH_HIP(hipSetDevice(ctx->device));
+    const std::string key = launch_key(field->view, integrator, kind, false);
+    DeviceBuffer pos(ctx, n * 3 * sizeof(float));
+    AdvectLaunch L{};
+    L.f = &field->view;
+    L.seed_kind = kind;
+    L.lat.lo0 = origin[0]; L.lat.lo1 = origin[1]; L.lat.lo2 = origin[2];
+    L.lat.st0 = spacing[0]; L.lat.st1 = spacing[1]; L.lat.st2 = spacing[2];
+    L.lat.d0 = shape[0]; L.lat.d1 = shape[1]; L.lat.d2 = shape[2];
+    if (delta) { L.lat.dl0 = delta[0]; L.lat.dl1 = delta[1]; L.lat.dl2 = delta[2]; }
+    L.uniform_rem = iterations;
+    L.pos_out = pos.as<float>();
+    L.n = n;
+    L.dt = step_size;
+    L.t0 = start_time;
+    L.integrator = integrator;
+    L.persistent = true;
+    // workgroups per CU x refill x queue walk (C order, or bricks of 2 / 4 / 8
+    // points per axis)
+    std::vector<Tuning> cands;
+    for (int bpc : {2, 3, 4, 6, 8})
+      for (int refill : {16, 32, 64})
+        for (int brick : {0, 2, 4, 8}) {
+          Tuning t = ctx->tune;
+          t.block = 256;
+          t.blocks_per_cu = bpc;
+          t.refill = refill;
+          t.order = brick ? 1 : 0;
+          t.lattice_brick = brick;
+          cands.push_back(t);
+        }
+    const Tuning best = tune_key(ctx, key, cands, [](const Tuning&) {},
+                                 [&] { launch_advect(ctx, L); }, best_ms);
+    if (best_out) tuning_to_abi(best, best_out);
   });
 }

@@ -403,6 +403,12 @@ typedef struct dpah_tuning {
                           2^k cells per axis stored one after another         */
   int order_brick;     /* queue ordering of seed arrays: bricks of at least
                           2^order_brick cells per axis, 1..5; 0: 4 cells      */
+  int lattice_brick;   /* whole-lattice launches (FTLE) walk their points in
+                          bricks of this many points per axis: 2, 4 or 8;
+                          0: 4                                                */
+  int slot_stage;      /* record slots of persistent recorded launches: 0 by
+                          launch size (8-slot blocks staged in LDS from 2^15
+                          rows), 1 one-slot runs, 2 staged blocks             */
 } dpah_tuning;
 int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
 /* Sets the context's tuning and drops autotuned entries. */
@@ -420,6 +426,28 @@ int dpah_autotune_endpoints(dpah_context* ctx, const dpah_field* field, const fl
                             uint64_t count, uint64_t uniform_remaining, float step_size,
                             float start_time, int integrator, dpah_tuning* best,
                             double* best_ms);
+
+/* Times candidate launch shapes of recorded (or endpoint) batch launches --
+ * workgroups per CU 3..8 x refill 16 / 64 x slot layout (one-slot runs or
+ * 8-slot blocks staged in LDS) -- on each group of the jobs of
+ * dpah_advect_batch (the groups that function forms), each candidate timed on
+ * the group's own persistent launch, and keeps the fastest per launch key
+ * (field shape, integrator, seed array, record flag).  best / best_ms: the
+ * group with the most particles.  No counterpart in dpa. */
+int dpah_autotune_batch(dpah_context* ctx, const dpah_job* jobs, uint64_t job_count,
+                        dpah_tuning* best, double* best_ms);
+
+/* Times candidate launch shapes of a whole-lattice launch -- workgroups per CU
+ * 2..8 x refill 16 / 32 / 64 x queue walk (C order, or bricks of 2, 4 or 8
+ * points per axis) -- and keeps the fastest for its launch key.  delta non-NULL:
+ * the six auxiliary particles per point of dpah_ftle_aux_device (origin,
+ * spacing, shape as there); NULL: the lattice points themselves, as
+ * dpah_ftle_lattice seeds them (origin = field offset, spacing = field spacing
+ * x stride, shape = points per axis).  No counterpart in dpa. */
+int dpah_autotune_lattice(dpah_context* ctx, const dpah_field* field, const float origin[3],
+                          const float spacing[3], const uint64_t shape[3], const float* delta,
+                          uint64_t iterations, float step_size, float start_time, int integrator,
+                          dpah_tuning* best, double* best_ms);
 
 /* The check behind dpah_tuning.fastdiv, exposed for diagnostics: the number of
  * floats x in [-bound, bound] whose reciprocal quotient (x*r corrected once by
