@@ -419,6 +419,7 @@ static void build_cells(dpah_context* ctx, dpah_field* f) {
   DPAH_HIP(hipGetLastError());
   ctx->sync();
   v.cells = f->d_cells;
+  if (8 * records >= (1ull << 32)) v.idx32 = 0;
 }
 
 static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shape[3], uint64_t nt,
@@ -460,6 +461,7 @@ static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shap
   v.cell_slice = 0;
   v.cell_bits = 0;
   v.cb1 = v.cb2 = 0;
+  v.idx32 = 3 * nodes * nt < (1ull << 32) ? 1u : 0u;
   try {
     set_division(ctx, v);
     build_cells(ctx, f);

@@ -44,6 +44,10 @@ struct FieldView {
   // exact and d * r is the same real number as d / h -- the same rounding, for
   // every d, with no check needed.
   uint32_t exactdiv;
+  // 1: every node's float offset (all slices) and every cell record's float4
+  // index fit 32 bits, so a reload forms its offset in 32-bit arithmetic and
+  // adds it to the base once
+  uint32_t idx32;
 };
 
 enum Status : uint8_t { kActive = 0, kCompleted = 1, kExited = 2, kZero = 3 };
@@ -215,6 +219,19 @@ __host__ __device__ inline int64_t cell_record(const FieldView& f, int32_t it, i
   return (((int64_t)it * (f.n0 - 1) + c0) * (f.n1 - 1) + c1) * (int64_t)(f.n2 - 1) + c2;
 }
 
+// cell_record for fields with idx32 (the record index and 8 times it fit 32 bits).
+__device__ __forceinline__ uint32_t cell_record32(const FieldView& f, uint32_t it, uint32_t c0,
+                                                  uint32_t c1, uint32_t c2) {
+  if (f.cell_bits) {   // field-uniform
+    const uint32_t k = f.cell_bits, m = (1u << k) - 1u;
+    const uint32_t brick = ((c0 >> k) * (uint32_t)f.cb1 + (c1 >> k)) * (uint32_t)f.cb2 + (c2 >> k);
+    const uint32_t inner = ((c0 & m) << (2 * k)) | ((c1 & m) << k) | (c2 & m);
+    return it * (uint32_t)f.cell_slice + (brick << (3 * k)) + inner;
+  }
+  return ((it * (uint32_t)(f.n0 - 1) + c0) * (uint32_t)(f.n1 - 1) + c1) * (uint32_t)(f.n2 - 1) +
+         c2;
+}
+
 // Eight corners from one cell record: six 16-B loads of one 128-B line.
 __device__ __forceinline__ void load_cell(const float4* rec, V3& L0, V3& L1, V3& L2, V3& L3,
                                           V3& L4, V3& L5, V3& L6, V3& L7) {
@@ -249,16 +266,28 @@ __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
   if (a0.s != cc.s0 || a1.s != cc.s1 || a2.s != cc.s2) {
     const int32_t c0 = (int32_t)a0.s, c1 = (int32_t)a1.s, c2 = (int32_t)a2.s;
     if (f.cells) {   // field-uniform branch
-      const float4* rec = f.cells + 8 * cell_record(f, 0, c0, c1, c2);
+      const float4* rec = f.idx32 ? f.cells + 8u * cell_record32(f, 0u, c0, c1, c2)
+                                  : f.cells + 8 * cell_record(f, 0, c0, c1, c2);
       load_cell(rec, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
     } else {
       // one base address per reload; the four rows sit at launch-uniform
       // distances and each row's two nodes 12 bytes apart
-      const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
-      const float* p0 = slice + 3 * ((int64_t)c0 * n12 + (int64_t)c1 * n2 + c2);
-      const float* p1 = p0 + 3 * n2;
-      const float* p2 = p0 + 3 * n12;
-      const float* p3 = p2 + 3 * n2;
+      const float *p0, *p1, *p2, *p3;
+      if (f.idx32) {   // field-uniform branch: the four row offsets in 32 bits
+        const uint32_t u2 = 3u * (uint32_t)f.n2, u12 = u2 * (uint32_t)f.n1;
+        const uint32_t o0 = 3u * (((uint32_t)c0 * (uint32_t)f.n1 + (uint32_t)c1) *
+                                  (uint32_t)f.n2 + (uint32_t)c2);
+        p0 = slice + o0;
+        p1 = slice + (o0 + u2);
+        p2 = slice + (o0 + u12);
+        p3 = slice + (o0 + u12 + u2);
+      } else {
+        const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
+        p0 = slice + 3 * ((int64_t)c0 * n12 + (int64_t)c1 * n2 + c2);
+        p1 = p0 + 3 * n2;
+        p2 = p0 + 3 * n12;
+        p3 = p2 + 3 * n2;
+      }
       cc.L0 = V3{p0[0], p0[1], p0[2]}; cc.L1 = V3{p0[3], p0[4], p0[5]};
       cc.L2 = V3{p1[0], p1[1], p1[2]}; cc.L3 = V3{p1[3], p1[4], p1[5]};
       cc.L4 = V3{p2[0], p2[1], p2[2]}; cc.L5 = V3{p2[3], p2[4], p2[5]};
@@ -285,14 +314,27 @@ __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, con
     const int32_t c0 = (int32_t)a0.s, c1 = (int32_t)a1.s, c2 = (int32_t)a2.s;
     const int32_t ct = (int32_t)at.s;
     if (f.cells) {   // field-uniform branch
-      const float4* r0 = f.cells + 8 * cell_record(f, ct, c0, c1, c2);
+      const float4* r0 = f.idx32 ? f.cells + 8u * cell_record32(f, ct, c0, c1, c2)
+                                 : f.cells + 8 * cell_record(f, ct, c0, c1, c2);
       const float4* r1 = r0 + 8 * f.cell_slice;
       load_cell(r0, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
       load_cell(r1, cc.M0, cc.M1, cc.M2, cc.M3, cc.M4, cc.M5, cc.M6, cc.M7);
     } else {
-      const int64_t n2 = f.n2, n12 = (int64_t)f.n1 * f.n2;
-      const int64_t b = (int64_t)c0 * n12 + (int64_t)c1 * n2 + c2;
-      const float* s0 = f.data + 3 * (int64_t)ct * f.slice;
+      int64_t n2, n12, b;
+      const float* s0;
+      if (f.idx32) {   // field-uniform branch
+        const uint32_t u2 = (uint32_t)f.n2, u12 = (uint32_t)f.n1 * u2;
+        n2 = u2;
+        n12 = u12;
+        b = 0;
+        s0 = f.data + 3u * ((uint32_t)ct * (uint32_t)f.slice + (uint32_t)c0 * u12 +
+                            (uint32_t)c1 * u2 + (uint32_t)c2);
+      } else {
+        n2 = f.n2;
+        n12 = (int64_t)f.n1 * f.n2;
+        b = (int64_t)c0 * n12 + (int64_t)c1 * n2 + c2;
+        s0 = f.data + 3 * (int64_t)ct * f.slice;
+      }
       const float* s1 = s0 + 3 * f.slice;
       cc.L0 = node(f, s0, b);            cc.M0 = node(f, s1, b);
       cc.L1 = node(f, s0, b + 1);        cc.M1 = node(f, s1, b + 1);
