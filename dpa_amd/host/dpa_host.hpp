@@ -124,14 +124,24 @@ class particle_advector {
     std::vector<particle_3d> inactive_particles;
     dpa_hip::integral_curves integral_curves;
   };
+  // One round of dpa's loop (pipeline.cpp:100-146): the active particles at
+  // its start ("load") and the wall time of its stages in milliseconds.
+  struct round_record {
+    double load = 0;
+    double load_balancing_ms = 0, advection_ms = 0, communication_ms = 0, time_ms = 0;
+  };
 
   particle_advector(device& dev, std::size_t particles_per_round, const std::string& load_balancer,
                     const std::string& integrator, scalar step_size, bool gather_particles,
                     bool record, bool use_64_bit_indices = false);
 
   // Advects and consumes `particles` (particle_advector.cpp:39-55, all rounds),
-  // with indices and colours already generated.
-  output advect(const resident_field& field, std::vector<particle_3d>& particles);
+  // with indices and colours already generated.  The rounds run as dpa's loop
+  // does, the tail of the active particles first, one library call each; a
+  // record per round goes to `rounds` when it is given.  inactive_particles
+  // keeps the seed order.
+  output advect(const resident_field& field, std::vector<particle_3d>& particles,
+                std::vector<round_record>* rounds = nullptr);
 
  private:
   device& dev_;

@@ -4,6 +4,7 @@
 // benchmark.hpp:106-116).
 #include <filesystem>
 #include <fstream>
+#include <limits>
 #include <sstream>
 #include <string>
 
@@ -332,7 +333,7 @@ void session_recorder::set(const std::string& name, double value) {
 // mpi_session::to_csv (benchmark.hpp:106-116) with one rank and one iteration.
 void session_recorder::to_csv(const std::string& filepath, int rank) const {
   std::ofstream s(filepath);
-  s.precision(9);
+  s.precision(std::numeric_limits<double>::max_digits10);
   s << "rank,name,iteration 0,\n";
   for (auto& r : records) s << rank << "," << r.first << "," << r.second << ",\n";
 }

@@ -18,7 +18,6 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 
 int pipeline::run_config(const arguments& a, int device_ordinal) {
   session_recorder recorder;
-  const auto t_total = std::chrono::steady_clock::now();
   std::cout << "Started pipeline on one GPU\n";
 
   std::cout << "data_loading\n";
@@ -53,7 +52,6 @@ int pipeline::run_config(const arguments& a, int device_ordinal) {
         offset, size, d(rng), a.seed_generation_iterations, 0, a.seed_generation_boundaries,
         a.seed_generation_random_seed);
   }
-  const std::size_t seeds = particles.size();
 
   std::cout << "particle_advection\n";
   // pipeline.cpp:155: 64-bit indices iff particles_per_round * iterations > 2^32 - 1
@@ -65,15 +63,21 @@ int pipeline::run_config(const arguments& a, int device_ordinal) {
                              a.particle_advector_load_balancer, a.particle_advector_integrator,
                              a.particle_advector_step_size, a.particle_advector_gather_particles,
                              a.particle_advector_record, use_64);
-  const auto t_adv = std::chrono::steady_clock::now();
-  particle_advector::output output = advector.advect(resident, particles);
-  const double adv_ms = ms_since(t_adv);
-  // one GPU pass covers every round; the CSV keeps dpa's record names
-  recorder.set("round.0.load", static_cast<double>(seeds));
-  recorder.set("round.0.load_balancing_time", 0.0);
-  recorder.set("round.0.advection_time", adv_ms);
-  recorder.set("round.0.communication_time", 0.0);
-  recorder.set("round.0.time", adv_ms);
+  // total_time covers the round loop only (pipeline.cpp:98-145); the records
+  // of each round follow dpa's names and order
+  std::vector<particle_advector::round_record> rounds;
+  const auto t_loop = std::chrono::steady_clock::now();
+  particle_advector::output output = advector.advect(resident, particles, &rounds);
+  const double loop_ms = ms_since(t_loop);
+  for (std::size_t r = 0; r < rounds.size(); ++r) {
+    const std::string k = "round." + std::to_string(r) + ".";
+    recorder.set(k + "load", rounds[r].load);
+    recorder.set(k + "load_balancing_time", rounds[r].load_balancing_ms);
+    recorder.set(k + "advection_time", rounds[r].advection_ms);
+    recorder.set(k + "communication_time", rounds[r].communication_ms);
+    recorder.set(k + "time", rounds[r].time_ms);
+  }
+  recorder.set("total_time", loop_ms);
 
   std::cout << "index_generation\n";
   if (a.particle_advector_record) index_generator::generate(output.integral_curves, use_64);
@@ -94,7 +98,6 @@ int pipeline::run_config(const arguments& a, int device_ordinal) {
     std::cout << "save_ftle_field\n";
     regular_grid_saver(a.output_dataset_filepath).save(ftle);
   }
-  recorder.set("total_time", ms_since(t_total));
   recorder.to_csv(a.output_dataset_filepath + ".benchmark.csv");
   return 0;
 }

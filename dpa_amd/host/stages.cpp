@@ -1,6 +1,7 @@
 // stages.cpp -- the stage classes of dpa_host.hpp over the C ABI.
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <cstring>
 #include <random>
 
@@ -47,61 +48,83 @@ particle_advector::particle_advector(device& dev, std::size_t particles_per_roun
       use_64_bit_(use_64_bit_indices) {}
 
 particle_advector::output particle_advector::advect(const resident_field& field,
-                                                    std::vector<particle_3d>& particles) {
+                                                    std::vector<particle_3d>& particles,
+                                                    std::vector<round_record>* rounds) {
+  using clock = std::chrono::steady_clock;
+  auto ms = [](clock::time_point a, clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
   const std::size_t n = particles.size();
-  std::vector<float> seeds(3 * n);
-  std::vector<uint64_t> remaining(n);
-  for (std::size_t i = 0; i < n; ++i) {
-    std::memcpy(&seeds[3 * i], particles[i].position.data(), 3 * sizeof(float));
-    remaining[i] = particles[i].remaining_iterations;
-  }
-  dpah_advect_params p{};
-  p.particles_per_round = particles_per_round_;
-  p.step_size = step_size_;
-  p.integrator = integrator_;
-  p.record = record_ ? 1 : 0;
-  p.use_64_bit_indices = use_64_bit_ ? 1 : 0;
-  dpah_result* r = nullptr;
-  check(dpah_advect(dev_.get(), field.get(), seeds.data(), remaining.data(), 0, n, &p, &r));
   output out;
-  try {
-    uint64_t rounds = 0;
-    check(dpah_result_round_count(r, &rounds));
-    for (uint64_t k = 0; k < rounds; ++k) {
-      uint64_t V = 0, I = 0;
-      int ib = 4;
-      check(dpah_result_round_sizes(r, k, &V, &I, &ib));
-      integral_curve c;
-      c.vertices.resize(V);
-      std::vector<scalar> colors(V);
-      if (ib == 8) {
-        std::vector<uint64_t> idx(I);
-        check(dpah_result_copy_round(r, k, c.vertices.data()->data(), colors.data(), idx.data()));
-        c.indices = std::move(idx);
-      } else {
-        std::vector<uint32_t> idx(I);
-        check(dpah_result_copy_round(r, k, c.vertices.data()->data(), colors.data(), idx.data()));
-        c.indices = std::move(idx);
+  out.inactive_particles.resize(n);
+  std::size_t active = n;
+  do {   // pipeline.cpp:100-146: at least one round, the tail of the particles first
+    round_record rec;
+    const auto t0 = clock::now();
+    rec.load = static_cast<double>(active);   // no load balancer: every active particle
+    const auto t1 = clock::now();
+    // compute_round_state, allocate / advect / prune (particle_advector.cpp:261-408):
+    // one call over the round's slice, which is one round of the library's loop
+    const std::size_t count = std::min(particles_per_round_, active);
+    const std::size_t first = active - count;
+    std::vector<float> seeds(3 * count);
+    std::vector<uint64_t> remaining(count);
+    for (std::size_t i = 0; i < count; ++i) {
+      std::memcpy(&seeds[3 * i], particles[first + i].position.data(), 3 * sizeof(float));
+      remaining[i] = particles[first + i].remaining_iterations;
+    }
+    dpah_advect_params p{};
+    p.particles_per_round = count ? count : 1;
+    p.step_size = step_size_;
+    p.integrator = integrator_;
+    p.record = record_ ? 1 : 0;
+    p.use_64_bit_indices = use_64_bit_ ? 1 : 0;
+    dpah_result* r = nullptr;
+    check(dpah_advect(dev_.get(), field.get(), seeds.data(), remaining.data(), 0, count, &p, &r));
+    try {
+      if (record_) {
+        uint64_t V = 0, I = 0;
+        int ib = 4;
+        check(dpah_result_round_sizes(r, 0, &V, &I, &ib));
+        integral_curve c;
+        c.vertices.resize(V);
+        std::vector<scalar> colors(V);
+        if (ib == 8) {
+          std::vector<uint64_t> idx(I);
+          check(dpah_result_copy_round(r, 0, c.vertices.data()->data(), colors.data(), idx.data()));
+          c.indices = std::move(idx);
+        } else {
+          std::vector<uint32_t> idx(I);
+          check(dpah_result_copy_round(r, 0, c.vertices.data()->data(), colors.data(), idx.data()));
+          c.indices = std::move(idx);
+        }
+        c.colors = std::move(colors);
+        out.integral_curves.push_back(std::move(c));
       }
-      c.colors = std::move(colors);
-      out.integral_curves.push_back(std::move(c));
+      std::vector<float> pos(3 * count);
+      std::vector<uint64_t> rem(count);
+      std::vector<uint8_t> status(count);
+      check(dpah_result_final_state(r, pos.data(), rem.data(), status.data()));
+      for (std::size_t i = 0; i < count; ++i) {
+        particle_3d q = particles[first + i];
+        std::memcpy(q.position.data(), &pos[3 * i], 3 * sizeof(float));
+        q.remaining_iterations = rem[i];
+        out.inactive_particles[first + i] = q;
+      }
+    } catch (...) {
+      dpah_result_free(r);
+      throw;
     }
-    std::vector<float> pos(3 * n);
-    std::vector<uint64_t> rem(n);
-    std::vector<uint8_t> status(n);
-    check(dpah_result_final_state(r, pos.data(), rem.data(), status.data()));
-    out.inactive_particles.reserve(n);
-    for (std::size_t i = 0; i < n; ++i) {
-      particle_3d q = particles[i];
-      std::memcpy(q.position.data(), &pos[3 * i], 3 * sizeof(float));
-      q.remaining_iterations = rem[i];
-      out.inactive_particles.push_back(q);
-    }
-  } catch (...) {
     dpah_result_free(r);
-    throw;
-  }
-  dpah_result_free(r);
+    const auto t2 = clock::now();
+    active = first;   // completion check (particle_advector.cpp:57-64): one rank
+    const auto t3 = clock::now();
+    rec.load_balancing_ms = ms(t0, t1);
+    rec.advection_ms = ms(t1, t2);
+    rec.communication_ms = ms(t2, t3);
+    rec.time_ms = ms(t0, t3);
+    if (rounds) rounds->push_back(rec);
+  } while (active > 0);
   particles.clear();   // advect consumes the active particles (particle_advector.cpp:398)
   return out;
 }

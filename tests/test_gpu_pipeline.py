@@ -72,4 +72,34 @@ def test_pipeline_curves_and_ftle(tmp_path):
     assert np.all(np.abs(got[fin] - want[fin]) <= 1e-3 * np.abs(want[fin]) + 1e-5 / 1.2)
     csv = (tmp_path / "out.h5.benchmark.csv").read_text().splitlines()
     assert csv[0] == "rank,name,iteration 0,"
-    assert any(l.startswith("0,total_time,") for l in csv)
+    bench = parse_benchmark(tmp_path / "out.h5.benchmark.csv")
+    rounds = bench["ranks"][0]["rounds"]
+    n, ppr = len(seeds), 300
+    assert len(rounds) == (n + ppr - 1) // ppr == len(ora["rounds"])
+    for r, rec in enumerate(rounds):
+        assert set(rec) == {"load", "load_balancing_time", "advection_time",
+                            "communication_time", "time"}
+        assert rec["load"] == n - r * ppr                  # active particles at the round's start
+        assert 0 < rec["advection_time"] <= rec["time"]
+    total = bench["ranks"][0]["total_time"]
+    assert sum(rec["time"] for rec in rounds) <= total * (1 + 1e-9)
+
+
+def parse_benchmark(path):
+    """The reading of utility/python/benchmark_parser.py:29-56: rows keyed by
+    rank and name, total_time per rank, round.<index>.<kind> per round."""
+    import csv
+    out = {"ranks": {}}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            rank, name, value = int(row["rank"]), row["name"], float(row["iteration 0"])
+            r = out["ranks"].setdefault(rank, {"rounds": []})
+            if name == "total_time":
+                r["total_time"] = value
+                continue
+            _, index, kind = name.split(".")
+            index = int(index)
+            while len(r["rounds"]) <= index:
+                r["rounds"].append({})
+            r["rounds"][index][kind] = value
+    return out
