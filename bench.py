@@ -180,6 +180,13 @@ def endpoint_loads(ctx, fld, sets, steps):
 
 # ------------------------------------------------ traffic (rocprofv3 --pmc)
 
+def under_profiler() -> bool:
+    """True inside a rocprofv3 run (its preloaded library is in the
+    environment), where a nested rocprofv3 must not start."""
+    return ("rocprof" in os.environ.get("LD_PRELOAD", "") or
+            any(k.startswith("ROCPROF") for k in os.environ))
+
+
 def _counter_rows(outdir: Path):
     import csv
     rows = []
@@ -215,6 +222,8 @@ def measure_traffic(args, kernel="k_advect_persistent"):
     prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not Path(prof).exists():
         return None, "rocprofv3 not found"
+    if under_profiler():
+        return None, "not measured: this run is itself under a profiler"
     out = {}
     env = dict(os.environ, TMPDIR="/tmp")
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
