@@ -305,16 +305,24 @@ def serve_trace(ctx, fields, trace, mode, steps_per_tick=0, saturate=False):
         failure = []
 
         def collector():
+            # frees every result as soon as it is ready, in completion order
             try:
-                i = 0
-                while i < len(trace):
-                    while i >= len(tickets):
-                        time.sleep(0.0005)
-                    t = tickets[i]
-                    server.wait(t, -1).free()
-                    results[i] = server.info(t)
-                    server.forget(t)
-                    i += 1
+                open_ = {}
+                seen = 0
+                while len(results) < len(trace):
+                    while seen < len(tickets):
+                        open_[seen] = tickets[seen]
+                        seen += 1
+                    freed = False
+                    for i, t in list(open_.items()):
+                        if server.poll(t) >= dpa_amd.TICKET_DONE:
+                            server.wait(t, -1).free()
+                            results[i] = server.info(t)
+                            server.forget(t)
+                            del open_[i]
+                            freed = True
+                    if not freed:
+                        time.sleep(0.0002)
             except Exception as e:   # surfaces in the caller
                 failure.append(e)
             finally:
@@ -361,21 +369,29 @@ def serve_trace(ctx, fields, trace, mode, steps_per_tick=0, saturate=False):
             "p50_device_done_ms": p50("device_seconds")}
 
 
-def serving_leg(ctx, fields, trace, saturate=False):
-    """The trace through the batched server and both comparators, results left
-    in device memory in all three."""
+SERVE_MODES = {"batched": "batched", "serial": "serial_library_calls",
+               "per-request": "per_request_launch"}
+
+
+def serving_leg(ctx, fields, trace, saturate=False, modes=tuple(SERVE_MODES)):
+    """The trace through the batched server and both comparators (those of
+    `modes`), results left in device memory in all of them."""
     import dpa_amd
+    code = {"batched": dpa_amd.SERVE_BATCHED, "serial": dpa_amd.SERVE_SERIAL,
+            "per-request": dpa_amd.SERVE_PER_REQUEST}
     out = {}
-    for key, mode in (("batched", dpa_amd.SERVE_BATCHED),
-                      ("serial_library_calls", dpa_amd.SERVE_SERIAL),
-                      ("per_request_launch", dpa_amd.SERVE_PER_REQUEST)):
-        r = serve_trace(ctx, fields, trace, mode, saturate=saturate)
+    for m in modes:
+        key = SERVE_MODES[m]
+        r = serve_trace(ctx, fields, trace, code[m], saturate=saturate)
         out[key] = r
         log(f"{'saturated' if saturate else 'poisson'} {key:22s} {r['throughput']:.3e} steps/s "
             f"makespan {r['makespan_s']:.2f} s p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms")
-    b = out["batched"]["throughput"]
-    out["batched_vs_per_request"] = b / max(out["per_request_launch"]["throughput"], 1e-9)
-    out["batched_vs_serial"] = b / max(out["serial_library_calls"]["throughput"], 1e-9)
+    if "batched" in out and "per_request_launch" in out:
+        out["batched_vs_per_request"] = (out["batched"]["throughput"] /
+                                         max(out["per_request_launch"]["throughput"], 1e-9))
+    if "batched" in out and "serial_library_calls" in out:
+        out["batched_vs_serial"] = (out["batched"]["throughput"] /
+                                    max(out["serial_library_calls"]["throughput"], 1e-9))
     return out
 
 
@@ -400,9 +416,11 @@ def serving_benchmark(ctx, base, args):
                          "+-0.5 cell, 200 steps, dt 0.02); results left in device memory "
                          "in every mode"),
                "ftle_requests": sum(r["kind"] == "ftle" for r in trace),
-               "poisson": dict(rate=args.serve_rate, **serving_leg(ctx, fields, trace))}
+               "poisson": dict(rate=args.serve_rate,
+                               **serving_leg(ctx, fields, trace, modes=args.serve_modes))}
         if args.serve_saturate:
-            out["saturated"] = serving_leg(ctx, fields, trace, saturate=True)
+            out["saturated"] = serving_leg(ctx, fields, trace, saturate=True,
+                                           modes=args.serve_modes)
         return out
     finally:
         for f in fields:
@@ -784,18 +802,19 @@ def run_serve(args):
     sweep = []
     for rate in rates:
         trace = W.serving_trace(rate, args.serve_requests)
-        leg = serving_leg(ctx, fields, trace)
+        leg = serving_leg(ctx, fields, trace, modes=args.serve_modes)
         leg["rate"] = rate
         sweep.append(leg)
     if args.serve_saturate:
         leg = serving_leg(ctx, fields, W.serving_trace(rates[-1], args.serve_requests),
-                          saturate=True)
+                          saturate=True, modes=args.serve_modes)
         leg["rate"] = "saturated"
         sweep.append(leg)
     for f in fields:
         f.release()
     ctx.close()
-    best = max(s["batched"]["throughput"] for s in sweep)
+    best = max(max(v["throughput"] for k, v in s.items() if isinstance(v, dict))
+               for s in sweep)
     out = base_line(args, best, 0.0, "f32")
     out["steps"] = 0
     out["config"] = {"workload": "served pathlines + FTLE slabs on 4 resident 256^3 x 16 "
@@ -825,6 +844,9 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--no-serve-saturate", dest="serve_saturate", action="store_false",
                     help="skip the saturation trace (every request submitted at once)")
+    ap.add_argument("--serve-modes", type=lambda s: tuple(s.split(",")),
+                    default=tuple(SERVE_MODES),
+                    help="serving modes to run: batched,serial,per-request (default all)")
     ap.add_argument("--rates", default="50,100,200,500")
     ap.add_argument("--order", type=int, choices=[0, 1], default=None,
                     help="spatially ordered, XCD-split work queue (default: library's)")

@@ -157,7 +157,7 @@ struct dpah_server {
   void launch_ftle(const std::shared_ptr<Request>& rp, int sid);
   bool launch_ready();
   bool poll_launches();
-  void finish(Request& r, hipStream_t s);
+  void finish_all(std::vector<std::shared_ptr<Request>>& done, hipStream_t s);
   void finish_ftle(Launch& L, hipStream_t s);
   void release_request(const std::shared_ptr<Request>& r, int state, int err,
                        const std::string& msg);
@@ -240,6 +240,13 @@ bool dpah_server::admit_locked() {
       continue;
     }
     if (free_slots.empty() || in_use + r->footprint > budget) break;
+    // and beside everything else the context holds, the finished results the
+    // caller has not freed yet included (admission is FIFO, so an admitted
+    // request always finishes and frees its place; a request larger than the
+    // context's budget goes ahead when nothing else runs and fails alone)
+    if (ctx->bytes_in_use.load() + r->footprint > ctx->memory_budget &&
+        free_slots.size() < opt.max_active)
+      break;
     r->slot = free_slots.back();
     free_slots.pop_back();
     in_use += r->footprint;
@@ -343,14 +350,23 @@ bool dpah_server::launch_ready() {
     for (int i = 0; i < kStreams; ++i)
       if (!launches[i].busy) { sid = i; break; }
     if (sid < 0) break;
-    if (persistent_ftle(*ready.front())) {
-      auto rp = ready.front();
-      ready.erase(ready.begin());
+    // one persistent FTLE launch at a time: each is sized to fill the chip on
+    // its own, and several at once share the L2 between distant lattices;
+    // the other streams take the next tick of pathlines meanwhile
+    bool ftle_busy = false;
+    for (int i = 0; i < kStreams; ++i) ftle_busy |= launches[i].busy && launches[i].ftle;
+    size_t pick = ready.size();
+    for (size_t i = 0; i < ready.size(); ++i)
+      if (!persistent_ftle(*ready[i]) || !ftle_busy) { pick = i; break; }
+    if (pick == ready.size()) break;
+    if (persistent_ftle(*ready[pick])) {
+      auto rp = ready[pick];
+      ready.erase(ready.begin() + (std::ptrdiff_t)pick);
       launch_ftle(rp, sid);
       launched = true;
       continue;
     }
-    const Request& head = *ready.front();
+    const Request& head = *ready[pick];
     const bool rec0 = head.spec.kind == DPAH_REQUEST_ADVECT && head.spec.params.record;
     const auto key = std::make_tuple(head.spec.field, head.spec.params.integrator, rec0);
     Launch& L = launches[sid];
@@ -463,6 +479,7 @@ bool dpah_server::poll_launches() {
     DPAH_HIP(hipStreamSynchronize(s));
     ctx->resolve_timings();
     const double t_seen = now_s();
+    std::vector<std::shared_ptr<Request>> finished;
     for (auto& rp : L.reqs) {
       Request& r = *rp;
       r.t_device = t_seen;
@@ -474,15 +491,15 @@ bool dpah_server::poll_launches() {
       if (cancelled) {
         release_request(rp, kCancelled, DPAH_ERR_CANCELLED, "");
       } else if (r.left_h == 0) {
-        try {
-          finish(r, s);
-          release_request(rp, kDone, DPAH_OK, "");
-        } catch (const Error& e) {
-          release_request(rp, kFailed, e.code, e.what());
-        }
+        finished.push_back(rp);
       } else {
         ready.push_back(rp);   // step budget used up: continue in a later launch
       }
+    }
+    finish_all(finished, s);
+    for (auto& rp : finished) {
+      if (rp->result) release_request(rp, kDone, DPAH_OK, "");
+      else release_request(rp, kFailed, rp->error ? rp->error : DPAH_ERR_HIP, rp->message);
     }
     L.reqs.clear();
     L.segs.reset();
@@ -492,63 +509,127 @@ bool dpah_server::poll_launches() {
   return progressed;
 }
 
-void dpah_server::finish(Request& r, hipStream_t s) {
-  const bool ftle = r.spec.kind == DPAH_REQUEST_FTLE_AUX;
-  auto res = std::make_unique<dpah_result>();
-  res->ctx = ctx;
-  const uint64_t n = r.count;
-  DPAH_HIP(hipMemcpyAsync(&r.steps_h, d_steps + r.slot, sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s));
-  if (ftle) {
-    const uint64_t P = ftle_points(r.spec);
-    res->ftle_count = P;
-    res->d_ftle = static_cast<float*>(ctx->alloc(std::max<uint64_t>(P, 1) * sizeof(float)));
-    FtleAuxArgs a{};
-    a.finals = r.pos.as<float>();
-    a.points = P;
-    a.twod0 = 2.0f * r.spec.delta[0]; a.twod1 = 2.0f * r.spec.delta[1];
-    a.twod2 = 2.0f * r.spec.delta[2];
-    a.T_abs = std::fabs(r.spec.params.step_size * (float)r.spec.iterations);
-    a.out = res->d_ftle;
-    if (P)
-      hipLaunchKernelGGL(k_ftle_aux_reduce, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s,
-                         a);
-    DPAH_HIP(hipGetLastError());
-  } else {
-    res->use64 = r.spec.params.use_64_bit_indices ? 1 : 0;
-    res->final_pos.resize(3 * n);
-    res->final_rem.resize(n);
-    res->final_status.resize(n);
-    if (n) {
-      DPAH_HIP(hipMemcpyAsync(res->final_pos.data(), r.pos.ptr, 3 * n * sizeof(float),
+// The results of every finished request of one launch, in two phases with one
+// synchronisation each: first every request's step count, final states (or
+// FTLE reduction) and every recorded round's vertex / index counts and scan,
+// then every round's writes, sized by the totals.  A request that fails keeps
+// no result and carries its error.
+void dpah_server::finish_all(std::vector<std::shared_ptr<Request>>& done, hipStream_t s) {
+  struct RoundCounts {
+    size_t req = 0, round = 0;
+    DeviceBuffer vcount, icount, sums;
+    uint64_t totals[2] = {0, 0};
+  };
+  std::vector<std::unique_ptr<dpah_result>> res(done.size());
+  std::vector<RoundCounts> counts;
+  auto fail = [&](size_t i, const Error& e) {
+    res[i].reset();
+    done[i]->error = e.code;
+    done[i]->message = e.what();
+  };
+  auto rows_of = [&](const Request& r, const RoundState& rs) {
+    return CompactRows{rs.slots.as<float>(), rs.stride, r.it.as<uint32_t>() + rs.first,
+                       r.special.as<uint8_t>() + rs.first, rs.count, 0,
+                       r.spec.params.use_64_bit_indices ? 1 : 0, 0u};
+  };
+  for (size_t i = 0; i < done.size(); ++i) {
+    Request& r = *done[i];
+    try {
+      const bool ftle = r.spec.kind == DPAH_REQUEST_FTLE_AUX;
+      res[i] = std::make_unique<dpah_result>();
+      dpah_result& o = *res[i];
+      o.ctx = ctx;
+      const uint64_t n = r.count;
+      DPAH_HIP(hipMemcpyAsync(&r.steps_h, d_steps + r.slot, sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, s));
-      DPAH_HIP(hipMemcpyAsync(res->final_rem.data(), r.rem_d.ptr, n * sizeof(uint64_t),
-                              hipMemcpyDeviceToHost, s));
-      DPAH_HIP(hipMemcpyAsync(res->final_status.data(), r.status.ptr, n, hipMemcpyDeviceToHost,
-                              s));
-    }
-    if (r.spec.params.record) {
-      for (RoundState& rs : r.rounds) {
-        Round round;
-        if (rs.count) {
-          Chunk ch = compact_rows(ctx, rs.slots.as<float>(), rs.stride, 0,
-                                  r.it.as<uint32_t>() + rs.first,
-                                  r.special.as<uint8_t>() + rs.first, rs.count, 0, res->use64, s);
-          round.V = ch.V;
-          round.I = ch.I;
-          round.chunks.push_back(ch);
-          if (!res->use64 && round.V > 0xffffffffull)
-            throw Error(DPAH_ERR_INVALID, "round has more than 2^32-1 vertices");
-        }
-        res->rounds.push_back(std::move(round));
+      if (ftle) {
+        const uint64_t P = ftle_points(r.spec);
+        o.ftle_count = P;
+        o.d_ftle = static_cast<float*>(ctx->alloc(std::max<uint64_t>(P, 1) * sizeof(float)));
+        FtleAuxArgs a{};
+        a.finals = r.pos.as<float>();
+        a.points = P;
+        a.twod0 = 2.0f * r.spec.delta[0]; a.twod1 = 2.0f * r.spec.delta[1];
+        a.twod2 = 2.0f * r.spec.delta[2];
+        a.T_abs = std::fabs(r.spec.params.step_size * (float)r.spec.iterations);
+        a.out = o.d_ftle;
+        if (P)
+          hipLaunchKernelGGL(k_ftle_aux_reduce, dim3((unsigned)((P + 255) / 256)), dim3(256), 0,
+                             s, a);
+        DPAH_HIP(hipGetLastError());
+        continue;
       }
+      o.use64 = r.spec.params.use_64_bit_indices ? 1 : 0;
+      o.final_pos.resize(3 * n);
+      o.final_rem.resize(n);
+      o.final_status.resize(n);
+      if (n) {
+        DPAH_HIP(hipMemcpyAsync(o.final_pos.data(), r.pos.ptr, 3 * n * sizeof(float),
+                                hipMemcpyDeviceToHost, s));
+        DPAH_HIP(hipMemcpyAsync(o.final_rem.data(), r.rem_d.ptr, n * sizeof(uint64_t),
+                                hipMemcpyDeviceToHost, s));
+        DPAH_HIP(hipMemcpyAsync(o.final_status.data(), r.status.ptr, n, hipMemcpyDeviceToHost,
+                                s));
+      }
+      if (!r.spec.params.record) continue;
+      for (size_t k = 0; k < r.rounds.size(); ++k) {
+        const RoundState& rs = r.rounds[k];
+        if (!rs.count) continue;
+        RoundCounts c;
+        c.req = i;
+        c.round = k;
+        c.vcount = DeviceBuffer(ctx, rs.count * sizeof(uint64_t));
+        c.icount = DeviceBuffer(ctx, rs.count * sizeof(uint64_t));
+        const uint64_t words = compact_sums_words(rs.count);
+        c.sums = DeviceBuffer(ctx, words * sizeof(uint64_t));
+        compact_count(ctx, rows_of(r, rs), c.vcount.as<uint64_t>(), c.icount.as<uint64_t>(),
+                      c.sums.as<uint64_t>(), s);
+        counts.push_back(std::move(c));
+      }
+    } catch (const Error& e) {
+      fail(i, e);
     }
   }
+  for (RoundCounts& c : counts) {   // the totals sit in the last two words of each scan
+    const uint64_t words = compact_sums_words(done[c.req]->rounds[c.round].count);
+    DPAH_HIP(hipMemcpyAsync(c.totals, c.sums.as<uint64_t>() + words - 2, 2 * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+  }
   DPAH_HIP(hipStreamSynchronize(s));
-  res->total_steps = r.steps_h;
-  r.steps = r.steps_h;
-  if (opt.results_on_host) materialize_host(res.get(), s);
-  r.result = res.release();
+  // every round of a request, empty ones too, in the request's round order
+  for (size_t i = 0; i < done.size(); ++i)
+    if (res[i] && done[i]->spec.kind == DPAH_REQUEST_ADVECT && done[i]->spec.params.record)
+      res[i]->rounds.resize(done[i]->rounds.size());
+  for (RoundCounts& c : counts) {
+    if (!res[c.req]) continue;
+    Request& r = *done[c.req];
+    try {
+      if (!res[c.req]->use64 && c.totals[0] > 0xffffffffull)
+        throw Error(DPAH_ERR_INVALID, "round has more than 2^32-1 vertices");
+      Chunk ch = compact_write(ctx, rows_of(r, r.rounds[c.round]), c.vcount.as<uint64_t>(),
+                               c.icount.as<uint64_t>(), c.totals[0], c.totals[1], s);
+      Round& round = res[c.req]->rounds[c.round];
+      round.V = ch.V;
+      round.I = ch.I;
+      round.chunks.push_back(ch);
+    } catch (const Error& e) {
+      fail(c.req, e);
+    }
+  }
+  DPAH_HIP(hipStreamSynchronize(s));   // the chunks are written before the counts are freed
+  counts.clear();
+  for (size_t i = 0; i < done.size(); ++i) {
+    if (!res[i]) continue;
+    Request& r = *done[i];
+    res[i]->total_steps = r.steps_h;
+    r.steps = r.steps_h;
+    try {
+      if (opt.results_on_host) materialize_host(res[i].get(), s);
+      r.result = res[i].release();
+    } catch (const Error& e) {
+      fail(i, e);
+    }
+  }
 }
 
 // FTLE requests of a batched server that runs particles to retirement: each
@@ -660,6 +741,10 @@ void dpah_server::finish_ftle(Launch& L, hipStream_t s) {
 
 void dpah_server::run_batched() {
   while (true) {
+    {   // results the caller freed while a call held the context count as free again
+      std::lock_guard<std::recursive_mutex> ctx_lk(ctx->mu);
+      ctx->drain_deferred();
+    }
     bool busy = false, free_stream = false;
     for (auto& L : launches) {
       busy |= L.busy;
