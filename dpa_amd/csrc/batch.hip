@@ -238,40 +238,44 @@ struct GroupRun {
   }
 
   void finish(dpah_context* ctx, std::vector<std::unique_ptr<dpah_result>>& out) const {
-    const uint64_t n1 = std::max<uint64_t>(N, 1);
-    std::vector<uint64_t> h_sums(sums_total);
+    // totals and final states in one read-back: [sums | remaining | positions
+    // | status], in the context's pinned staging when it fits
+    const uint64_t bytes = 8 * sums_total + 21 * N;
+    std::unique_ptr<uint8_t[]> heap;
+    uint8_t* stage = static_cast<uint8_t*>(ctx->host_staging(std::max<uint64_t>(bytes, 1)));
+    if (!stage) {
+      heap.reset(new uint8_t[bytes]);
+      stage = heap.get();
+    }
+    const uint64_t* h_sums = reinterpret_cast<const uint64_t*>(stage);
+    const uint64_t* h_rem = reinterpret_cast<const uint64_t*>(stage + 8 * sums_total);
+    const float* h_pos = reinterpret_cast<const float*>(stage + 8 * sums_total + 8 * N);
+    const uint8_t* h_status = stage + 8 * sums_total + 20 * N;
     if (record && rows_total)
-      DPAH_HIP(hipMemcpyAsync(h_sums.data(), d_sums.ptr, sums_total * sizeof(uint64_t),
+      DPAH_HIP(hipMemcpyAsync(stage, d_sums.ptr, sums_total * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, ctx->stream));
-    // staging without zero-filling (every element is overwritten)
-    std::unique_ptr<float[]> h_pos(new float[3 * n1]);
-    std::unique_ptr<uint64_t[]> h_rem_out(new uint64_t[n1]);
-    std::unique_ptr<uint8_t[]> h_status(new uint8_t[n1]);
-    DPAH_HIP(hipMemcpyAsync(h_pos.get(), d_pos.ptr, 3 * N * sizeof(float),
-                            hipMemcpyDeviceToHost, ctx->stream));
-    DPAH_HIP(hipMemcpyAsync(h_rem_out.get(), d_rem_out.ptr, N * sizeof(uint64_t),
-                            hipMemcpyDeviceToHost, ctx->stream));
-    DPAH_HIP(hipMemcpyAsync(h_status.get(), d_status.ptr, N, hipMemcpyDeviceToHost,
-                            ctx->stream));
+    if (N) {
+      DPAH_HIP(hipMemcpyAsync(stage + 8 * sums_total, d_rem_out.ptr, N * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+      DPAH_HIP(hipMemcpyAsync(stage + 8 * sums_total + 8 * N, d_pos.ptr, 3 * N * sizeof(float),
+                              hipMemcpyDeviceToHost, ctx->stream));
+      DPAH_HIP(hipMemcpyAsync(stage + 8 * sums_total + 20 * N, d_status.ptr, N,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    }
     ctx->sync();
 
+    // every round's writes first, so the device works while the host copies
+    // the final states out of the staging below
     for (const JobPlan& p : plans) {
       auto res = std::make_unique<dpah_result>();
       res->ctx = ctx;
       res->use64 = p.use64;
-      res->final_pos.assign(h_pos.get() + 3 * p.base, h_pos.get() + 3 * (p.base + p.count));
-      res->final_rem.assign(h_rem_out.get() + p.base, h_rem_out.get() + p.base + p.count);
-      res->final_status.assign(h_status.get() + p.base, h_status.get() + p.base + p.count);
-      uint64_t left = 0;
-      for (uint64_t i = 0; i < p.count; ++i) left += res->final_rem[i];
-      res->total_steps = p.rem_sum - left;
       if (record) {
         for (const RoundPlan& r : p.rounds) {
           Round round;
           if (r.count) {
             const size_t s = (size_t)r.seg;
-            const uint64_t* tot =
-                h_sums.data() + sums_off[s] + compact_sums_words(segs[s].count) - 2;
+            const uint64_t* tot = h_sums + sums_off[s] + compact_sums_words(segs[s].count) - 2;
             if (!res->use64)
               DPAH_REQUIRE(tot[0] <= 0xffffffffull,
                            "round has more than 2^32-1 vertices; 32-bit indices cannot address it");
@@ -286,6 +290,15 @@ struct GroupRun {
         }
       }
       out[p.job] = std::move(res);
+    }
+    for (const JobPlan& p : plans) {
+      dpah_result* res = out[p.job].get();
+      res->final_pos.assign(h_pos + 3 * p.base, h_pos + 3 * (p.base + p.count));
+      res->final_rem.assign(h_rem + p.base, h_rem + p.base + p.count);
+      res->final_status.assign(h_status + p.base, h_status + p.base + p.count);
+      uint64_t left = 0;
+      for (uint64_t i = 0; i < p.count; ++i) left += res->final_rem[i];
+      res->total_steps = p.rem_sum - left;
     }
     if (record) ctx->sync();   // the chunks are complete when the results are handed out
   }

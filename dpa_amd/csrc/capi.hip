@@ -52,6 +52,24 @@ void dpah_context::drain_deferred() {
   for (dpah_result* r : take) delete r;
 }
 
+void* dpah_context::host_staging(uint64_t bytes) {
+  if (bytes > kPinnedStagingMax) return nullptr;
+  if (bytes > pinned_bytes) {
+    if (pinned) {
+      DPAH_HIP(hipStreamSynchronize(stream));
+      (void)hipHostFree(pinned);
+      pinned = nullptr;
+      pinned_bytes = 0;
+    }
+    // 2 MiB granules, at least 8 MiB, so that small reads never regrow it
+    const uint64_t want = std::max<uint64_t>((bytes + (2ull << 20) - 1) & ~((2ull << 20) - 1),
+                                             8ull << 20);
+    DPAH_HIP(hipHostMalloc(&pinned, want, hipHostMallocDefault));
+    pinned_bytes = want;
+  }
+  return pinned;
+}
+
 void* dpah_context::alloc(uint64_t bytes) {
   drain_deferred();
   const uint64_t want = round_block(bytes);
@@ -235,6 +253,7 @@ DPAH_EXPORT int dpah_destroy(dpah_context* ctx) {
     if (ctx->d_counters) ctx->release(ctx->d_counters, dpah::kCounterCount * sizeof(unsigned long long));
     ctx->trim_cache();
     for (auto& kv : ctx->block_size) (void)hipFree(kv.first);   // blocks never released
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });
@@ -636,15 +655,24 @@ dpah_result* dpah::run_advect(dpah_context* ctx, const dpah_field* field, const 
         active = first;
       } while (active > 0);
     }
+    uint8_t* stage = count ? static_cast<uint8_t*>(ctx->host_staging(21 * count)) : nullptr;
     if (count) {
-      DPAH_HIP(hipMemcpyAsync(res->final_pos.data(), d_pos.ptr, 3 * count * sizeof(float),
-                              hipMemcpyDeviceToHost, ctx->stream));
-      DPAH_HIP(hipMemcpyAsync(res->final_rem.data(), d_rem_out.ptr, count * sizeof(uint64_t),
-                              hipMemcpyDeviceToHost, ctx->stream));
-      DPAH_HIP(hipMemcpyAsync(res->final_status.data(), d_status.ptr, count, hipMemcpyDeviceToHost,
+      // final states through the pinned staging when they fit
+      void* pos = stage ? static_cast<void*>(stage + 8 * count) : res->final_pos.data();
+      void* rem = stage ? static_cast<void*>(stage) : res->final_rem.data();
+      void* st = stage ? static_cast<void*>(stage + 20 * count) : res->final_status.data();
+      DPAH_HIP(hipMemcpyAsync(pos, d_pos.ptr, 3 * count * sizeof(float), hipMemcpyDeviceToHost,
                               ctx->stream));
+      DPAH_HIP(hipMemcpyAsync(rem, d_rem_out.ptr, count * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+      DPAH_HIP(hipMemcpyAsync(st, d_status.ptr, count, hipMemcpyDeviceToHost, ctx->stream));
     }
     ctx->sync();
+    if (stage) {
+      std::memcpy(res->final_rem.data(), stage, 8 * count);
+      std::memcpy(res->final_pos.data(), stage + 8 * count, 12 * count);
+      std::memcpy(res->final_status.data(), stage + 20 * count, count);
+    }
     res->total_steps = total_steps;
     return res.release();
   }

@@ -159,6 +159,15 @@ struct dpah_context {
   std::mutex deferred_mu;
   std::vector<dpah_result*> deferred;
   void drain_deferred();   // call with mu held
+  // Pinned host memory for reading final states back: hipMemcpyAsync into
+  // pageable memory goes through the runtime's small staging buffers at a few
+  // GB/s and blocks the host meanwhile.  Grown on demand up to
+  // kPinnedStagingMax (larger reads go to their pageable destination
+  // directly); used with mu held and copied out before the call returns.
+  static constexpr uint64_t kPinnedStagingMax = 256ull << 20;
+  void* pinned = nullptr;
+  uint64_t pinned_bytes = 0;
+  void* host_staging(uint64_t bytes);   // nullptr above kPinnedStagingMax
   void* alloc(uint64_t bytes);
   void release(void* p, uint64_t bytes);
   hipEvent_t take_event();
