@@ -35,6 +35,14 @@ using namespace dpah;
 
 namespace {
 
+// A new request's device counters: particles not yet retired, particle-steps.
+__global__ void k_request_counters(unsigned int* left, unsigned long long* steps, unsigned int n) {
+  if (threadIdx.x == 0) {
+    *left = n;
+    *steps = 0;
+  }
+}
+
 double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -332,11 +340,10 @@ void dpah_server::setup(Request& r, hipStream_t s) {
   }
   DPAH_HIP(hipGetLastError());
   r.left_h = (unsigned int)n;
-  DPAH_HIP(hipMemcpyAsync(d_left + r.slot, &r.left_h, sizeof(unsigned int), hipMemcpyHostToDevice,
-                          s));
   r.steps_h = 0;
-  DPAH_HIP(hipMemcpyAsync(d_steps + r.slot, &r.steps_h, sizeof(unsigned long long),
-                          hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_request_counters, dim3(1), dim3(64), 0, s, d_left + r.slot,
+                     d_steps + r.slot, (unsigned int)n);
+  DPAH_HIP(hipGetLastError());
   r.set_up = true;
 }
 
@@ -473,10 +480,13 @@ bool dpah_server::poll_launches() {
       progressed = true;
       continue;
     }
-    for (auto& rp : L.reqs)
-      DPAH_HIP(hipMemcpyAsync(&rp->left_h, d_left + rp->slot, sizeof(unsigned int),
-                              hipMemcpyDeviceToHost, s));
+    // every slot's counter in one read (the engine thread holds the context,
+    // so its pinned staging is free)
+    auto* left = static_cast<unsigned int*>(ctx->host_staging(opt.max_active * sizeof(unsigned int)));
+    DPAH_HIP(hipMemcpyAsync(left, d_left, opt.max_active * sizeof(unsigned int),
+                            hipMemcpyDeviceToHost, s));
     DPAH_HIP(hipStreamSynchronize(s));
+    for (auto& rp : L.reqs) rp->left_h = left[rp->slot];
     ctx->resolve_timings();
     const double t_seen = now_s();
     std::vector<std::shared_ptr<Request>> finished;
@@ -532,6 +542,22 @@ void dpah_server::finish_all(std::vector<std::shared_ptr<Request>>& done, hipStr
                        r.special.as<uint8_t>() + rs.first, rs.count, 0,
                        r.spec.params.use_64_bit_indices ? 1 : 0, 0u};
   };
+  // phase one's reads go to the context's pinned staging (the engine thread
+  // holds the context): every slot's step count, then each advection
+  // request's [remaining u64[n] | positions f32[3n] | status u8[n]]; requests
+  // past the staging's limit read into their result vectors directly
+  const uint64_t steps_bytes = (uint64_t)opt.max_active * sizeof(unsigned long long);
+  std::vector<uint64_t> stage_off(done.size(), 0);
+  uint64_t stage_bytes = steps_bytes;
+  for (size_t i = 0; i < done.size(); ++i) {
+    if (done[i]->spec.kind != DPAH_REQUEST_ADVECT) continue;
+    stage_off[i] = stage_bytes;
+    stage_bytes += (21 * done[i]->count + 7) / 8 * 8;
+  }
+  uint8_t* stage = static_cast<uint8_t*>(ctx->host_staging(stage_bytes));
+  const bool staged = stage != nullptr;
+  if (!staged) stage = static_cast<uint8_t*>(ctx->host_staging(steps_bytes));
+  DPAH_HIP(hipMemcpyAsync(stage, d_steps, steps_bytes, hipMemcpyDeviceToHost, s));
   for (size_t i = 0; i < done.size(); ++i) {
     Request& r = *done[i];
     try {
@@ -540,8 +566,6 @@ void dpah_server::finish_all(std::vector<std::shared_ptr<Request>>& done, hipStr
       dpah_result& o = *res[i];
       o.ctx = ctx;
       const uint64_t n = r.count;
-      DPAH_HIP(hipMemcpyAsync(&r.steps_h, d_steps + r.slot, sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, s));
       if (ftle) {
         const uint64_t P = ftle_points(r.spec);
         o.ftle_count = P;
@@ -564,12 +588,13 @@ void dpah_server::finish_all(std::vector<std::shared_ptr<Request>>& done, hipStr
       o.final_rem.resize(n);
       o.final_status.resize(n);
       if (n) {
-        DPAH_HIP(hipMemcpyAsync(o.final_pos.data(), r.pos.ptr, 3 * n * sizeof(float),
-                                hipMemcpyDeviceToHost, s));
-        DPAH_HIP(hipMemcpyAsync(o.final_rem.data(), r.rem_d.ptr, n * sizeof(uint64_t),
-                                hipMemcpyDeviceToHost, s));
-        DPAH_HIP(hipMemcpyAsync(o.final_status.data(), r.status.ptr, n, hipMemcpyDeviceToHost,
-                                s));
+        uint8_t* h = stage + stage_off[i];
+        DPAH_HIP(hipMemcpyAsync(staged ? static_cast<void*>(h + 8 * n) : o.final_pos.data(),
+                                r.pos.ptr, 3 * n * sizeof(float), hipMemcpyDeviceToHost, s));
+        DPAH_HIP(hipMemcpyAsync(staged ? static_cast<void*>(h) : o.final_rem.data(), r.rem_d.ptr,
+                                n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        DPAH_HIP(hipMemcpyAsync(staged ? static_cast<void*>(h + 20 * n) : o.final_status.data(),
+                                r.status.ptr, n, hipMemcpyDeviceToHost, s));
       }
       if (!r.spec.params.record) continue;
       for (size_t k = 0; k < r.rounds.size(); ++k) {
@@ -596,6 +621,16 @@ void dpah_server::finish_all(std::vector<std::shared_ptr<Request>>& done, hipStr
                             hipMemcpyDeviceToHost, s));
   }
   DPAH_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < done.size(); ++i) {
+    Request& r = *done[i];
+    r.steps_h = reinterpret_cast<const unsigned long long*>(stage)[r.slot];
+    const uint64_t n = r.count;
+    if (!res[i] || !staged || r.spec.kind != DPAH_REQUEST_ADVECT || !n) continue;
+    const uint8_t* h = stage + stage_off[i];
+    std::memcpy(res[i]->final_rem.data(), h, 8 * n);
+    std::memcpy(res[i]->final_pos.data(), h + 8 * n, 12 * n);
+    std::memcpy(res[i]->final_status.data(), h + 20 * n, n);
+  }
   // every round of a request, empty ones too, in the request's round order
   for (size_t i = 0; i < done.size(); ++i)
     if (res[i] && done[i]->spec.kind == DPAH_REQUEST_ADVECT && done[i]->spec.params.record)
@@ -711,9 +746,11 @@ void dpah_server::launch_ftle(const std::shared_ptr<Request>& rp, int sid) {
 void dpah_server::finish_ftle(Launch& L, hipStream_t s) {
   auto rp = L.reqs.front();
   Request& r = *rp;
-  DPAH_HIP(hipMemcpyAsync(&r.steps_h, L.counters + 1, sizeof(unsigned long long),
+  auto* steps = static_cast<unsigned long long*>(ctx->host_staging(sizeof(unsigned long long)));
+  DPAH_HIP(hipMemcpyAsync(steps, L.counters + 1, sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));
   DPAH_HIP(hipStreamSynchronize(s));
+  r.steps_h = *steps;
   ctx->resolve_timings();
   r.t_device = now_s();
   bool cancelled;
