@@ -559,7 +559,9 @@ def make_context(args):
     over = {k: v for k, v in (("order", args.order), ("blocks_per_cu", args.blocks_per_cu),
                               ("block", args.block), ("refill", args.refill),
                               ("cells", args.cells), ("fastdiv", args.fastdiv),
-                              ("cell_brick", args.cell_brick), ("order_brick", args.order_brick))
+                              ("cell_brick", args.cell_brick), ("order_brick", args.order_brick),
+                              ("lattice_brick", args.lattice_brick),
+                              ("lattice_group", args.lattice_group))
             if v is not None}
     if over:
         ctx.set_tuning(**over)
@@ -879,6 +881,10 @@ def main() -> int:
                     help="queue ordering bricks of at least 2^k cells per axis (0: 4 cells)")
     ap.add_argument("--cell-brick", type=int, choices=[0, 2, 3, 4, 5], default=None,
                     help="cell layout record order of the fields (0: C order; k: 2^k-cell bricks)")
+    ap.add_argument("--lattice-brick", type=int, choices=[0, 2, 4, 8], default=None,
+                    help="whole-lattice launches: points per brick edge")
+    ap.add_argument("--lattice-group", type=int, choices=[0, 1, 2, 4, 8, 16], default=None,
+                    help="whole-lattice launches: bricks per group edge")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="skip the hipGraph replay leg")
     ap.add_argument("--autotune", dest="autotune", action="store_true", default=False,

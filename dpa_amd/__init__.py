@@ -68,7 +68,8 @@ class _Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("order", C.c_int),
                 ("parts", C.c_int), ("refill", C.c_int), ("cells", C.c_int),
                 ("fastdiv", C.c_int), ("cell_brick", C.c_int), ("order_brick", C.c_int),
-                ("lattice_brick", C.c_int), ("slot_stage", C.c_int)]
+                ("lattice_brick", C.c_int), ("slot_stage", C.c_int),
+                ("lattice_group", C.c_int)]
 
 
 def _tuning_dict(t) -> dict:

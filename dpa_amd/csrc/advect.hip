@@ -134,16 +134,25 @@ bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch
 // advection, 39.7 ms in C order).  Whole lattices in one launch only, since
 // the walk is over the whole lattice and the outputs are indexed by lattice
 // point.
-uint32_t lattice_brick(const Tuning& T) {
-  return T.lattice_brick ? (uint32_t)T.lattice_brick : kLatticeBrick;
+uint32_t lattice_brick(const Tuning& T, int seed_kind) {
+  if (T.lattice_brick) return (uint32_t)T.lattice_brick;
+  return seed_kind == kSeedAux ? kAuxBrick : kLatticeBrick;
+}
+
+// Bricks per group edge of the grouped walk; 1: no groups.
+uint32_t lattice_group(const Tuning& T, int seed_kind) {
+  if (T.lattice_group) return (uint32_t)T.lattice_group;
+  return seed_kind == kSeedAux ? 1u : kLatticeGroup;
 }
 
 bool brick_launch(const Tuning& T, const AdvectLaunch& L) {
   if (!(L.persistent && T.order && L.nseg == 0 && L.slots == nullptr && L.lat.first == 0))
     return false;
   const uint64_t points = L.lat.d0 * L.lat.d1 * L.lat.d2;
-  // the walk's 32-bit arithmetic: positions and a whole slab of bricks
-  if ((uint64_t)lattice_brick(T) * L.lat.d1 * L.lat.d2 >= (1ull << 32)) return false;
+  // the walk's 32-bit arithmetic: positions and a whole slab of bricks / groups
+  if ((uint64_t)lattice_brick(T, L.seed_kind) * lattice_group(T, L.seed_kind) * L.lat.d1 *
+          L.lat.d2 >= (1ull << 32))
+    return false;
   if (L.seed_kind == kSeedLattice) return L.n == points && L.n < (1ull << 32);
   if (L.seed_kind == kSeedAux) return L.n == 6 * points && L.n < (1ull << 32);
   return false;
@@ -262,7 +271,9 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   if (ordered_launch(ctx, T, L)) {
     order_queue(ctx, T, a, q, L.order_scratch, s);
   } else if (brick_launch(T, L)) {
-    a.lat.brick = lattice_brick(T);
+    a.lat.brick = lattice_brick(T, L.seed_kind);
+    const uint32_t g = lattice_group(T, L.seed_kind);
+    a.lat.group = g > 1 ? a.lat.brick * g : 0u;
     set_parts(a, (uint32_t)std::max(1, std::min(T.parts, kMaxParts)));
   }
   const bool unsteady = L.f->nt > 1;

@@ -30,6 +30,9 @@ struct SeedLattice {
   // bricks of brick^3 points (lattice_brick_index), the six particles of an
   // auxiliary point together; 0: in C order
   uint32_t brick;
+  // and, when nonzero, walks groups of group^3 points (a multiple of brick)
+  // one after another, bricks inside a group (lattice_walk_index)
+  uint32_t group;
 };
 
 // Queue position q -> C-order index of the lattice point it starts, when the
@@ -60,6 +63,36 @@ __host__ __device__ inline uint64_t lattice_brick_index(uint32_t d0, uint32_t d1
   const uint32_t i1 = r / h2;
   const uint32_t i2 = r - i1 * h2;
   return ((uint64_t)(B * b0 + i0) * d1 + (B * b1 + i1)) * (uint64_t)d2 + (B * b2 + i2);
+}
+
+// The walk with groups of G^3 points (G a multiple of B) above the bricks:
+// groups in C order, the groups on the far faces cut to the lattice, and
+// inside each group (an h0 x h1 x h2 lattice of its own) the brick walk
+// above.  A stretch of consecutive queue positions -- what a part of the
+// queue has in flight at once -- then covers a compact block of the lattice
+// rather than a slab of brick rows.  G = 0: the brick walk alone.  Needs G
+// d1 d2 below 2^32.
+__host__ __device__ inline uint64_t lattice_walk_index(uint32_t d0, uint32_t d1, uint32_t d2,
+                                                       uint32_t B, uint32_t G, uint32_t q) {
+  if (G == 0) return lattice_brick_index(d0, d1, d2, B, q);
+  const uint32_t slab = G * d1 * d2;
+  const uint32_t g0 = q / slab;
+  uint32_t r = q - g0 * slab;
+  const uint32_t h0 = d0 - G * g0 < G ? d0 - G * g0 : G;
+  const uint32_t row = h0 * G * d2;
+  const uint32_t g1 = r / row;
+  r -= g1 * row;
+  const uint32_t h1 = d1 - G * g1 < G ? d1 - G * g1 : G;
+  const uint32_t block = h0 * h1 * G;
+  const uint32_t g2 = r / block;
+  r -= g2 * block;
+  const uint32_t h2 = d2 - G * g2 < G ? d2 - G * g2 : G;
+  const uint32_t local = (uint32_t)lattice_brick_index(h0, h1, h2, B, r);
+  const uint32_t i0 = local / (h1 * h2);
+  const uint32_t rem = local - i0 * h1 * h2;
+  const uint32_t i1 = rem / h2;
+  const uint32_t i2 = rem - i1 * h2;
+  return ((uint64_t)(G * g0 + i0) * d1 + (G * g1 + i1)) * (uint64_t)d2 + (G * g2 + i2);
 }
 
 // Record slots: rows tiled by 64, each row's slots grouped in blocks of 2^g
@@ -291,9 +324,10 @@ __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, flo
       const uint32_t d0 = (uint32_t)a.lat.d0, d1 = (uint32_t)a.lat.d1, d2 = (uint32_t)a.lat.d2;
       if (a.seed_kind == kSeedAux) {
         const uint32_t point = (uint32_t)q / 6u;
-        return 6 * lattice_brick_index(d0, d1, d2, a.lat.brick, point) + ((uint32_t)q - 6u * point);
+        return 6 * lattice_walk_index(d0, d1, d2, a.lat.brick, a.lat.group, point) +
+               ((uint32_t)q - 6u * point);
       }
-      return lattice_brick_index(d0, d1, d2, a.lat.brick, (uint32_t)q);
+      return lattice_walk_index(d0, d1, d2, a.lat.brick, a.lat.group, (uint32_t)q);
     }
     return q;
   }

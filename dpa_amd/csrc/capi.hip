@@ -876,6 +876,7 @@ DPAH_EXPORT int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out) {
     out->order_brick = ctx->tune.order_brick;
     out->lattice_brick = ctx->tune.lattice_brick;
     out->slot_stage = ctx->tune.slot_stage;
+    out->lattice_group = ctx->tune.lattice_group;
   });
 }
 
@@ -898,6 +899,9 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
                      t->lattice_brick == 8,
                  "lattice_brick must be 0, 2, 4 or 8");
     DPAH_REQUIRE(t->slot_stage >= 0 && t->slot_stage <= 2, "slot_stage must be 0, 1 or 2");
+    DPAH_REQUIRE(t->lattice_group == 0 || t->lattice_group == 1 || t->lattice_group == 2 ||
+                     t->lattice_group == 4 || t->lattice_group == 8 || t->lattice_group == 16,
+                 "lattice_group must be 0, 1, 2, 4, 8 or 16");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     ctx->tune.block = t->block;
     ctx->tune.blocks_per_cu = t->blocks_per_cu;
@@ -910,6 +914,7 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     ctx->tune.order_brick = t->order_brick;
     ctx->tune.lattice_brick = t->lattice_brick;
     ctx->tune.slot_stage = t->slot_stage;
+    ctx->tune.lattice_group = t->lattice_group;
     ctx->tuned.clear();   // explicit settings replace autotuned ones
   });
 }

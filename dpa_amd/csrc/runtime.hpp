@@ -100,8 +100,15 @@ struct Tuning {
   // axis (0: 4-cell bricks; larger when the grid needs more than 64 per axis)
   int order_brick = 0;
   // whole-lattice launches walk their points in bricks of this many points per
-  // axis (2, 4 or 8); 0: kLatticeBrick
+  // axis (2, 4 or 8); 0: by launch kind (kLatticeBrick for lattices of points,
+  // kAuxBrick for auxiliary lattices: FTLE ±½-cell 59.4 ms at 4, 58.4 at 2)
   int lattice_brick = 0;
+  // and in groups of this many bricks per axis above them (2, 4, 8 or 16), so
+  // that what a queue part has in flight is a compact block; 1: no groups;
+  // 0: by launch kind (kLatticeGroup for lattices of points -- dpa-lattice
+  // FTLE 11.47 ms without, 10.87 with groups of 2 -- none for auxiliary
+  // lattices, which lose 2-5% with groups)
+  int lattice_group = 0;
   // record slots of persistent recorded launches: 0 by launch size (8-slot
   // blocks staged in LDS from kStageMinRows rows), 1 one-slot runs, 2 staged
   int slot_stage = 0;
@@ -272,6 +279,8 @@ struct OrderScratch {
 // and few enough for 32-bit positions.
 constexpr uint64_t kOrderMinParticles = 1ull << 15;
 constexpr uint32_t kLatticeBrick = 4;   // edge of a brick-ordered lattice queue, in points
+constexpr uint32_t kAuxBrick = 2;       // the same for auxiliary (six-particle) lattices
+constexpr uint32_t kLatticeGroup = 2;   // bricks per group edge, lattices of points
 
 // Launchers (advect.hip / compact.hip).
 struct AdvectLaunch {

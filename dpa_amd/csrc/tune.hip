@@ -42,7 +42,7 @@ void save_cache(const dpah_context* ctx, const std::string& key, const Tuning& t
   out << ctx->arch << " " << ctx->cu_count << " " << key << " " << t.block << " "
       << t.blocks_per_cu << " " << t.order << " " << t.parts << " " << t.refill << " " << t.cells
       << " " << t.fastdiv << " " << t.cell_brick << " " << t.order_brick << " " << ms << " "
-      << t.lattice_brick << " " << t.slot_stage << "\n";
+      << t.lattice_brick << " " << t.slot_stage << " " << t.lattice_group << "\n";
   if (!out) throw Error(DPAH_ERR_INVALID, "cannot write the autotune cache " + ctx->autotune_path);
 }
 
@@ -64,8 +64,9 @@ void load_autotune_cache(dpah_context* ctx) {
     if (!(ss >> arch >> cus >> key >> t.block >> t.blocks_per_cu >> t.order >> t.parts >>
           t.refill >> t.cells >> t.fastdiv >> t.cell_brick >> t.order_brick >> ms))
       continue;
-    int lb = 0, st = 0;
+    int lb = 0, st = 0, lg = 0;
     if (ss >> lb >> st) { t.lattice_brick = lb; t.slot_stage = st; }
+    if (ss >> lg) t.lattice_group = lg;
     if (arch != ctx->arch || cus != ctx->cu_count) continue;
     const bool ok = t.block >= 64 && t.block <= 256 && t.block % 64 == 0 &&
                     t.blocks_per_cu >= 0 && t.blocks_per_cu <= 32 && (t.order == 0 || t.order == 1) &&
@@ -75,7 +76,9 @@ void load_autotune_cache(dpah_context* ctx) {
                     t.order_brick >= 0 && t.order_brick <= 5 &&
                     (t.lattice_brick == 0 || t.lattice_brick == 2 || t.lattice_brick == 4 ||
                      t.lattice_brick == 8) &&
-                    t.slot_stage >= 0 && t.slot_stage <= 2;
+                    t.slot_stage >= 0 && t.slot_stage <= 2 &&
+                    (t.lattice_group == 0 || t.lattice_group == 1 || t.lattice_group == 2 ||
+                     t.lattice_group == 4 || t.lattice_group == 8 || t.lattice_group == 16);
     if (ok) ctx->tuned[key] = t;
   }
 }
@@ -92,6 +95,7 @@ void tuning_to_abi(const Tuning& t, dpah_tuning* out) {
   out->order_brick = t.order_brick;
   out->lattice_brick = t.lattice_brick;
   out->slot_stage = t.slot_stage;
+  out->lattice_group = t.lattice_group;
 }
 
 // Times `launch` (one launch on the context's stream) under every candidate
@@ -224,18 +228,20 @@ DPAH_EXPORT int dpah_autotune_lattice(dpah_context* ctx, const dpah_field* field
     L.t0 = start_time;
     L.integrator = integrator;
     L.persistent = true;
-    // workgroups per CU x refill x queue walk (C order, or bricks of 2 / 4 / 8
-    // points per axis)
+    // workgroups per CU x refill x queue walk (C order; bricks of 2 / 4 / 8
+    // points per axis, alone or in groups of 2 or 4 bricks per axis)
+    const int walks[][2] = {{0, 1}, {2, 1}, {4, 1}, {8, 1}, {2, 2}, {4, 2}, {4, 4}};
     std::vector<Tuning> cands;
-    for (int bpc : {2, 3, 4, 6, 8})
-      for (int refill : {16, 32, 64})
-        for (int brick : {0, 2, 4, 8}) {
+    for (int bpc : {2, 3, 4, 6})
+      for (int refill : {32, 64})
+        for (const auto& w : walks) {
           Tuning t = ctx->tune;
           t.block = 256;
           t.blocks_per_cu = bpc;
           t.refill = refill;
-          t.order = brick ? 1 : 0;
-          t.lattice_brick = brick;
+          t.order = w[0] ? 1 : 0;
+          t.lattice_brick = w[0];
+          t.lattice_group = w[1];
           cands.push_back(t);
         }
     const Tuning best = tune_key(ctx, key, cands, [](const Tuning&) {},

@@ -405,10 +405,14 @@ typedef struct dpah_tuning {
                           2^order_brick cells per axis, 1..5; 0: 4 cells      */
   int lattice_brick;   /* whole-lattice launches (FTLE) walk their points in
                           bricks of this many points per axis: 2, 4 or 8;
-                          0: 4                                                */
+                          0: 4 for lattices of points, 2 for auxiliary ones   */
   int slot_stage;      /* record slots of persistent recorded launches: 0 by
                           launch size (8-slot blocks staged in LDS from 2^15
                           rows), 1 one-slot runs, 2 staged blocks             */
+  int lattice_group;   /* whole-lattice launches walk groups of this many
+                          bricks per axis one after another, bricks inside a
+                          group: 2, 4, 8 or 16; 1: no groups; 0: 2 for
+                          lattices of points, none for auxiliary ones         */
 } dpah_tuning;
 int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
 /* Sets the context's tuning and drops autotuned entries. */

@@ -58,7 +58,7 @@ def test_reciprocal_division_is_correctly_rounded(tmp_path):
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
 def test_lattice_brick_order_is_a_bijection(tmp_path):
     words = build_and_run(tmp_path, "brick_order_check").split()
-    assert int(words[1]) == 27 and int(words[3]) > 100000000 and int(words[5]) == 0, words
+    assert int(words[1]) == 35 and int(words[3]) > 160000000 and int(words[5]) == 0, words
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")

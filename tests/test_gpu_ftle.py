@@ -106,8 +106,9 @@ def test_ftle_estimate_from_given_positions(ctx):
 @pytest.mark.parametrize("kind", ["lattice", "aux"])
 def test_brick_walk_matches_c_order(ctx, kind):
     # whole-lattice launches walk their points in 4x4x4 bricks cut at the far
-    # faces (odd sides here); the volume and the step count must equal the
-    # C-order walk bit for bit
+    # faces (odd sides here), or in groups of bricks above them (groups cut at
+    # the far faces too); the volume and the step count must equal the C-order
+    # walk bit for bit
     say(f"case brick walk vs C order, {kind}")
     field, h = W.abc_field((23, 17, 13), spacing=(2 * math.pi / 22,) * 3)
     off = np.zeros(3, f32)
@@ -118,8 +119,10 @@ def test_brick_walk_matches_c_order(ctx, kind):
     before = ctx.tuning()
     out = []
     try:
-        for order in (1, 0):
-            ctx.set_tuning(order=order)
+        walks = (dict(order=1), dict(order=1, lattice_group=1), dict(order=1, lattice_group=2),
+                 dict(order=1, lattice_brick=2, lattice_group=4), dict(order=0))
+        for walk in walks:
+            ctx.set_tuning(**{**before, **walk})
             if kind == "lattice":
                 vol, steps = ctx.ftle_lattice(fld, (1.0, 1.0, 1.0), iters, dt)
             else:
@@ -134,5 +137,7 @@ def test_brick_walk_matches_c_order(ctx, kind):
     finally:
         ctx.set_tuning(**before)
         fld.release()
-    assert out[0][1] == out[1][1] and out[0][1] > 1000
-    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    assert out[-1][1] > 1000
+    for vol, steps in out[:-1]:
+        assert steps == out[-1][1]
+        assert np.array_equal(vol.view(np.uint32), out[-1][0].view(np.uint32))

@@ -105,7 +105,8 @@ def test_set_tuning_rejects_bad_shapes(ctx):
     for bad in (dict(block=96), dict(block=512), dict(blocks_per_cu=-1), dict(blocks_per_cu=33),
                 dict(order=2),
                 dict(parts=9), dict(parts=0), dict(cell_brick=1), dict(cell_brick=6),
-                dict(order_brick=-1), dict(order_brick=6)):
+                dict(order_brick=-1), dict(order_brick=6), dict(lattice_group=3),
+                dict(lattice_group=32)):
         with pytest.raises(dpa_amd.DpahError):
             ctx.set_tuning(**bad)
 
