@@ -264,8 +264,25 @@ struct GroupRun {
     }
     ctx->sync();
 
-    // every round's writes first, so the device works while the host copies
-    // the final states out of the staging below
+    // every round's writes first, in one launch, so the device works while
+    // the host copies the final states out of the staging below
+    std::vector<WriteJob> writes;
+    if (record) {
+      for (const JobPlan& p : plans)
+        for (const RoundPlan& r : p.rounds) {
+          if (!r.count) continue;
+          const size_t s = (size_t)r.seg;
+          const uint64_t* tot = h_sums + sums_off[s] + compact_sums_words(segs[s].count) - 2;
+          if (!p.use64)
+            DPAH_REQUIRE(tot[0] <= 0xffffffffull,
+                         "round has more than 2^32-1 vertices; 32-bit indices cannot address it");
+          writes.push_back(WriteJob{rows_of(s), d_vcount.as<uint64_t>() + row_off[s],
+                                    d_icount.as<uint64_t>() + row_off[s], tot[0], tot[1]});
+        }
+    }
+    DeviceBuffer table;
+    std::vector<Chunk> chunks = compact_write_many(ctx, writes, table, ctx->stream);
+    size_t w = 0;
     for (const JobPlan& p : plans) {
       auto res = std::make_unique<dpah_result>();
       res->ctx = ctx;
@@ -274,14 +291,7 @@ struct GroupRun {
         for (const RoundPlan& r : p.rounds) {
           Round round;
           if (r.count) {
-            const size_t s = (size_t)r.seg;
-            const uint64_t* tot = h_sums + sums_off[s] + compact_sums_words(segs[s].count) - 2;
-            if (!res->use64)
-              DPAH_REQUIRE(tot[0] <= 0xffffffffull,
-                           "round has more than 2^32-1 vertices; 32-bit indices cannot address it");
-            Chunk ch = compact_write(ctx, rows_of(s), d_vcount.as<uint64_t>() + row_off[s],
-                                     d_icount.as<uint64_t>() + row_off[s], tot[0], tot[1],
-                                     ctx->stream);
+            const Chunk& ch = chunks[w++];
             round.V = ch.V;
             round.I = ch.I;
             round.chunks.push_back(ch);

@@ -37,23 +37,27 @@ void compact_count(dpah_context* ctx, const CompactRows& r, uint64_t* vcount, ui
   (void)ctx;
 }
 
-Chunk compact_write(dpah_context* ctx, const CompactRows& r, const uint64_t* vcount,
-                    const uint64_t* icount, uint64_t vertices, uint64_t segments,
-                    hipStream_t st) {
-  Chunk ch;
-  if (r.count == 0) return ch;
+namespace {
+
+// The chunk (one allocation: vertices | colours | indices) and the launch
+// arguments of one round's writes; returns the workgroup count, 0 for an
+// empty round.
+uint64_t plan_write(dpah_context* ctx, const CompactRows& r, const uint64_t* vcount,
+                    const uint64_t* icount, uint64_t vertices, uint64_t segments, Chunk& ch,
+                    CompactArgs& a) {
+  ch = Chunk{};
+  a = CompactArgs{};
+  if (r.count == 0) return 0;
   ch.V = vertices;
   ch.I = 2 * segments;
   const uint64_t ib = r.use64 ? 8 : 4;
   ch.bytes = ch.V * 12 + ch.V * 4 + ch.I * ib;
-  // one allocation per chunk, carved into vertices | colors | indices
   if (ch.bytes) {
     void* base = ctx->alloc(ch.bytes);
     ch.vertices = static_cast<float*>(base);
     ch.colors = ch.vertices + 3 * ch.V;
     ch.indices = ch.colors + ch.V;
   }
-  CompactArgs a{};
   a.slots = r.slots;
   a.stride = r.stride;
   a.steps = r.steps;
@@ -81,18 +85,71 @@ Chunk compact_write(dpah_context* ctx, const CompactRows& r, const uint64_t* vco
   const uint64_t split = std::min(chunks, (want + row_groups - 1) / row_groups);
   const uint64_t span_slots = (chunks + split - 1) / split * kCompactChunk;
   const uint64_t spans = (r.stride + span_slots - 1) / span_slots;
-  const uint64_t groups = row_groups * spans;
+  const uint64_t groups = row_groups * spans;   // a multiple of xcds
   DPAH_REQUIRE(span_slots < (1ull << 31), "rows too long for one compaction");
   DPAH_REQUIRE(groups < (1ull << 31), "too many rows or slots in one compaction");
   a.tiles = (uint32_t)tiles;
   a.spans = (uint32_t)spans;
   a.span_slots = (uint32_t)span_slots;
   a.xcds = (uint32_t)xcds;
+  return groups;
+}
+
+}  // namespace
+
+Chunk compact_write(dpah_context* ctx, const CompactRows& r, const uint64_t* vcount,
+                    const uint64_t* icount, uint64_t vertices, uint64_t segments,
+                    hipStream_t st) {
+  Chunk ch;
+  CompactArgs a;
+  const uint64_t groups = plan_write(ctx, r, vcount, icount, vertices, segments, ch, a);
+  if (!groups) return ch;
   ctx->timed("compact_write", st, [&] {
     hipLaunchKernelGGL(k_compact_write, dim3((unsigned)groups), dim3(kCompactThreads), 0, st, a);
   });
   DPAH_HIP(hipGetLastError());
   return ch;
+}
+
+std::vector<Chunk> compact_write_many(dpah_context* ctx, const std::vector<WriteJob>& jobs,
+                                      DeviceBuffer& table, hipStream_t st) {
+  std::vector<Chunk> out(jobs.size());
+  std::vector<CompactSeg> segs;
+  uint64_t first = 0;
+  try {
+    for (size_t i = 0; i < jobs.size(); ++i) {
+      CompactSeg g{};
+      const uint64_t groups = plan_write(ctx, jobs[i].rows, jobs[i].vcount, jobs[i].icount,
+                                         jobs[i].vertices, jobs[i].segments, out[i], g.a);
+      if (!groups) continue;
+      g.first = (uint32_t)first;
+      first += groups;
+      DPAH_REQUIRE(first < (1ull << 31), "too many workgroups in one compaction launch");
+      segs.push_back(g);
+    }
+  } catch (...) {
+    for (Chunk& c : out)
+      if (c.vertices) ctx->release(c.vertices, c.bytes);
+    throw;
+  }
+  if (segs.empty()) return out;
+  try {
+    table = DeviceBuffer(ctx, segs.size() * sizeof(CompactSeg));
+    DPAH_HIP(hipMemcpyAsync(table.ptr, segs.data(), segs.size() * sizeof(CompactSeg),
+                            hipMemcpyHostToDevice, st));
+    const CompactSeg* d = table.as<CompactSeg>();
+    const uint32_t n = (uint32_t)segs.size();
+    ctx->timed("compact_write", st, [&] {
+      hipLaunchKernelGGL(k_compact_write_many, dim3((unsigned)first), dim3(kCompactThreads), 0,
+                         st, d, n);
+    });
+    DPAH_HIP(hipGetLastError());
+  } catch (...) {
+    for (Chunk& c : out)
+      if (c.vertices) ctx->release(c.vertices, c.bytes);
+    throw;
+  }
+  return out;
 }
 
 Chunk compact_rows(dpah_context* ctx, const float* slots, uint64_t stride, uint32_t g,

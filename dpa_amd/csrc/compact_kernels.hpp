@@ -180,12 +180,14 @@ typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
 // no swizzle; loads beyond the resource's byte size return zero.
 constexpr int kBufferWord3 = 0x00020000;
 
-static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(CompactArgs a) {
+// The body of one write workgroup; `block` is its workgroup index within the
+// launch of `a` (blockIdx.x for a launch of its own).
+__device__ __forceinline__ void compact_write_block(const CompactArgs& a, uint32_t block) {
   __shared__ float buf[kGroupRows * kBandRow];
   __shared__ uint32_t len[kGroupRows];   // k + 2 slots to emit, 0: none (absent or special)
   __shared__ uint64_t vbase[kGroupRows], ibase[kGroupRows];
   const uint32_t t = threadIdx.x;
-  const uint32_t xcd = blockIdx.x % a.xcds, local = blockIdx.x / a.xcds;
+  const uint32_t xcd = block % a.xcds, local = block / a.xcds;
   const uint32_t per_tile = a.spans * kGroupsPerTile;
   const uint32_t tile = (local / per_tile) * a.xcds + xcd, rest = local % per_tile;
   const uint32_t span = rest / kGroupsPerTile, grp = rest % kGroupsPerTile;
@@ -298,6 +300,30 @@ static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(Compac
     const uint64_t rr = row0 + t;
     if (rr < a.count && a.special[rr]) compact_special_row(a, rr);
   }
+}
+
+static __global__ __launch_bounds__(kCompactThreads) void k_compact_write(CompactArgs a) {
+  compact_write_block(a, blockIdx.x);
+}
+
+// The writes of several rounds in one launch: segment s owns workgroups
+// [first, first of s + 1); each segment's workgroup count is a multiple of
+// the XCD count, so block % xcds is the same XCD as in a launch of its own,
+// and every segment's workgroups do exactly what they would do there.
+struct CompactSeg {
+  CompactArgs a;
+  uint32_t first;
+};
+
+static __global__ __launch_bounds__(kCompactThreads) void k_compact_write_many(
+    const CompactSeg* segs, uint32_t nseg) {
+  uint32_t lo = 0, hi = nseg;   // last segment with first <= blockIdx.x
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (segs[mid].first <= blockIdx.x) lo = mid; else hi = mid;
+  }
+  const CompactArgs a = segs[lo].a;
+  compact_write_block(a, blockIdx.x - segs[lo].first);
 }
 
 // ---- exclusive scan of two uint64 arrays in place (block sums + fix-up) ----

@@ -407,5 +407,17 @@ void compact_count(dpah_context* ctx, const CompactRows& r, uint64_t* vcount, ui
 Chunk compact_write(dpah_context* ctx, const CompactRows& r, const uint64_t* vcount,
                     const uint64_t* icount, uint64_t vertices, uint64_t segments,
                     hipStream_t stream);
+// The writes of several rounds (counted and scanned, totals read back) in one
+// launch: the chunks in job order (empty for empty rounds).  `table` receives
+// the launch's segment table, which must outlive the launch (keep it until
+// the stream is synchronised).  On an error no chunk is left allocated.
+struct WriteJob {
+  CompactRows rows;
+  const uint64_t* vcount;
+  const uint64_t* icount;
+  uint64_t vertices, segments;
+};
+std::vector<Chunk> compact_write_many(dpah_context* ctx, const std::vector<WriteJob>& jobs,
+                                      DeviceBuffer& table, hipStream_t stream);
 
 }  // namespace dpah

@@ -635,23 +635,43 @@ void dpah_server::finish_all(std::vector<std::shared_ptr<Request>>& done, hipStr
   for (size_t i = 0; i < done.size(); ++i)
     if (res[i] && done[i]->spec.kind == DPAH_REQUEST_ADVECT && done[i]->spec.params.record)
       res[i]->rounds.resize(done[i]->rounds.size());
+  // every round's writes in one launch; a round that cannot be written
+  // fails its request, and a launch that cannot be made fails them all
+  std::vector<WriteJob> writes;
+  std::vector<const RoundCounts*> written;
   for (RoundCounts& c : counts) {
     if (!res[c.req]) continue;
-    Request& r = *done[c.req];
-    try {
-      if (!res[c.req]->use64 && c.totals[0] > 0xffffffffull)
-        throw Error(DPAH_ERR_INVALID, "round has more than 2^32-1 vertices");
-      Chunk ch = compact_write(ctx, rows_of(r, r.rounds[c.round]), c.vcount.as<uint64_t>(),
-                               c.icount.as<uint64_t>(), c.totals[0], c.totals[1], s);
-      Round& round = res[c.req]->rounds[c.round];
-      round.V = ch.V;
-      round.I = ch.I;
-      round.chunks.push_back(ch);
-    } catch (const Error& e) {
-      fail(c.req, e);
+    if (!res[c.req]->use64 && c.totals[0] > 0xffffffffull) {
+      fail(c.req, Error(DPAH_ERR_INVALID, "round has more than 2^32-1 vertices"));
+      continue;
     }
+    Request& r = *done[c.req];
+    writes.push_back(WriteJob{rows_of(r, r.rounds[c.round]), c.vcount.as<uint64_t>(),
+                              c.icount.as<uint64_t>(), c.totals[0], c.totals[1]});
+    written.push_back(&c);
+  }
+  DeviceBuffer table;
+  std::vector<Chunk> orphans;   // written for a request that failed meanwhile
+  try {
+    std::vector<Chunk> chunks = compact_write_many(ctx, writes, table, s);
+    for (size_t k = 0; k < written.size(); ++k) {
+      const RoundCounts& c = *written[k];
+      if (!res[c.req]) {
+        orphans.push_back(chunks[k]);
+        continue;
+      }
+      Round& round = res[c.req]->rounds[c.round];
+      round.V = chunks[k].V;
+      round.I = chunks[k].I;
+      round.chunks.push_back(chunks[k]);
+    }
+  } catch (const Error& e) {
+    for (const RoundCounts* c : written)
+      if (res[c->req]) fail(c->req, e);
   }
   DPAH_HIP(hipStreamSynchronize(s));   // the chunks are written before the counts are freed
+  for (const Chunk& c : orphans)      // and before an orphan's block can be handed out again
+    if (c.vertices) ctx->release(c.vertices, c.bytes);
   counts.clear();
   for (size_t i = 0; i < done.size(); ++i) {
     if (!res[i]) continue;
