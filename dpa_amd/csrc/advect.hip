@@ -23,9 +23,24 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
     uint64_t grid = (uint64_t)ctx->cu_count * bpc;
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
+    const bool lattice = a.seed_kind != kSeedArray;
+    DPAH_REQUIRE(!(kRecord && lattice), "lattice-seeded persistent launches record no curves");
     ctx->timed("advect_persistent", s, [&] {
       // the load-counting instance exists for steady dpa-mode endpoint launches
       constexpr bool kCountable = !kRecord && kIntegrator == kDpaRk4 && !kUnsteady;
+      if constexpr (!kRecord) {
+        if (lattice) {
+          if (kCountable && a.loads_out)
+            hipLaunchKernelGGL((k_advect_persistent<false, kIntegrator, kUnsteady, kDiv,
+                                                    kSlotDirect, kCountable, true>),
+                               dim3((unsigned)grid), dim3(block), 0, s, a);
+          else
+            hipLaunchKernelGGL((k_advect_persistent<false, kIntegrator, kUnsteady, kDiv,
+                                                    kSlotDirect, false, true>),
+                               dim3((unsigned)grid), dim3(block), 0, s, a);
+          return;
+        }
+      }
       if (kRecord && a.slot_shift == kStagedShift)
         hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
                                                 kRecord ? kSlotDeferred : kSlotDirect>),

@@ -74,25 +74,31 @@ __host__ __device__ inline uint64_t lattice_brick_index(uint32_t d0, uint32_t d1
 // d1 d2 below 2^32.
 __host__ __device__ inline uint64_t lattice_walk_index(uint32_t d0, uint32_t d1, uint32_t d2,
                                                        uint32_t B, uint32_t G, uint32_t q) {
-  if (G == 0) return lattice_brick_index(d0, d1, d2, B, q);
-  const uint32_t slab = G * d1 * d2;
-  const uint32_t g0 = q / slab;
-  uint32_t r = q - g0 * slab;
-  const uint32_t h0 = d0 - G * g0 < G ? d0 - G * g0 : G;
-  const uint32_t row = h0 * G * d2;
-  const uint32_t g1 = r / row;
-  r -= g1 * row;
-  const uint32_t h1 = d1 - G * g1 < G ? d1 - G * g1 : G;
-  const uint32_t block = h0 * h1 * G;
-  const uint32_t g2 = r / block;
-  r -= g2 * block;
-  const uint32_t h2 = d2 - G * g2 < G ? d2 - G * g2 : G;
-  const uint32_t local = (uint32_t)lattice_brick_index(h0, h1, h2, B, r);
-  const uint32_t i0 = local / (h1 * h2);
-  const uint32_t rem = local - i0 * h1 * h2;
+  // G = 0: one group, the whole lattice (one brick walk in the code either way)
+  uint32_t h0 = d0, h1 = d1, h2 = d2, r = q, o0 = 0, o1 = 0, o2 = 0;
+  if (G != 0) {
+    const uint32_t slab = G * d1 * d2;
+    const uint32_t g0 = q / slab;
+    r = q - g0 * slab;
+    h0 = d0 - G * g0 < G ? d0 - G * g0 : G;
+    const uint32_t row = h0 * G * d2;
+    const uint32_t g1 = r / row;
+    r -= g1 * row;
+    h1 = d1 - G * g1 < G ? d1 - G * g1 : G;
+    const uint32_t block = h0 * h1 * G;
+    const uint32_t g2 = r / block;
+    r -= g2 * block;
+    h2 = d2 - G * g2 < G ? d2 - G * g2 : G;
+    o0 = G * g0; o1 = G * g1; o2 = G * g2;
+  }
+  const uint64_t local = lattice_brick_index(h0, h1, h2, B, r);
+  if (G == 0) return local;
+  const uint32_t l = (uint32_t)local;
+  const uint32_t i0 = l / (h1 * h2);
+  const uint32_t rem = l - i0 * h1 * h2;
   const uint32_t i1 = rem / h2;
   const uint32_t i2 = rem - i1 * h2;
-  return ((uint64_t)(G * g0 + i0) * d1 + (G * g1 + i1)) * (uint64_t)d2 + (G * g2 + i2);
+  return ((uint64_t)(o0 + i0) * d1 + (o1 + i1)) * (uint64_t)d2 + (o2 + i2);
 }
 
 // Record slots: rows tiled by 64, each row's slots grouped in blocks of 2^g
@@ -293,9 +299,12 @@ __device__ __forceinline__ void lattice_point(const SeedLattice& g, uint64_t i, 
   { const float m = g.st2 * (float)i2; p2 = g.lo2 + m; }
 }
 
+// kLattice false: the launch seeds from an array (a.seed_kind is kSeedArray),
+// and the lattice forms are compiled out of the kernel.
+template <bool kLattice = true>
 __device__ __forceinline__ void seed_position(const AdvectArgs& a, uint64_t pid, float& p0,
                                               float& p1, float& p2) {
-  if (a.seed_kind == kSeedArray) {
+  if (!kLattice || a.seed_kind == kSeedArray) {
     const float* s = a.seeds + 3 * pid;
     p0 = s[0]; p1 = s[1]; p2 = s[2];
   } else if (a.seed_kind == kSeedLattice) {
@@ -316,11 +325,11 @@ __device__ __forceinline__ void seed_position(const AdvectArgs& a, uint64_t pid,
 // Queue ordinal -> particle index and slot row (binary search over segments;
 // slot shift kShift, which is the launch's a.slot_shift); ~0 for the padding
 // ordinals at the end of a segment's last tile.
-template <uint32_t kShift = 0>
+template <uint32_t kShift = 0, bool kLattice = true>
 __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, float*& row) {
   if (a.nseg == 0) {
     row = a.slots + slot_float(q, a.stride, 0, kShift);
-    if (a.lat.brick) {   // brick-ordered lattice (whole-lattice endpoint launches)
+    if (kLattice && a.lat.brick) {   // brick-ordered lattice (whole-lattice endpoint launches)
       const uint32_t d0 = (uint32_t)a.lat.d0, d1 = (uint32_t)a.lat.d1, d2 = (uint32_t)a.lat.d2;
       if (a.seed_kind == kSeedAux) {
         const uint32_t point = (uint32_t)q / 6u;
@@ -343,14 +352,14 @@ __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, flo
   return s.first + local;
 }
 
-template <bool kRecord, int kSlots = kSlotDirect>
+template <bool kRecord, int kSlots = kSlotDirect, bool kLattice = true>
 __device__ __forceinline__ void begin_particle(const AdvectArgs& a, Lane& L, uint64_t q) {
   float* row = nullptr;
-  const uint64_t pid = resolve<kSlots == kSlotDirect ? 0u : kStagedShift>(a, q, row);
+  const uint64_t pid = resolve<kSlots == kSlotDirect ? 0u : kStagedShift, kLattice>(a, q, row);
   L.pid = pid;
   if (pid == ~0ull) return;   // segment padding: the lane stays idle
   cache_clear(L.cc);
-  seed_position(a, pid, L.p0, L.p1, L.p2);
+  seed_position<kLattice>(a, pid, L.p0, L.p1, L.p2);
   L.rem = a.rem_in ? a.rem_in[pid] : a.uniform_rem;
   L.it = 0;
   if constexpr (kRecord) {
@@ -450,9 +459,14 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
 // a.loads_out (the roofline's byte count; a measurement instance, for steady
 // dpa-mode endpoint launches: one sample per step, so a step reloaded exactly
 // when it changed the cache's cell key).
+// kLattice: the launch seeds a lattice (a.seed_kind kSeedLattice or kSeedAux,
+// endpoint launches only); false compiles the lattice seeding and walk out of
+// a seed-array launch's refill, where their uniform values would otherwise
+// add to the scalar registers the step loop keeps.
 template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv, int kSlots = kSlotDirect,
-          bool kCountLoads = false>
+          bool kCountLoads = false, bool kLattice = false>
 __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
+  static_assert(!kLattice || !kRecord, "lattice-seeded launches record no curves");
   // slot index (record) and time (time-variant fields, classic RK4) need the
   // step count inside the loop; steady dpa-mode endpoints only at the end
   constexpr bool kCountIt = kRecord || kUnsteady || kIntegrator == kRk4;
@@ -494,7 +508,7 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
         if (idle) {
           const uint64_t pos = lo + base + lane_rank(need);
           const uint64_t q = pos < hi ? (a.order ? (uint64_t)a.order[pos] : pos) : a.n;
-          if (q < a.n) begin_particle<kRecord, kSlots>(a, L, q);   // q < n also guards the order table
+          if (q < a.n) begin_particle<kRecord, kSlots, kLattice>(a, L, q);   // q < n also guards the order table
         }
         if (lo + base + cnt >= hi) {
           part = part + 1 == a.parts ? 0 : part + 1;
