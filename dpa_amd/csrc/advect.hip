@@ -1,6 +1,7 @@
 // advect.hip -- launchers of the K_advect kernels (advect_kernels.hpp) and of
 // the queue ordering that precedes a persistent launch (order_kernels.hpp).
 #include <algorithm>
+#include <type_traits>
 
 #include "advect_kernels.hpp"
 #include "order_kernels.hpp"
@@ -41,17 +42,26 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
           return;
         }
       }
-      if (kRecord && a.slot_shift == kStagedShift)
-        hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
-                                                kRecord ? kSlotDeferred : kSlotDirect>),
-                           dim3((unsigned)grid), dim3(block), 0, s, a);
-      else if (kCountable && a.loads_out)
-        hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
-                                                kSlotDirect, kCountable>),
-                           dim3((unsigned)grid), dim3(block), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv>),
-                           dim3((unsigned)grid), dim3(block), 0, s, a);
+      // seed arrays on a cell layout with 32-bit record offsets (every
+      // benchmark field) run instances that assume so
+      auto go = [&](auto cells32) {
+        constexpr bool kC = decltype(cells32)::value;
+        if (kRecord && a.slot_shift == kStagedShift)
+          hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
+                                                  kRecord ? kSlotDeferred : kSlotDirect, false,
+                                                  false, kC>),
+                             dim3((unsigned)grid), dim3(block), 0, s, a);
+        else if (kCountable && a.loads_out)
+          hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
+                                                  kSlotDirect, kCountable, false, kC>),
+                             dim3((unsigned)grid), dim3(block), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
+                                                  kSlotDirect, false, false, kC>),
+                             dim3((unsigned)grid), dim3(block), 0, s, a);
+      };
+      if (a.f.cells && a.f.idx32) go(std::true_type{});
+      else                        go(std::false_type{});
     });
   } else {
     const uint64_t grid = (a.n + 255) / 256;
