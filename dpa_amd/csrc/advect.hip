@@ -31,6 +31,9 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
       constexpr bool kCountable = !kRecord && kIntegrator == kDpaRk4 && !kUnsteady;
       if constexpr (!kRecord) {
         if (lattice) {
+          // (an instance that assumes the node layout with 32-bit offsets
+          // measured 58.39 -> 58.20 ms on the auxiliary FTLE launch and 10.87
+          // -> 11.00 on the dpa-lattice one: not kept)
           if (kCountable && a.loads_out)
             hipLaunchKernelGGL((k_advect_persistent<false, kIntegrator, kUnsteady, kDiv,
                                                     kSlotDirect, kCountable, true>),
@@ -44,8 +47,8 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
       }
       // seed arrays on a cell layout with 32-bit record offsets (every
       // benchmark field) run instances that assume so
-      auto go = [&](auto cells32) {
-        constexpr bool kC = decltype(cells32)::value;
+      auto go = [&](auto layout) {
+        constexpr int kC = decltype(layout)::value;
         if (kRecord && a.slot_shift == kStagedShift)
           hipLaunchKernelGGL((k_advect_persistent<kRecord, kIntegrator, kUnsteady, kDiv,
                                                   kRecord ? kSlotDeferred : kSlotDirect, false,
@@ -60,8 +63,8 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
                                                   kSlotDirect, false, false, kC>),
                              dim3((unsigned)grid), dim3(block), 0, s, a);
       };
-      if (a.f.cells && a.f.idx32) go(std::true_type{});
-      else                        go(std::false_type{});
+      if (a.f.cells && a.f.idx32) go(std::integral_constant<int, kLayoutCells32>{});
+      else                        go(std::integral_constant<int, kLayoutAny>{});
     });
   } else {
     const uint64_t grid = (a.n + 255) / 256;

@@ -20,6 +20,9 @@ enum SeedKind : int {
   kSeedAux = 2,      // pid = 6*point + a: lattice point +- delta along axis a/2
 };
 
+// Field layouts a persistent kernel instance may assume (k_advect_persistent).
+enum : int { kLayoutAny = 0, kLayoutCells32 = 1 };
+
 struct SeedLattice {
   float lo0, lo1, lo2;        // offset
   float st0, st1, st2;        // stride (physical)
@@ -463,17 +466,21 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
 // endpoint launches only); false compiles the lattice seeding and walk out of
 // a seed-array launch's refill, where their uniform values would otherwise
 // add to the scalar registers the step loop keeps.
-// kCells32: the field has a cell layout with 32-bit record offsets (the
-// launch checks a.f.cells and a.f.idx32); the node-layout and 64-bit reload
-// paths drop out of the loop (256^3 endpoints: 369 -> 305 vector and 259 ->
-// 217 scalar instructions of code, 100 -> 80 scalar registers).
+// kLayout: what the launch checked of the field before it chose the
+// instance -- kLayoutCells32, a cell layout with 32-bit record offsets;
+// kLayoutAny, nothing.  The other reload paths drop out of the loop (256^3
+// endpoints on cells: 369 -> 305 vector and 259 -> 217 scalar instructions of
+// code, 100 -> 80 scalar registers).
 template <bool kRecord, int kIntegrator, bool kUnsteady, int kDiv, int kSlots = kSlotDirect,
-          bool kCountLoads = false, bool kLattice = false, bool kCells32 = false>
+          bool kCountLoads = false, bool kLattice = false, int kLayout = kLayoutAny>
 __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   static_assert(!kLattice || !kRecord, "lattice-seeded launches record no curves");
-  if constexpr (kCells32) {
-    __builtin_assume(a.f.cells != nullptr);
-    __builtin_assume(a.f.idx32 != 0);
+  // the field as the step loop sees it, with what the launch checked made
+  // known to the compiler
+  FieldView fv = a.f;
+  if constexpr (kLayout == kLayoutCells32) {
+    __builtin_assume(fv.cells != nullptr);
+    fv.idx32 = 1;
   }
   // slot index (record) and time (time-variant fields, classic RK4) need the
   // step count inside the loop; steady dpa-mode endpoints only at the end
@@ -532,7 +539,7 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
         float k0 = 0.0f, k1 = 0.0f, k2 = 0.0f;
         if constexpr (kCountLoads) { k0 = L.cc.s0; k1 = L.cc.s1; k2 = L.cc.s2; }
         const uint8_t st =
-            step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt, kSlots>(a.f, a.coef, a.t0, L);
+            step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt, kSlots>(fv, a.coef, a.t0, L);
         if constexpr (kCountLoads)
           loads += (L.cc.s0 != k0 || L.cc.s1 != k1 || L.cc.s2 != k2) ? 1u : 0u;
         if (st != kActive) {
@@ -545,7 +552,7 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
       bool full = false;   // this step's slot completed a block
       if (busy) {
         const uint32_t it0 = L.it;
-        st = step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt, kSlots>(a.f, a.coef, a.t0, L);
+        st = step_lane<kRecord, kIntegrator, kUnsteady, kDiv, kCountIt, kSlots>(fv, a.coef, a.t0, L);
         full = L.it != it0 && L.it % kSlotGroup == kSlotGroup - 1;
       }
       // full blocks go out before a terminal vertex reuses the stage
