@@ -8,7 +8,7 @@
 // (field, integrator, record) group that advances every admitted particle by
 // up to `steps_per_tick` steps; with no step budget an FTLE request instead
 // runs as its own persistent lattice launch (the brick-walked queue of
-// dpah_ftle_aux_device).  Up to four launches run at once, one per stream.
+// dpah_ftle_aux_device).  Up to three launches run at once, one per stream.
 // Two comparators run each request alone, in arrival order: the per-request
 // mode allocates, uploads, launches one thread per particle, compacts and
 // frees for every request; the serial mode goes through the library's
@@ -135,7 +135,13 @@ struct Launch {
 };
 
 struct dpah_server {
-  static constexpr int kStreams = 4;
+  // Three streams: with the context's own stream that makes four, the number
+  // of hardware queues a process gets by default (GPU_MAX_HW_QUEUES); a fifth
+  // stream shares a queue with another and waits behind its work.  On four
+  // resident 256^3 x 16 fields at 500 req/s: 4 streams 1.30e11 steps/s, p99
+  // 234-247 ms; 3 streams 1.36e11, p99 106-114 ms; 2 streams 1.36e11, p50
+  // 24 ms instead of 13.
+  static constexpr int kStreams = 3;
   dpah_context* ctx = nullptr;
   dpah_server_options opt{};
   uint64_t budget = 0;

@@ -226,7 +226,7 @@ int dpah_ftle_aux_device(dpah_context* ctx, const dpah_field* field, const float
  * keeps fields resident, admits advection and FTLE queries by memory and
  * batches them continuously.  One engine thread owns the device.  In
  * DPAH_SERVE_BATCHED mode admitted requests are launched as soon as one of
- * four streams is free; the requests waiting at that moment that share
+ * three streams is free; the requests waiting at that moment that share
  * (field, integrator, record) go into ONE launch, which advances each particle
  * by up to steps_per_tick steps (0: to retirement) and puts unfinished
  * requests back in line; results are compacted on the launch's stream.
