@@ -45,7 +45,7 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
           return;
         }
       }
-      // seed arrays on a cell layout with 32-bit record offsets (every
+      // seed arrays on a cell layout read as one buffer resource (every
       // benchmark field) run instances that assume so
       auto go = [&](auto layout) {
         constexpr int kC = decltype(layout)::value;
@@ -63,7 +63,7 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
                                                   kSlotDirect, false, false, kC>),
                              dim3((unsigned)grid), dim3(block), 0, s, a);
       };
-      if (a.f.cells && a.f.idx32) go(std::integral_constant<int, kLayoutCells32>{});
+      if (a.f.cells && a.f.cell_bytes32) go(std::integral_constant<int, kLayoutCells32>{});
       else                        go(std::integral_constant<int, kLayoutAny>{});
     });
   } else {

@@ -20,7 +20,8 @@ enum SeedKind : int {
   kSeedAux = 2,      // pid = 6*point + a: lattice point +- delta along axis a/2
 };
 
-// Field layouts a persistent kernel instance may assume (k_advect_persistent).
+// Field layouts a persistent kernel instance may assume (k_advect_persistent):
+// kLayoutCells32, a cell layout read as one buffer resource (cell_bytes32).
 enum : int { kLayoutAny = 0, kLayoutCells32 = 1 };
 
 struct SeedLattice {
@@ -467,7 +468,7 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
 // a seed-array launch's refill, where their uniform values would otherwise
 // add to the scalar registers the step loop keeps.
 // kLayout: what the launch checked of the field before it chose the
-// instance -- kLayoutCells32, a cell layout with 32-bit record offsets;
+// instance -- kLayoutCells32, a cell layout read as one buffer resource;
 // kLayoutAny, nothing.  The other reload paths drop out of the loop (256^3
 // endpoints on cells: 369 -> 305 vector and 259 -> 217 scalar instructions of
 // code, 100 -> 80 scalar registers).
@@ -480,7 +481,7 @@ __global__ __launch_bounds__(256) void k_advect_persistent(AdvectArgs a) {
   FieldView fv = a.f;
   if constexpr (kLayout == kLayoutCells32) {
     __builtin_assume(fv.cells != nullptr);
-    fv.idx32 = 1;
+    __builtin_assume(fv.cell_bytes32 != 0);
   }
   // slot index (record) and time (time-variant fields, classic RK4) need the
   // step count inside the loop; steady dpa-mode endpoints only at the end

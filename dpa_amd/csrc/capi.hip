@@ -391,6 +391,9 @@ static void set_division(dpah_context* ctx, FieldView& v) {
       v.exactdiv |= 1u << a;
       continue;
     }
+    // the fast remainder's division needs non-zero remainders of at least
+    // 2^-90: a spacing's ulp is one when the spacing is at least 2^-60
+    if (!(h[a] >= 0x1p-60f)) continue;
     const float B = std::nextafter(2.0f * (float)(n[a] - 1) * h[a], INFINITY);
     if (!(B < INFINITY)) continue;
     uint32_t hb, bb;
@@ -439,6 +442,11 @@ static void build_cells(dpah_context* ctx, dpah_field* f) {
   ctx->sync();
   v.cells = f->d_cells;
   if (8 * records >= (1ull << 32)) v.idx32 = 0;
+  // buffer reloads: the layout below 2^32 bytes and the factors of
+  // cell_byte24's products (record counts per axis and plane) below 2^24
+  const uint64_t plane = k ? (uint64_t)v.cb1 * (uint64_t)v.cb2
+                           : (uint64_t)(v.n1 - 1) * (uint64_t)(v.n2 - 1);
+  v.cell_bytes32 = bytes < (1ull << 32) && plane < (1ull << 24) ? (uint32_t)bytes : 0u;
 }
 
 static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shape[3], uint64_t nt,
@@ -481,6 +489,12 @@ static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shap
   v.cell_bits = 0;
   v.cb1 = v.cb2 = 0;
   v.idx32 = 3 * nodes * nt < (1ull << 32) ? 1u : 0u;
+  // buffer reloads: the node array below 2^32 bytes and a plane's bytes
+  // 12 n1 n2 below 2^24 (every axis already is), the factors of node_byte24's
+  // products
+  v.node_bytes32 =
+      bytes < (1ull << 32) && 12 * shape[1] * shape[2] < (1ull << 24) ? (uint32_t)bytes : 0u;
+  v.cell_bytes32 = 0;
   try {
     set_division(ctx, v);
     build_cells(ctx, f);

@@ -176,9 +176,6 @@ constexpr uint32_t kCompactChunk = kRunLanes - 1;
 constexpr uint32_t kGroupsPerTile = kTileRows / kGroupRows;
 constexpr uint32_t kBandRow = 3 * (kCompactChunk + 2);   // staged slots x 3 floats, odd
 typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
-// Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format,
-// no swizzle; loads beyond the resource's byte size return zero.
-constexpr int kBufferWord3 = 0x00020000;
 
 // The body of one write workgroup; `block` is its workgroup index within the
 // launch of `a` (blockIdx.x for a launch of its own).

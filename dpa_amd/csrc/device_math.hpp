@@ -48,7 +48,24 @@ struct FieldView {
   // index fit 32 bits, so a reload forms its offset in 32-bit arithmetic and
   // adds it to the base once
   uint32_t idx32;
+  // Byte sizes of the node array (all slices) and of the cell layout when a
+  // reload can address them with 32-bit byte offsets formed by 24-bit
+  // multiplies (the cell layout as one buffer resource): the size is below
+  // 2^32 and every factor of the offset's products below 2^24 (node_byte24,
+  // cell_byte24); 0 otherwise.
+  uint32_t node_bytes32, cell_bytes32;
 };
+
+// Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format,
+// no swizzle; loads beyond the resource's byte size return zero.
+constexpr int kBufferWord3 = 0x00020000;
+
+// a * b for a, b < 2^24 and a product below 2^32: one full-rate 24-bit
+// multiply (v_mul_u32_u24 / v_mad_u32_u24) where a 32-bit multiply is a
+// quarter-rate instruction.  The masks tell the compiler the operand ranges.
+__device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) {
+  return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+}
 
 enum Status : uint8_t { kActive = 0, kCompleted = 1, kExited = 2, kZero = 3 };
 enum Integrator : int { kDpaRk4 = 0, kRk4 = 1 };
@@ -100,22 +117,42 @@ __device__ __forceinline__ float div_by(float d, float h, float r) {
   return div_by_fast(d, h, r);
 }
 
-// div_by on three axes with one test for the rare operands: the three fast
-// quotients, then one branch (taken by no lane in practice) that divides the
-// axes whose operand is tiny, zero or NaN -- the same bits as three div_by
-// calls, with one divergent branch instead of three.
-__device__ __forceinline__ void div_by3(float d0, float h0, float r0, float d1, float h1, float r1,
-                                        float d2, float h2, float r2, float& q0, float& q1,
-                                        float& q2) {
-  q0 = div_by_fast(d0, h0, r0);
-  q1 = div_by_fast(d1, h1, r1);
-  q2 = div_by_fast(d2, h2, r2);
-  const bool t0 = div_tiny(d0), t1 = div_tiny(d1), t2 = div_tiny(d2);
-  if (t0 || t1 || t2) {
-    if (t0) q0 = d0 / h0;
-    if (t1) q1 = d1 / h1;
-    if (t2) q2 = d2 / h2;
-  }
+// kDivFast locates one axis with no test on the way: the quotient, its floor,
+// the remainder as one fmaf and the weight, all from the verified reciprocal.
+// Where that is not the correctly rounded result, the caller takes
+// exact_axis() in one branch for all axes; the branch is needed
+//   * when |d| < 2^-90 (zero, tiny or NaN): the fast quotient may differ from
+//     d / h (a tiny negative d whose quotient rounds to -0 lies in cell -0,
+//     inside; the fast floor may say -1);
+//   * when the cell is inside and some remainder m = d - |s| h is negative.
+// Otherwise d is not tiny, so the fast quotient is d / h's (the reciprocal
+// form was verified for every such d in the axis's range; beyond it both
+// floors lie outside the grid) and s is the floor of the rounded quotient, as
+// regular_grid.hpp computes it: the inside test on the fast s is the
+// reference's.  s is the floor k of the exact quotient or k + 1 (exact_fmod);
+// k + 1 makes the exact d - (k+1) h negative and non-zero -- a multiple of the
+// coarser of d's and h's ulps -- so its rounding m stays negative, and m >= 0
+// means s == k: m is fmod's exact result.  m is then +0 (a node plane: the
+// fast quotient of 0 is +0, as 0 / h) or at least the spacing's ulp when
+// s >= 1, or d itself when s == 0; fast division needs at least 2^-90, so
+// verified spacings are at least 2^-60 (field creation enforces it).  Lanes
+// outside the grid use no weight.
+__device__ __forceinline__ void fast_axis(float d, float h, float r, float& s, float& w,
+                                          float& m) {
+  s = floorf(div_by_fast(d, h, r));
+  m = __builtin_fmaf(-__builtin_fabsf(s), h, d);
+  w = div_by_fast(m, h, r);
+}
+
+// 1 or 0, integers so that the axes combine with | and & and no branch
+__device__ __forceinline__ uint32_t fast_axis_tiny(float d) {
+  return __builtin_fabsf(d) >= 0x1p-90f ? 0u : 1u;
+}
+__device__ __forceinline__ uint32_t fast_axis_roundup(float m) { return m < 0.0f ? 1u : 0u; }
+
+__device__ __forceinline__ void exact_axis(float d, float h, float r, float& s, float& w) {
+  s = floorf(div_by(d, h, r));
+  w = div_by(exact_fmod(d, h, s), h, r);
 }
 
 // Counts the floats x in [-B, B] (bit patterns 0..hi of |x|, both signs) where
@@ -240,6 +277,79 @@ __device__ __forceinline__ void load_cell(const float4* rec, V3& L0, V3& L1, V3&
   L4 = V3{d.x, d.y, d.z}; L5 = V3{d.w, e.x, e.y}; L6 = V3{e.z, e.w, g.x}; L7 = V3{g.y, g.z, g.w};
 }
 
+// Byte offset of node (c0, c1, c2) of slice 0, for fields with node_bytes32
+// (the plane's bytes 12 n1 n2 below 2^24; the offset below 2^32): three
+// 24-bit multiply-adds, the 12 bytes of a node folded into the factors.
+__device__ __forceinline__ uint32_t node_byte24(const FieldView& f, uint32_t c0, uint32_t c1,
+                                                uint32_t c2) {
+  return mul24(c0, 12u * (uint32_t)f.n1 * (uint32_t)f.n2) + mul24(c1, 12u * (uint32_t)f.n2) +
+         mul24(c2, 12u);
+}
+
+// Byte offset of cell record (c0, c1, c2) of slice 0, for fields with
+// cell_bytes32 (the brick counts, or the cell counts in C order, and their
+// plane products below 2^24).
+__device__ __forceinline__ uint32_t cell_byte24(const FieldView& f, uint32_t c0, uint32_t c1,
+                                                uint32_t c2) {
+  uint32_t rec;
+  if (f.cell_bits) {   // field-uniform
+    const uint32_t k = f.cell_bits, m = (1u << k) - 1u;
+    const uint32_t brick = mul24(c0 >> k, (uint32_t)f.cb1 * (uint32_t)f.cb2) +
+                           mul24(c1 >> k, (uint32_t)f.cb2) + (c2 >> k);
+    rec = (brick << (3 * k)) + (((c0 & m) << (2 * k)) | ((c1 & m) << k) | (c2 & m));
+  } else {
+    rec = mul24(c0, (uint32_t)(f.n1 - 1) * (uint32_t)(f.n2 - 1)) + mul24(c1, (uint32_t)(f.n2 - 1)) +
+          c2;
+  }
+  return rec << 7;   // 128 bytes per record
+}
+
+// Loads through a raw buffer resource over [base, base + bytes): voffset in
+// VGPRs, a uniform soffset, constant parts in the instruction's offset field.
+// The host build of this header (the tests' CPU checks) reads the same bytes
+// through a pointer.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __amdgpu_buffer_rsrc_t BufferRes;
+__device__ __forceinline__ BufferRes buffer_res(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           kBufferWord3);
+}
+__device__ __forceinline__ float4 buf_load4(BufferRes r, uint32_t voff, uint32_t soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                     __uint_as_float(v[3]));
+}
+#else
+typedef const char* BufferRes;
+__device__ __forceinline__ BufferRes buffer_res(const void* base, uint32_t) {
+  return (const char*)base;
+}
+__device__ __forceinline__ float4 buf_load4(BufferRes r, uint32_t voff, uint32_t soff) {
+  const float* p = (const float*)(r + voff + soff);
+  return make_float4(p[0], p[1], p[2], p[3]);
+}
+#endif
+
+// One node row of a cell (two nodes along axis 2, 24 bytes) at a 32-bit byte
+// offset from a uniform base: the loads take the base in scalar registers and
+// the offset in one vector register.
+__device__ __forceinline__ void load_row(const char* base, uint32_t off, V3& a, V3& b) {
+  const float* p = (const float*)(base + off);
+  a = V3{p[0], p[1], p[2]};
+  b = V3{p[3], p[4], p[5]};
+}
+
+// A cell record (six 16-byte pieces of one 128-byte line) from a buffer.
+__device__ __forceinline__ void load_cell_buf(BufferRes r, uint32_t voff, uint32_t soff, V3& L0,
+                                              V3& L1, V3& L2, V3& L3, V3& L4, V3& L5, V3& L6,
+                                              V3& L7) {
+  const float4 a = buf_load4(r, voff, soff), b = buf_load4(r, voff + 16u, soff);
+  const float4 c = buf_load4(r, voff + 32u, soff), d = buf_load4(r, voff + 48u, soff);
+  const float4 e = buf_load4(r, voff + 64u, soff), g = buf_load4(r, voff + 80u, soff);
+  L0 = V3{a.x, a.y, a.z}; L1 = V3{a.w, b.x, b.y}; L2 = V3{b.z, b.w, c.x}; L3 = V3{c.y, c.z, c.w};
+  L4 = V3{d.x, d.y, d.z}; L5 = V3{d.w, e.x, e.y}; L6 = V3{e.z, e.w, g.x}; L7 = V3{g.y, g.z, g.w};
+}
+
 // The eight corners of the cell a particle last sampled, kept in registers.
 // A particle moves a fraction of a cell per step, so most steps sample the
 // same cell again and need no memory access at all; only lanes that cross
@@ -265,10 +375,24 @@ __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
                                         CellCache& cc) {
   if (a0.s != cc.s0 || a1.s != cc.s1 || a2.s != cc.s2) {
     const int32_t c0 = (int32_t)a0.s, c1 = (int32_t)a1.s, c2 = (int32_t)a2.s;
-    if (f.cells) {   // field-uniform branch
+    if (f.cells && f.cell_bytes32) {   // field-uniform branches
+      load_cell_buf(buffer_res(f.cells, f.cell_bytes32), cell_byte24(f, c0, c1, c2), 0u, cc.L0,
+                    cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
+    } else if (f.cells) {
       const float4* rec = f.idx32 ? f.cells + 8u * cell_record32(f, 0u, c0, c1, c2)
                                   : f.cells + 8 * cell_record(f, 0, c0, c1, c2);
       load_cell(rec, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
+    } else if (f.node_bytes32) {   // (slice is f.data: every caller samples slice 0)
+      // one 32-bit byte offset per reload from the array's uniform base (a
+      // buffer resource here would add scalar registers that lattice
+      // instances spill); the four rows at uniform distances
+      const char* b = (const char*)f.data;
+      const uint32_t v = node_byte24(f, c0, c1, c2);
+      const uint32_t r1 = 12u * (uint32_t)f.n2, r2 = 12u * (uint32_t)f.n1 * (uint32_t)f.n2;
+      load_row(b, v, cc.L0, cc.L1);
+      load_row(b, v + r1, cc.L2, cc.L3);
+      load_row(b, v + r2, cc.L4, cc.L5);
+      load_row(b, v + (r2 + r1), cc.L6, cc.L7);
     } else {
       // one base address per reload; the four rows sit at launch-uniform
       // distances and each row's two nodes 12 bytes apart
@@ -313,12 +437,27 @@ __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, con
   if (a0.s != cc.s0 || a1.s != cc.s1 || a2.s != cc.s2 || at.s != cc.st) {
     const int32_t c0 = (int32_t)a0.s, c1 = (int32_t)a1.s, c2 = (int32_t)a2.s;
     const int32_t ct = (int32_t)at.s;
-    if (f.cells) {   // field-uniform branch
+    if (f.cells && f.cell_bytes32) {   // field-uniform branches
+      const BufferRes r = buffer_res(f.cells, f.cell_bytes32);
+      const uint32_t sb = (uint32_t)f.cell_slice << 7;   // bytes per slice
+      const uint32_t v = (uint32_t)ct * sb + cell_byte24(f, c0, c1, c2);
+      load_cell_buf(r, v, 0u, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
+      load_cell_buf(r, v, sb, cc.M0, cc.M1, cc.M2, cc.M3, cc.M4, cc.M5, cc.M6, cc.M7);
+    } else if (f.cells) {
       const float4* r0 = f.idx32 ? f.cells + 8u * cell_record32(f, ct, c0, c1, c2)
                                  : f.cells + 8 * cell_record(f, ct, c0, c1, c2);
       const float4* r1 = r0 + 8 * f.cell_slice;
       load_cell(r0, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
       load_cell(r1, cc.M0, cc.M1, cc.M2, cc.M3, cc.M4, cc.M5, cc.M6, cc.M7);
+    } else if (f.node_bytes32) {
+      const char* b = (const char*)f.data;
+      const uint32_t sb = 12u * (uint32_t)f.slice;   // bytes per slice
+      const uint32_t v = (uint32_t)ct * sb + node_byte24(f, c0, c1, c2);
+      const uint32_t r1 = 12u * (uint32_t)f.n2, r2 = 12u * (uint32_t)f.n1 * (uint32_t)f.n2;
+      load_row(b, v, cc.L0, cc.L1);                   load_row(b, v + sb, cc.M0, cc.M1);
+      load_row(b, v + r1, cc.L2, cc.L3);              load_row(b, v + (sb + r1), cc.M2, cc.M3);
+      load_row(b, v + r2, cc.L4, cc.L5);              load_row(b, v + (sb + r2), cc.M4, cc.M5);
+      load_row(b, v + (r2 + r1), cc.L6, cc.L7);       load_row(b, v + (sb + r2 + r1), cc.M6, cc.M7);
     } else {
       int64_t n2, n12, b;
       const float* s0;
@@ -362,6 +501,43 @@ __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, con
   return lerp3(B0, B1, w0, m0);
 }
 
+// The kDivFast location of a sample: every axis (and time, for a time-variant
+// field) on the fast path, then one branch for the lanes that need the exact
+// forms -- a tiny, zero or non-finite offset on any axis, or a cell inside the
+// grid with a quotient rounded up to the next integer on some axis
+// (fast_axis).  Returns 1 when the cell is inside.  dt is t - to (unused for
+// steady fields).
+template <bool kUnsteady>
+__device__ __forceinline__ uint32_t locate_fast(const FieldView& f, float d0, float d1, float d2,
+                                                float dt, Axis& a0, Axis& a1, Axis& a2,
+                                                Axis& at) {
+  float m0, m1, m2;
+  fast_axis(d0, f.h0, f.r0, a0.s, a0.w, m0);
+  fast_axis(d1, f.h1, f.r1, a1.s, a1.w, m1);
+  fast_axis(d2, f.h2, f.r2, a2.s, a2.w, m2);
+  uint32_t in = axis_inside(a0.s, f.n0) & axis_inside(a1.s, f.n1) & axis_inside(a2.s, f.n2);
+  uint32_t tiny = fast_axis_tiny(d0) | fast_axis_tiny(d1) | fast_axis_tiny(d2);
+  uint32_t up = fast_axis_roundup(m0) | fast_axis_roundup(m1) | fast_axis_roundup(m2);
+  if constexpr (kUnsteady) {
+    float mt;
+    fast_axis(dt, f.th, f.rt, at.s, at.w, mt);
+    in = in & axis_inside(at.s, f.nt);
+    tiny = tiny | fast_axis_tiny(dt);
+    up = up | fast_axis_roundup(mt);
+  }
+  if (tiny | (in & up)) {
+    exact_axis(d0, f.h0, f.r0, a0.s, a0.w);
+    exact_axis(d1, f.h1, f.r1, a1.s, a1.w);
+    exact_axis(d2, f.h2, f.r2, a2.s, a2.w);
+    in = axis_inside(a0.s, f.n0) & axis_inside(a1.s, f.n1) & axis_inside(a2.s, f.n2);
+    if constexpr (kUnsteady) {
+      exact_axis(dt, f.th, f.rt, at.s, at.w);
+      in = in & axis_inside(at.s, f.nt);
+    }
+  }
+  return in;
+}
+
 // contains + interpolate at (p, t).  Returns false when p (or t, for a
 // time-variant field) lies outside; v is the stored (unswizzled) vector.
 // kDiv: the launch's DivMode.
@@ -370,42 +546,45 @@ __device__ __forceinline__ bool sample(const FieldView& f, float p0, float p1, f
                                        V3& v, CellCache& cc) {
   const float d0 = p0 - f.o0, d1 = p1 - f.o1, d2 = p2 - f.o2;
   Axis a0, a1, a2;
-  if constexpr (kDiv == kDivFast) {   // axis_floor on three axes, one rare-operand branch
-    float q0, q1, q2;
-    div_by3(d0, f.h0, f.r0, d1, f.h1, f.r1, d2, f.h2, f.r2, q0, q1, q2);
-    a0.s = floorf(q0); a1.s = floorf(q1); a2.s = floorf(q2);
+  if constexpr (kDiv == kDivFast) {
+    Axis at;
+    const uint32_t in = locate_fast<kUnsteady>(f, d0, d1, d2, t - f.to, a0, a1, a2, at);
+    if constexpr (kUnsteady) {
+      if (!in) return false;
+      v = quadlinear(f, a0, a1, a2, at, cc);
+    } else {
+      (void)t;
+      if (!in) return false;
+      v = trilinear(f, f.data, a0, a1, a2, cc);
+    }
+    return true;
   } else {
     a0.s = axis_floor<kDiv>(d0, f.h0, f.r0);
     a1.s = axis_floor<kDiv>(d1, f.h1, f.r1);
     a2.s = axis_floor<kDiv>(d2, f.h2, f.r2);
-  }
-  // every axis tested together: one branch instead of one per axis
-  uint32_t in = axis_inside(a0.s, f.n0) & axis_inside(a1.s, f.n1) & axis_inside(a2.s, f.n2);
-  if constexpr (kUnsteady) {
-    const float dt = t - f.to;
-    Axis at;
-    at.s = axis_floor<kDiv>(dt, f.th, f.rt);
-    in = in & axis_inside(at.s, f.nt);
-    if (!in) return false;
-    a0.w = axis_weight<kDiv>(d0, a0.s, f.h0, f.r0);
-    a1.w = axis_weight<kDiv>(d1, a1.s, f.h1, f.r1);
-    a2.w = axis_weight<kDiv>(d2, a2.s, f.h2, f.r2);
-    at.w = axis_weight<kDiv>(dt, at.s, f.th, f.rt);
-    v = quadlinear(f, a0, a1, a2, at, cc);
-  } else {
-    (void)t;
-    if (!in) return false;
-    if constexpr (kDiv == kDivFast) {   // axis_weight on three axes, one rare-operand branch
-      div_by3(exact_fmod(d0, f.h0, a0.s), f.h0, f.r0, exact_fmod(d1, f.h1, a1.s), f.h1, f.r1,
-              exact_fmod(d2, f.h2, a2.s), f.h2, f.r2, a0.w, a1.w, a2.w);
-    } else {
+    // every axis tested together: one branch instead of one per axis
+    uint32_t in = axis_inside(a0.s, f.n0) & axis_inside(a1.s, f.n1) & axis_inside(a2.s, f.n2);
+    if constexpr (kUnsteady) {
+      const float dt = t - f.to;
+      Axis at;
+      at.s = axis_floor<kDiv>(dt, f.th, f.rt);
+      in = in & axis_inside(at.s, f.nt);
+      if (!in) return false;
       a0.w = axis_weight<kDiv>(d0, a0.s, f.h0, f.r0);
       a1.w = axis_weight<kDiv>(d1, a1.s, f.h1, f.r1);
       a2.w = axis_weight<kDiv>(d2, a2.s, f.h2, f.r2);
+      at.w = axis_weight<kDiv>(dt, at.s, f.th, f.rt);
+      v = quadlinear(f, a0, a1, a2, at, cc);
+    } else {
+      (void)t;
+      if (!in) return false;
+      a0.w = axis_weight<kDiv>(d0, a0.s, f.h0, f.r0);
+      a1.w = axis_weight<kDiv>(d1, a1.s, f.h1, f.r1);
+      a2.w = axis_weight<kDiv>(d2, a2.s, f.h2, f.r2);
+      v = trilinear(f, f.data, a0, a1, a2, cc);
     }
-    v = trilinear(f, f.data, a0, a1, a2, cc);
+    return true;
   }
-  return true;
 }
 
 // Eigen isZero() at float dummy_precision (particle_advector.cpp:363).

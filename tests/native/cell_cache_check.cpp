@@ -8,8 +8,10 @@
 // of two the reciprocal multiply with fmod as one fmaf (kDivExact); so every
 // mode is pinned against the correctly rounded divisions and fmodf, also at
 // boundary probes (signed zeros, denormals, faces, the last cell).  Each
-// axis's floor and weight are also pinned directly to floorf and fmodf.
-// Prints "samples S shifts F diffs D" and exits non-zero on a diff.
+// axis's floor and weight are also pinned directly to floorf and fmodf, and
+// kDivFast's fast path with its exact branch on spacings (0.1, 1/3, 2 pi / 511)
+// where rounded quotients land on the next integer at ordinary offsets.
+// Prints "samples S shifts F roundups R diffs D" and exits non-zero on a diff.
 #include <hip/hip_runtime.h>
 #undef __device__
 #define __device__ __attribute__((host)) __attribute__((device))
@@ -25,7 +27,12 @@ using namespace dpah;
 
 template <bool kUnsteady, int kDiv>
 static long walk(const FieldView& f, std::mt19937& rng, long& samples, long& shifts) {
-  const FieldView& plain = f;   // the cold reference divides on every axis
+  // the cold reference divides on every axis and reads the nodes through
+  // 64-bit pointer offsets (no cell layout, no buffer offsets)
+  FieldView plain = f;
+  plain.cells = nullptr;
+  plain.idx32 = 0;
+  plain.node_bytes32 = plain.cell_bytes32 = 0;
   std::uniform_real_distribution<float> u(-1.0f, 1.0f);
   std::uniform_real_distribution<float> p0(0.0f, (float)(f.n0 - 1)), p1(0.0f, (float)(f.n1 - 1)),
       p2(0.0f, (float)(f.n2 - 1)), pt(0.0f, (float)(f.nt - 1));
@@ -62,7 +69,10 @@ static long walk(const FieldView& f, std::mt19937& rng, long& samples, long& shi
 // (inside flag or value bits) that differ from the dividing path.
 template <int kDiv>
 static long probes(const FieldView& e, long& samples) {
-  const FieldView& plain = e;
+  FieldView plain = e;   // dividing, pointer reads of the nodes
+  plain.cells = nullptr;
+  plain.idx32 = 0;
+  plain.node_bytes32 = plain.cell_bytes32 = 0;
   const float tiny[] = {0.0f, -0.0f, 1e-45f, -1e-45f, 1e-38f, -1e-38f, 1e-30f, -1e-30f,
                         -1e-7f, 1e-7f};
   long diffs = 0;
@@ -132,6 +142,75 @@ static long axis_probes(float o, float h, int32_t n, std::mt19937& rng, long& sa
   return diffs;
 }
 
+// kDivFast's location as sample() runs it (locate_fast: the fast path, the
+// exact branch for tiny offsets and inside cells with a rounded-up quotient), against floorf(d / h) and fmodf(d, h) / h.  Offsets are
+// the boundary set of axis_probes plus, for every k < n, the floats nearest
+// below k h whose rounded quotient is k (the floor of the rounded quotient is
+// one above the exact floor there: on spacings such as 0.1, 1/3 or 2 pi / 511
+// this happens at ordinary positions like d = 0.5).  `roundups` counts those
+// offsets, so a run shows that it met them.
+static long split_probes(float h, int32_t n, std::mt19937& rng, long& samples, long& roundups) {
+  // the same offset on all three axes (and time) of a grid with spacing h
+  FieldView g{};
+  g.n0 = g.n1 = g.n2 = g.nt = n;
+  g.h0 = g.h1 = g.h2 = g.th = h;
+  g.r0 = g.r1 = g.r2 = g.rt = 1.0f / h;
+  const float tiny[] = {0.0f, -0.0f, 1e-45f, -1e-45f, 1e-38f, -1e-38f, 1e-30f, -1e-30f,
+                        -1e-7f, 1e-7f};
+  std::vector<float> xs(std::begin(tiny), std::end(tiny));
+  for (int k = 0; k < n; ++k) {
+    const double exact = (double)k * (double)h;   // exact: a float times a small integer
+    float d = (float)exact;
+    for (int j = 0; j < 4; ++j) d = std::nextafter(d, -1e30f);
+    for (int j = 0; j < 9; ++j) {
+      xs.push_back(d);
+      if ((double)d < exact && floorf(d / h) >= (float)k) ++roundups;
+      d = std::nextafter(d, 1e30f);
+    }
+  }
+  std::uniform_real_distribution<float> u(-h, h * (float)n);
+  for (int k = 0; k < 20000; ++k) xs.push_back(u(rng));
+  long diffs = 0;
+  for (float d : xs) {
+    Axis a[4];
+    const uint32_t in = locate_fast<true>(g, d, d, d, d, a[0], a[1], a[2], a[3]);
+    const float ref_s = floorf(d / h);
+    ++samples;
+    if (in != axis_inside(ref_s, n)) ++diffs;
+    for (const Axis& x : a) {
+      if (std::memcmp(&x.s, &ref_s, 4) != 0) ++diffs;
+      if (!axis_inside(ref_s, n)) continue;
+      const float ref_w = fmodf(d, h) / h;
+      if (std::memcmp(&x.w, &ref_w, 4) != 0) ++diffs;
+    }
+  }
+  return diffs;
+}
+
+// The cell layout of a steady field as the library's k_build_cells writes it:
+// record cell_record(c0, c1, c2) (C order, or bricks of 2^k cells per axis)
+// holds the eight corners in order 4a + 2b + c, 24 floats padded to 32.
+static std::vector<float4> build_cells(FieldView& f, const std::vector<float>& nodes, uint32_t k) {
+  auto bricks = [&](int32_t n) { return (int32_t)(((uint32_t)n - 1 + (1u << k) - 1) >> k); };
+  f.cell_bits = k;
+  f.cb1 = k ? bricks(f.n1) : 0;
+  f.cb2 = k ? bricks(f.n2) : 0;
+  f.cell_slice = k ? ((int64_t)bricks(f.n0) * f.cb1 * f.cb2) << (3 * k)
+                   : (int64_t)(f.n0 - 1) * (f.n1 - 1) * (f.n2 - 1);
+  std::vector<float4> cells(8 * (size_t)f.cell_slice, make_float4(0, 0, 0, 0));
+  for (int32_t c0 = 0; c0 + 1 < f.n0; ++c0)
+    for (int32_t c1 = 0; c1 + 1 < f.n1; ++c1)
+      for (int32_t c2 = 0; c2 + 1 < f.n2; ++c2) {
+        float* rec = (float*)&cells[8 * (size_t)cell_record(f, 0, c0, c1, c2)];
+        for (int q = 0; q < 8; ++q) {
+          const size_t node = ((size_t)(c0 + (q >> 2)) * f.n1 + (c1 + ((q >> 1) & 1))) * f.n2 +
+                              (c2 + (q & 1));
+          for (int c = 0; c < 3; ++c) rec[3 * q + c] = nodes[3 * node + c];
+        }
+      }
+  return cells;
+}
+
 int main() {
   const int n0 = 19, n1 = 23, n2 = 29, nt = 5;
   std::vector<float> data(3ull * n0 * n1 * n2 * nt);
@@ -192,6 +271,44 @@ int main() {
   diffs += probes<kDivFast>(e0, samples);
   diffs += probes<kDivFast>(f0, samples);
   diffs += probes<kDivFast>(unit, samples);
-  std::printf("samples %ld shifts %ld diffs %ld\n", samples, shifts, diffs);
+  // spacings whose rounded quotients land on the next integer at ordinary
+  // offsets (the verified-reciprocal spacings of the tests' fields); the
+  // kDivFast sample on such a grid against the dividing one, at its faces
+  // and their neighbours
+  long roundups = 0;
+  for (float h : {0.1f, 1.0f / 3.0f, (float)(2.0 * M_PI / 511.0), (float)(2.0 * M_PI / 63.0),
+                  0.75f, 3.0f})
+    diffs += split_probes(h, 200, rng, samples, roundups);
+  FieldView r = f0;
+  r.h0 = 0.1f; r.h1 = 1.0f / 3.0f; r.h2 = (float)(2.0 * M_PI / 511.0);
+  r.r0 = 1.0f / r.h0; r.r1 = 1.0f / r.h1; r.r2 = 1.0f / r.h2;
+  diffs += probes<kDivFast>(r, samples);
+  diffs += walk<false, kDivFast>(r, rng, samples, shifts);
+  // buffer reloads (24-bit offset arithmetic, byte offsets, rows in soffset)
+  // against the pointer path: the node array of steady and time-variant
+  // fields, and cell layouts in C order and in bricks of 2 and 8 cells
+  {
+    FieldView b = f;
+    b.node_bytes32 = (uint32_t)(data.size() * sizeof(float));
+    diffs += walk<false, kDivFast>(b, rng, samples, shifts);
+    diffs += walk<false, kDivide>(b, rng, samples, shifts);
+    FieldView bt = g;
+    bt.node_bytes32 = (uint32_t)(data.size() * sizeof(float));
+    diffs += walk<true, kDivide>(bt, rng, samples, shifts);
+    diffs += walk<true, kDivFast>(bt, rng, samples, shifts);
+    for (uint32_t k : {0u, 1u, 3u}) {
+      FieldView c = f;
+      const std::vector<float4> cells = build_cells(c, data, k);
+      c.cells = cells.data();
+      c.cell_bytes32 = (uint32_t)(cells.size() * sizeof(float4));
+      diffs += walk<false, kDivFast>(c, rng, samples, shifts);
+      diffs += probes<kDivFast>(c, samples);
+    }
+  }
+  std::printf("samples %ld shifts %ld roundups %ld diffs %ld\n", samples, shifts, roundups, diffs);
+  if (roundups < 100) {
+    std::printf("too few offsets whose rounded quotient is one above the floor\n");
+    return 1;
+  }
   return diffs != 0;
 }

@@ -44,8 +44,8 @@ def build_and_run(tmp_path, name):
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
 def test_warm_cache_equals_full_reload(tmp_path):
     words = build_and_run(tmp_path, "cell_cache_check").split()
-    samples, shifts = int(words[1]), int(words[3])
-    assert samples > 100000 and shifts > 10000
+    samples, shifts, roundups = int(words[1]), int(words[3]), int(words[5])
+    assert samples > 100000 and shifts > 10000 and roundups >= 100
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
