@@ -695,7 +695,8 @@ def run_ftle(args, dpa_mode=False):
         rec, node = ctx.cell_loads()
     finally:
         ctx.set_count_cell_loads(False)
-    particles = points * (1 if dpa_mode else 6)
+    particles = points if dpa_mode else dpa_amd.aux_particles(c["lattice_spacing"], c["delta"],
+                                                              shape)
     # loads + each particle's final position (lattice seeds are computed, not read)
     alg = rec * RECORD_BYTES + node * NODE_GROUP_BYTES + 12 * particles
     log(f"cell loads per pass: {rec} records, {node} node groups ({(rec + node) / steps:.3f} "
@@ -708,7 +709,8 @@ def run_ftle(args, dpa_mode=False):
     out["config"] = {"workload": ("FTLE dpa lattice mode" if dpa_mode else
                                   "FTLE auxiliary +-0.5 cell neighbours") +
                                  ", ABC 512x512x128, T=4 at dt 0.02",
-                     "lattice_points": points, "particle_steps_per_pass": steps,
+                     "lattice_points": points, "particles_per_pass": particles,
+                     "particle_steps_per_pass": steps,
                      "points_per_s": args.steps * points / elapsed,
                      "tuning": tuned or "library defaults"}
     out["roofline"] = roofline(adv_ms, alg, steps)

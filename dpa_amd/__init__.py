@@ -689,6 +689,21 @@ class Server:
         return {k: getattr(s, k) for k, _ in _ServerStats._fields_}
 
 
+def aux_particles(spacing, delta, shape) -> int:
+    """Particles the library advects for an auxiliary-neighbour FTLE lattice of
+    `shape` points: with delta half the spacing on every axis (float32, every
+    axis below 2^24 points) a particle between two neighbouring points is one
+    particle -- three per point plus the -delta particles of the low faces --
+    else six per point."""
+    sp = np.asarray(spacing, np.float32).reshape(3)
+    dl = np.asarray(delta, np.float32).reshape(3)
+    d0, d1, d2 = (int(x) for x in shape)
+    if all(dl[i] > 0 and np.float32(2.0) * dl[i] == sp[i] and int(shape[i]) < (1 << 24)
+           for i in range(3)):
+        return 3 * d0 * d1 * d2 + d1 * d2 + d0 * d2 + d0 * d1
+    return 6 * d0 * d1 * d2
+
+
 def result_ftle(res: Result) -> np.ndarray:
     n = C.c_uint64()
     _check(res.ctx.lib.dpah_result_ftle(res.h, None, C.byref(n)))

@@ -156,21 +156,22 @@ bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch
 }
 
 // Lattices need no sort: the kernel unranks queue positions into 4x4x4 bricks
-// of lattice points (lattice_brick_index), six consecutive particles per point
-// for the auxiliary form.  A lattice in C order gives a wave 64 points along
+// of lattice points (lattice_brick_index), a point's auxiliary particles
+// consecutive (six, or three when shared; the shared lattice's face particles
+// follow in order).  A lattice in C order gives a wave 64 points along
 // one row; bricks give it a 4x4x4 block (dpa-mode FTLE on 512x512x128: 16.1 ms
 // advection, 39.7 ms in C order).  Whole lattices in one launch only, since
 // the walk is over the whole lattice and the outputs are indexed by lattice
 // point.
 uint32_t lattice_brick(const Tuning& T, int seed_kind) {
   if (T.lattice_brick) return (uint32_t)T.lattice_brick;
-  return seed_kind == kSeedAux ? kAuxBrick : kLatticeBrick;
+  return seed_kind == kSeedLattice ? kLatticeBrick : kAuxBrick;
 }
 
 // Bricks per group edge of the grouped walk; 1: no groups.
 uint32_t lattice_group(const Tuning& T, int seed_kind) {
   if (T.lattice_group) return (uint32_t)T.lattice_group;
-  return seed_kind == kSeedAux ? 1u : kLatticeGroup;
+  return seed_kind == kSeedLattice ? kLatticeGroup : 1u;
 }
 
 bool brick_launch(const Tuning& T, const AdvectLaunch& L) {
@@ -183,6 +184,8 @@ bool brick_launch(const Tuning& T, const AdvectLaunch& L) {
     return false;
   if (L.seed_kind == kSeedLattice) return L.n == points && L.n < (1ull << 32);
   if (L.seed_kind == kSeedAux) return L.n == 6 * points && L.n < (1ull << 32);
+  if (L.seed_kind == kSeedAuxShared)
+    return L.n == aux_shared_count(L.lat.d0, L.lat.d1, L.lat.d2) && L.n < (1ull << 32);
   return false;
 }
 

@@ -18,6 +18,8 @@ enum SeedKind : int {
   kSeedArray = 0,    // seeds[pid]
   kSeedLattice = 1,  // lattice point pid in C order (uniform_seed_generator.cpp:33-36)
   kSeedAux = 2,      // pid = 6*point + a: lattice point +- delta along axis a/2
+  kSeedAuxShared = 3,  // delta = stride / 2: each particle shared by two points
+                       // is advected once (aux_shared_particle)
 };
 
 // Field layouts a persistent kernel instance may assume (k_advect_persistent):
@@ -289,18 +291,80 @@ __device__ __forceinline__ void put_slot(const Lane& L, uint32_t j, float x, flo
   }
 }
 
+// Multi-index of lattice point i (plus the launch's first point) in C order
+// (indexing.hpp:10-20).
+__device__ __forceinline__ void lattice_unravel(const SeedLattice& g, uint64_t i, uint64_t& i0,
+                                                uint64_t& i1, uint64_t& i2) {
+  i += g.first;
+  i2 = i % g.d2;
+  const uint64_t r = i / g.d2;
+  i1 = r % g.d1;
+  i0 = r / g.d1;
+}
+
 // position = offset + stride * multi_index, each product and sum rounded
-// (uniform_seed_generator.cpp:35-36 with indexing.hpp:10-20's C-order unravel).
+// (uniform_seed_generator.cpp:35-36).
 __device__ __forceinline__ void lattice_point(const SeedLattice& g, uint64_t i, float& p0,
                                               float& p1, float& p2) {
-  i += g.first;
-  const uint64_t i2 = i % g.d2;
-  const uint64_t r = i / g.d2;
-  const uint64_t i1 = r % g.d1;
-  const uint64_t i0 = r / g.d1;
+  uint64_t i0, i1, i2;
+  lattice_unravel(g, i, i0, i1, i2);
   { const float m = g.st0 * (float)i0; p0 = g.lo0 + m; }
   { const float m = g.st1 * (float)i1; p1 = g.lo1 + m; }
   { const float m = g.st2 * (float)i2; p2 = g.lo2 + m; }
+}
+
+// The auxiliary particle of lattice point `point` moved by +delta (plus) or
+// -delta along axis ax: that coordinate is lo + fl(st k +- delta), the offset
+// position rounded once (an fmaf), the others the lattice point's.  With
+// delta = st / 2 the +delta particle of point k and the -delta particle of
+// point k + 1 are the same real number st k + delta, so the same float.
+__device__ __forceinline__ void aux_particle(const SeedLattice& g, uint64_t point, uint32_t ax,
+                                             bool plus, float& p0, float& p1, float& p2) {
+  uint64_t i0, i1, i2;
+  lattice_unravel(g, point, i0, i1, i2);
+  { const float m = g.st0 * (float)i0; p0 = g.lo0 + m; }
+  { const float m = g.st1 * (float)i1; p1 = g.lo1 + m; }
+  { const float m = g.st2 * (float)i2; p2 = g.lo2 + m; }
+  if (ax == 0) {
+    const float m = __builtin_fmaf(g.st0, (float)i0, plus ? g.dl0 : -g.dl0); p0 = g.lo0 + m;
+  } else if (ax == 1) {
+    const float m = __builtin_fmaf(g.st1, (float)i1, plus ? g.dl1 : -g.dl1); p1 = g.lo1 + m;
+  } else {
+    const float m = __builtin_fmaf(g.st2, (float)i2, plus ? g.dl2 : -g.dl2); p2 = g.lo2 + m;
+  }
+}
+
+// Particles of an auxiliary lattice whose offsets are half its stride, over
+// d0 whole slabs along axis 0 (points first .. first + d0 d1 d2 - 1, first a
+// multiple of d1 d2): particle 3 p + a (p < N = d0 d1 d2, chunk-local C order)
+// is point p's +delta particle along axis a, which is also point p + e_a's
+// -delta particle; then the -delta particles of the points on the chunk's
+// low faces, which no point inside the chunk owns: axis 0 (d1 d2 of them, C
+// order over (i1, i2)), axis 1 (d0 d2, over (i0, i2)), axis 2 (d0 d1, over
+// (i0, i1)).
+__host__ __device__ inline uint64_t aux_shared_count(uint64_t d0, uint64_t d1, uint64_t d2) {
+  return 3 * d0 * d1 * d2 + d1 * d2 + d0 * d2 + d0 * d1;
+}
+
+__device__ __forceinline__ void aux_shared_particle(const SeedLattice& g, uint64_t pid, float& p0,
+                                                    float& p1, float& p2) {
+  const uint64_t n3 = 3 * g.d0 * g.d1 * g.d2;
+  if (pid < n3) {
+    const uint64_t point = pid / 3;
+    aux_particle(g, point, (uint32_t)(pid - 3 * point), true, p0, p1, p2);
+    return;
+  }
+  uint64_t f = pid - n3, point;
+  uint32_t ax;
+  if (f < g.d1 * g.d2) {                       // (0, i1, i2) = f
+    ax = 0; point = f;
+  } else if ((f -= g.d1 * g.d2) < g.d0 * g.d2) {   // (i0, 0, i2), f = i0 d2 + i2
+    ax = 1; point = (f / g.d2) * g.d1 * g.d2 + f % g.d2;
+  } else {                                     // (i0, i1, 0), f = i0 d1 + i1
+    f -= g.d0 * g.d2;
+    ax = 2; point = f * g.d2;
+  }
+  aux_particle(g, point, ax, false, p0, p1, p2);
 }
 
 // kLattice false: the launch seeds from an array (a.seed_kind is kSeedArray),
@@ -313,16 +377,12 @@ __device__ __forceinline__ void seed_position(const AdvectArgs& a, uint64_t pid,
     p0 = s[0]; p1 = s[1]; p2 = s[2];
   } else if (a.seed_kind == kSeedLattice) {
     lattice_point(a.lat, pid, p0, p1, p2);
-  } else {
+  } else if (a.seed_kind == kSeedAux) {
     const uint64_t point = pid / 6;
     const uint32_t aux = (uint32_t)(pid - 6 * point);
-    lattice_point(a.lat, point, p0, p1, p2);
-    const bool plus = aux & 1;
-    switch (aux >> 1) {
-      case 0: p0 = plus ? p0 + a.lat.dl0 : p0 - a.lat.dl0; break;
-      case 1: p1 = plus ? p1 + a.lat.dl1 : p1 - a.lat.dl1; break;
-      default: p2 = plus ? p2 + a.lat.dl2 : p2 - a.lat.dl2; break;
-    }
+    aux_particle(a.lat, point, aux >> 1, (aux & 1) != 0, p0, p1, p2);
+  } else {
+    aux_shared_particle(a.lat, pid, p0, p1, p2);
   }
 }
 
@@ -339,6 +399,12 @@ __device__ __forceinline__ uint64_t resolve(const AdvectArgs& a, uint64_t q, flo
         const uint32_t point = (uint32_t)q / 6u;
         return 6 * lattice_walk_index(d0, d1, d2, a.lat.brick, a.lat.group, point) +
                ((uint32_t)q - 6u * point);
+      }
+      if (a.seed_kind == kSeedAuxShared) {   // points' particles walked, faces in order
+        if ((uint32_t)q >= 3u * d0 * d1 * d2) return q;
+        const uint32_t point = (uint32_t)q / 3u;
+        return 3 * lattice_walk_index(d0, d1, d2, a.lat.brick, a.lat.group, point) +
+               ((uint32_t)q - 3u * point);
       }
       return lattice_walk_index(d0, d1, d2, a.lat.brick, a.lat.group, (uint32_t)q);
     }
@@ -463,7 +529,7 @@ __global__ __launch_bounds__(256) void k_advect_simple(AdvectArgs a) {
 // a.loads_out (the roofline's byte count; a measurement instance, for steady
 // dpa-mode endpoint launches: one sample per step, so a step reloaded exactly
 // when it changed the cache's cell key).
-// kLattice: the launch seeds a lattice (a.seed_kind kSeedLattice or kSeedAux,
+// kLattice: the launch seeds a lattice (a.seed_kind kSeedLattice, kSeedAux or kSeedAuxShared,
 // endpoint launches only); false compiles the lattice seeding and walk out of
 // a seed-array launch's refill, where their uniform values would otherwise
 // add to the scalar registers the step loop keeps.

@@ -163,6 +163,37 @@ DPAH_EXPORT int dpah_ftle_lattice(dpah_context* ctx, const dpah_field* field, co
   });
 }
 
+int dpah::aux_lattice(SeedLattice& lat, const float origin[3], const float lattice_spacing[3],
+                      const uint64_t shape[3], const float delta[3], uint64_t* particles) {
+  lat = SeedLattice{};
+  lat.lo0 = origin[0]; lat.lo1 = origin[1]; lat.lo2 = origin[2];
+  lat.st0 = lattice_spacing[0]; lat.st1 = lattice_spacing[1]; lat.st2 = lattice_spacing[2];
+  lat.d0 = shape[0]; lat.d1 = shape[1]; lat.d2 = shape[2];
+  lat.dl0 = delta[0]; lat.dl1 = delta[1]; lat.dl2 = delta[2];
+  bool shared = true;
+  for (int i = 0; i < 3; ++i)   // st = 2 delta exactly, so st - delta = delta
+    shared = shared && delta[i] > 0.0f && 2.0f * delta[i] == lattice_spacing[i] &&
+             shape[i] < (1ull << 24);
+  const uint64_t points = shape[0] * shape[1] * shape[2];
+  if (particles) *particles = shared ? aux_shared_count(shape[0], shape[1], shape[2]) : 6 * points;
+  return shared ? kSeedAuxShared : kSeedAux;
+}
+
+void dpah::aux_reduce(const float* finals, int kind, const SeedLattice& lat, const float delta[3],
+                      float step_size, uint64_t iterations, float* out, hipStream_t s) {
+  FtleAuxArgs a{};
+  a.finals = finals;
+  a.points = lat.d0 * lat.d1 * lat.d2;
+  a.twod0 = 2.0f * delta[0]; a.twod1 = 2.0f * delta[1]; a.twod2 = 2.0f * delta[2];
+  a.T_abs = ftle_time(step_size, iterations);
+  a.out = out;
+  a.shared = kind == kSeedAuxShared ? 1u : 0u;
+  a.d0 = lat.d0; a.d1 = lat.d1; a.d2 = lat.d2;
+  if (a.points)
+    hipLaunchKernelGGL(k_ftle_aux_reduce, dim3(grid_for(a.points)), dim3(256), 0, s, a);
+  DPAH_HIP(hipGetLastError());
+}
+
 void dpah::ftle_aux_into(dpah_context* ctx, const dpah_field* field, const float origin[3],
                          const float lattice_spacing[3], const uint64_t lattice_shape[3],
                          const float delta[3], uint64_t iterations, float step_size,
@@ -176,40 +207,46 @@ void dpah::ftle_aux_into(dpah_context* ctx, const dpah_field* field, const float
   DPAH_REQUIRE(iterations < (1ull << 32) - 2, "iterations must be below 2^32 - 2");
   const uint64_t points = lattice_shape[0] * lattice_shape[1] * lattice_shape[2];
   DPAH_REQUIRE(points == 0 || ftle_out, "ftle_out is null");
-  SeedLattice lat{};
-  lat.lo0 = origin[0]; lat.lo1 = origin[1]; lat.lo2 = origin[2];
-  lat.st0 = lattice_spacing[0]; lat.st1 = lattice_spacing[1]; lat.st2 = lattice_spacing[2];
-  lat.d0 = lattice_shape[0]; lat.d1 = lattice_shape[1]; lat.d2 = lattice_shape[2];
-  lat.dl0 = delta[0]; lat.dl1 = delta[1]; lat.dl2 = delta[2];
-  // points are processed in chunks of at most 2^25 (2.4 GB of flow map)
-  const uint64_t chunk = std::min<uint64_t>(points, 1ull << 25);
-  DeviceBuffer finals(ctx, std::max<uint64_t>(chunk, 1) * 18 * sizeof(float));
+  SeedLattice lat;
+  const int kind = aux_lattice(lat, origin, lattice_spacing, lattice_shape, delta, nullptr);
+  // points are processed in chunks of at most 2^25 whose final positions take
+  // at most a quarter of the memory budget; with shared particles in whole
+  // slabs along axis 0 (each chunk advects its own low-face particles, so a
+  // chunk after the first repeats one slab of the previous chunk's +delta
+  // particles along axis 0, and its particle-steps are counted again)
+  const uint64_t plane = lat.d1 * lat.d2;
+  const bool shared = kind == kSeedAuxShared && plane <= (1ull << 25);
+  const uint64_t per_point = (shared ? 3 : 6) * 3 * sizeof(float);
+  uint64_t cap = std::min<uint64_t>(1ull << 25,
+                                    std::max<uint64_t>(ctx->memory_budget / 4 / per_point, 1));
+  if (shared) cap = std::max<uint64_t>(cap / plane, 1) * plane;
+  const uint64_t chunk = std::min<uint64_t>(points, cap);
+  const uint64_t per_chunk =
+      shared ? aux_shared_count(chunk / plane, lat.d1, lat.d2) : 6 * chunk;
+  DeviceBuffer finals(ctx, std::max<uint64_t>(per_chunk, 1) * 3 * sizeof(float));
   uint64_t steps = 0;
   for (uint64_t p0 = 0; p0 < points; p0 += chunk) {
     const uint64_t pn = std::min(chunk, points - p0);
     AdvectLaunch L{};
     L.f = &field->view;
-    L.seed_kind = kSeedAux;
+    L.seed_kind = shared ? kSeedAuxShared : kSeedAux;
     L.lat = lat;
     L.lat.first = p0;
+    if (shared) L.lat.d0 = pn / plane;   // the chunk's slabs
     L.uniform_rem = iterations;
     L.pos_out = finals.as<float>();
-    L.n = 6 * pn;
+    L.n = shared ? aux_shared_count(L.lat.d0, lat.d1, lat.d2) : 6 * pn;
     L.dt = step_size;
     L.t0 = start_time;
     L.integrator = integrator;
     L.persistent = persistent;
     launch_advect(ctx, L);
-    FtleAuxArgs a{};
-    a.finals = finals.as<float>();
-    a.points = pn;
-    a.twod0 = 2.0f * delta[0]; a.twod1 = 2.0f * delta[1]; a.twod2 = 2.0f * delta[2];
-    a.T_abs = ftle_time(step_size, iterations);
-    a.out = ftle_out + p0;
+    SeedLattice part = L.lat;
+    if (!shared) { part.d0 = pn; part.d1 = 1; part.d2 = 1; }   // six per point: points only
     ctx->timed("ftle_aux_reduce", [&] {
-      hipLaunchKernelGGL(k_ftle_aux_reduce, dim3(grid_for(pn)), dim3(256), 0, ctx->stream, a);
+      aux_reduce(finals.as<float>(), L.seed_kind, part, delta, step_size, iterations,
+                 ftle_out + p0, ctx->stream);
     });
-    DPAH_HIP(hipGetLastError());
     unsigned long long s = 0;
     DPAH_HIP(hipMemcpyAsync(&s, ctx->d_counters + 1, sizeof(s), hipMemcpyDeviceToHost,
                             ctx->stream));

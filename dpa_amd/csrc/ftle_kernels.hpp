@@ -57,24 +57,46 @@ __device__ __forceinline__ float ftle_from_jacobian(const float J[3][3], float T
 // ---------------------------------------------------------- auxiliary ----
 
 struct FtleAuxArgs {
-  const float* finals;   // [points][6][3]: -d0, +d0, -d1, +d1, -d2, +d2
+  // six per point, [points][6][3]: -d0, +d0, -d1, +d1, -d2, +d2; or, with
+  // `shared`, the particles of aux_shared_particle over a d0 x d1 x d2 chunk
+  const float* finals;
   uint64_t points;
   float twod0, twod1, twod2;      // fl(2 * delta_d)
   float T_abs;
   float* out;            // [points]
+  uint32_t shared;
+  uint64_t d0, d1, d2;
 };
 
+// Per point: J[:, d] = (final(+delta_d) - final(-delta_d)) / (2 delta_d), then
+// C = J^T J, lambda_max and ln(sqrt(lambda_max)) / T (ftle_from_jacobian).
 static __global__ __launch_bounds__(256) void k_ftle_aux_reduce(FtleAuxArgs a) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.points) return;
-  const float* f = a.finals + 18 * p;
+  const float* lo[3];
+  const float* hi[3];
+  if (a.shared) {
+    // point p's own +delta particles; its -delta particles are its lower
+    // neighbours' +delta ones, or the chunk's face particles on a low face
+    const uint64_t i2 = p % a.d2, r = p / a.d2, i1 = r % a.d1, i0 = r / a.d1;
+    const uint64_t f0 = 3 * a.points, f1 = f0 + a.d1 * a.d2, f2 = f1 + a.d0 * a.d2;
+    for (int d = 0; d < 3; ++d) hi[d] = a.finals + 3 * (3 * p + d);
+    lo[0] = a.finals + 3 * (i0 > 0 ? 3 * (p - a.d1 * a.d2) + 0 : f0 + i1 * a.d2 + i2);
+    lo[1] = a.finals + 3 * (i1 > 0 ? 3 * (p - a.d2) + 1 : f1 + i0 * a.d2 + i2);
+    lo[2] = a.finals + 3 * (i2 > 0 ? 3 * (p - 1) + 2 : f2 + i0 * a.d1 + i1);
+  } else {
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = a.finals + 18 * p + 6 * d;
+      hi[d] = lo[d] + 3;
+    }
+  }
   float J[3][3];
   const float den[3] = {a.twod0, a.twod1, a.twod2};
 #pragma unroll
   for (int d = 0; d < 3; ++d)
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
-      const float diff = f[6 * d + 3 + e] - f[6 * d + e];
+      const float diff = hi[d][e] - lo[d][e];
       J[e][d] = diff / den[d];
     }
   a.out[p] = ftle_from_jacobian(J, a.T_abs);

@@ -352,6 +352,18 @@ dpah_result* run_advect(dpah_context* ctx, const dpah_field* field, const float*
                         const uint64_t* remaining, uint64_t uniform_remaining, uint64_t count,
                         const dpah_advect_params* params, bool persistent);
 
+// The particles of an auxiliary-neighbour FTLE lattice of `shape` points:
+// with delta half the lattice spacing on every axis (and every axis below
+// 2^24 points) a particle between two neighbouring points is advected once
+// (kSeedAuxShared: 3 per point plus the low faces' -delta particles), else six
+// per point (kSeedAux).  Fills the lattice's offsets and returns the seed kind.
+int aux_lattice(SeedLattice& lat, const float origin[3], const float lattice_spacing[3],
+                const uint64_t shape[3], const float delta[3], uint64_t* particles);
+// The per-point reduction of an auxiliary lattice's final positions (layout of
+// seed kind `kind` over lat's d0 x d1 x d2 points) into out, on stream s.
+void aux_reduce(const float* finals, int kind, const SeedLattice& lat, const float delta[3],
+                float step_size, uint64_t iterations, float* out, hipStream_t s);
+
 // Auxiliary-neighbour FTLE into a device volume; run_ftle_aux returns it in a
 // result (the serving engine's per-request form).
 void ftle_aux_into(dpah_context* ctx, const dpah_field* field, const float origin[3],

@@ -292,13 +292,8 @@ void dpah_server::setup(Request& r, hipStream_t s) {
   ia.status = r.status.as<uint8_t>();
   ia.special = r.special.as<uint8_t>();
   if (ftle) {
-    ia.seed_kind = kSeedAux;
-    SeedLattice& L = ia.lat;
-    L.lo0 = r.spec.origin[0]; L.lo1 = r.spec.origin[1]; L.lo2 = r.spec.origin[2];
-    L.st0 = r.spec.lattice_spacing[0]; L.st1 = r.spec.lattice_spacing[1];
-    L.st2 = r.spec.lattice_spacing[2];
-    L.d0 = r.spec.lattice_shape[0]; L.d1 = r.spec.lattice_shape[1]; L.d2 = r.spec.lattice_shape[2];
-    L.dl0 = r.spec.delta[0]; L.dl1 = r.spec.delta[1]; L.dl2 = r.spec.delta[2];
+    ia.seed_kind = aux_lattice(ia.lat, r.spec.origin, r.spec.lattice_spacing,
+                               r.spec.lattice_shape, r.spec.delta, nullptr);
     ia.uniform_rem = r.spec.iterations;
   } else {
     // staged inputs live until the request finishes (another stream may reuse
@@ -576,17 +571,11 @@ void dpah_server::finish_all(std::vector<std::shared_ptr<Request>>& done, hipStr
         const uint64_t P = ftle_points(r.spec);
         o.ftle_count = P;
         o.d_ftle = static_cast<float*>(ctx->alloc(std::max<uint64_t>(P, 1) * sizeof(float)));
-        FtleAuxArgs a{};
-        a.finals = r.pos.as<float>();
-        a.points = P;
-        a.twod0 = 2.0f * r.spec.delta[0]; a.twod1 = 2.0f * r.spec.delta[1];
-        a.twod2 = 2.0f * r.spec.delta[2];
-        a.T_abs = std::fabs(r.spec.params.step_size * (float)r.spec.iterations);
-        a.out = o.d_ftle;
-        if (P)
-          hipLaunchKernelGGL(k_ftle_aux_reduce, dim3((unsigned)((P + 255) / 256)), dim3(256), 0,
-                             s, a);
-        DPAH_HIP(hipGetLastError());
+        SeedLattice lat;
+        const int kind = aux_lattice(lat, r.spec.origin, r.spec.lattice_spacing,
+                                     r.spec.lattice_shape, r.spec.delta, nullptr);
+        aux_reduce(r.pos.as<float>(), kind, lat, r.spec.delta, r.spec.params.step_size,
+                   r.spec.iterations, o.d_ftle, s);
         continue;
       }
       o.use64 = r.spec.params.use_64_bit_indices ? 1 : 0;
@@ -718,24 +707,17 @@ void dpah_server::launch_ftle(const std::shared_ptr<Request>& rp, int sid) {
   hipStream_t s = streams[sid];
   const uint64_t P = ftle_points(r.spec);
   try {
-    r.pos = DeviceBuffer(ctx, P * 18 * sizeof(float));   // the six particles' final positions
+    AdvectLaunch A{};
+    A.f = &r.spec.field->view;
+    A.seed_kind = aux_lattice(A.lat, r.spec.origin, r.spec.lattice_spacing, r.spec.lattice_shape,
+                              r.spec.delta, &A.n);
+    r.pos = DeviceBuffer(ctx, A.n * 3 * sizeof(float));   // the particles' final positions
     auto res = std::make_unique<dpah_result>();
     res->ctx = ctx;
     res->ftle_count = P;
     res->d_ftle = static_cast<float*>(ctx->alloc(P * sizeof(float)));
-    AdvectLaunch A{};
-    A.f = &r.spec.field->view;
-    A.seed_kind = kSeedAux;
-    SeedLattice& lat = A.lat;
-    lat.lo0 = r.spec.origin[0]; lat.lo1 = r.spec.origin[1]; lat.lo2 = r.spec.origin[2];
-    lat.st0 = r.spec.lattice_spacing[0]; lat.st1 = r.spec.lattice_spacing[1];
-    lat.st2 = r.spec.lattice_spacing[2];
-    lat.d0 = r.spec.lattice_shape[0]; lat.d1 = r.spec.lattice_shape[1];
-    lat.d2 = r.spec.lattice_shape[2];
-    lat.dl0 = r.spec.delta[0]; lat.dl1 = r.spec.delta[1]; lat.dl2 = r.spec.delta[2];
     A.uniform_rem = r.spec.iterations;
     A.pos_out = r.pos.as<float>();
-    A.n = 6 * P;
     A.dt = r.spec.params.step_size;
     A.t0 = r.spec.params.start_time;
     A.integrator = r.spec.params.integrator;
@@ -743,15 +725,8 @@ void dpah_server::launch_ftle(const std::shared_ptr<Request>& rp, int sid) {
     A.counters = L.counters;
     A.stream = s;
     launch_advect(ctx, A);
-    FtleAuxArgs a{};
-    a.finals = r.pos.as<float>();
-    a.points = P;
-    a.twod0 = 2.0f * r.spec.delta[0]; a.twod1 = 2.0f * r.spec.delta[1];
-    a.twod2 = 2.0f * r.spec.delta[2];
-    a.T_abs = std::fabs(r.spec.params.step_size * (float)r.spec.iterations);
-    a.out = res->d_ftle;
-    hipLaunchKernelGGL(k_ftle_aux_reduce, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, a);
-    DPAH_HIP(hipGetLastError());
+    aux_reduce(r.pos.as<float>(), A.seed_kind, A.lat, r.spec.delta, r.spec.params.step_size,
+               r.spec.iterations, res->d_ftle, s);
     DPAH_HIP(hipEventRecord(L.done, s));
     r.result = res.release();
   } catch (const Error& e) {
@@ -994,7 +969,9 @@ DPAH_EXPORT int dpah_server_submit(dpah_server* s, const dpah_request* req, uint
     r->spec = *req;
     if (req->kind == DPAH_REQUEST_FTLE_AUX) {
       DPAH_REQUIRE(req->iterations < (1ull << 32) - 2, "iterations must be below 2^32 - 2");
-      r->count = 6 * ftle_points(*req);
+      SeedLattice lat;
+      aux_lattice(lat, req->origin, req->lattice_spacing, req->lattice_shape, req->delta,
+                  &r->count);
       r->spec.params.record = 0;
       r->spec.seeds = nullptr;
       r->spec.remaining = nullptr;

@@ -207,8 +207,16 @@ DPAH_EXPORT int dpah_autotune_lattice(dpah_context* ctx, const dpah_field* field
     DPAH_REQUIRE(iterations <= (1ull << 32) - 3, "iterations must be below 2^32 - 2");
     const uint64_t points = shape[0] * shape[1] * shape[2];
     DPAH_REQUIRE(points > 0, "empty lattice");
-    const int kind = delta ? kSeedAux : kSeedLattice;
-    const uint64_t n = delta ? 6 * points : points;
+    SeedLattice lat{};
+    uint64_t n = points;
+    int kind = kSeedLattice;
+    if (delta) {
+      kind = aux_lattice(lat, origin, spacing, shape, delta, &n);
+    } else {
+      lat.lo0 = origin[0]; lat.lo1 = origin[1]; lat.lo2 = origin[2];
+      lat.st0 = spacing[0]; lat.st1 = spacing[1]; lat.st2 = spacing[2];
+      lat.d0 = shape[0]; lat.d1 = shape[1]; lat.d2 = shape[2];
+    }
     DPAH_REQUIRE(n < (1ull << 32), "lattice too large to tune in one launch");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     DPAH_HIP(hipSetDevice(ctx->device));
@@ -217,10 +225,7 @@ DPAH_EXPORT int dpah_autotune_lattice(dpah_context* ctx, const dpah_field* field
     AdvectLaunch L{};
     L.f = &field->view;
     L.seed_kind = kind;
-    L.lat.lo0 = origin[0]; L.lat.lo1 = origin[1]; L.lat.lo2 = origin[2];
-    L.lat.st0 = spacing[0]; L.lat.st1 = spacing[1]; L.lat.st2 = spacing[2];
-    L.lat.d0 = shape[0]; L.lat.d1 = shape[1]; L.lat.d2 = shape[2];
-    if (delta) { L.lat.dl0 = delta[0]; L.lat.dl1 = delta[1]; L.lat.dl2 = delta[2]; }
+    L.lat = lat;
     L.uniform_rem = iterations;
     L.pos_out = pos.as<float>();
     L.n = n;

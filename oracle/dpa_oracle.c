@@ -31,6 +31,12 @@
  *                            against its closed-form sum / integral.  Each
  *                            planted slip (DPO_PLANT) fails its pins.
  *   lattice seeds            count and C order closed form.
+ *   aux seeds                the moved coordinate against the exactly rounded
+ *                            rational origin + round(spacing * i +- delta) for
+ *                            random spacings, offsets and indices; with delta =
+ *                            spacing / 2, point k's +delta particle equals point
+ *                            k + 1's -delta particle bit for bit (and differs
+ *                            from the two-rounding form at some k).
  *   ftle                     diagonal linear field closed form, general linear
  *                            field vs numpy float64 SVD, eigen special cases vs
  *                            numpy eigvalsh.
@@ -572,6 +578,27 @@ uint64_t dpo_lattice_seeds(const float offset[3], const float size[3],
       }
     }
   return count;
+}
+
+void dpo_aux_seeds(const float origin[3], const float spacing[3], const uint64_t shape[3],
+                   const float delta[3], float* out) {
+  const uint64_t count = shape[0] * shape[1] * shape[2];
+  for (uint64_t index = 0; index < count; ++index) {
+    uint64_t sub[3], rest = index;
+    for (int i = 2; i >= 0; --i) { sub[i] = rest % shape[i]; rest /= shape[i]; }
+    float base[3];
+    for (int i = 0; i < 3; ++i) {
+      const float m = spacing[i] * (float)sub[i];
+      base[i] = origin[i] + m;
+    }
+    for (int a = 0; a < 6; ++a) {
+      float* p = out + 3 * (6 * index + (uint64_t)a);
+      const int ax = a / 2;
+      for (int i = 0; i < 3; ++i) p[i] = base[i];
+      const float m = fmaf(spacing[ax], (float)sub[ax], (a & 1) ? delta[ax] : -delta[ax]);
+      p[ax] = origin[ax] + m;
+    }
+  }
 }
 
 /* -------------------------------------------------------------- ftle ---- */

@@ -110,6 +110,17 @@ void     dpo_colors (const float* vertices, uint64_t count, float* out); /* colo
 uint64_t dpo_lattice_seeds(const float offset[3], const float size[3],
                            const float stride[3], uint64_t per_dim[3], float* out);
 
+/* Seeds of the auxiliary-neighbour FTLE (the FTLE benchmark's +-delta mode,
+ * no dpa counterpart): per lattice point i = (i0, i1, i2) of a d0 x d1 x d2
+ * lattice in C order, six particles -delta_0, +delta_0, -delta_1, +delta_1,
+ * -delta_2, +delta_2; each is the lattice point origin + fl(spacing * i)
+ * (uniform_seed_generator.cpp:35-36) with the one moved coordinate
+ * origin + fl(spacing * i +- delta): the offset position rounded once (fmaf),
+ * so that with delta = spacing / 2 the +delta particle of point k and the
+ * -delta particle of point k + 1 are the same float.  out: points x 6 x 3. */
+void dpo_aux_seeds(const float origin[3], const float spacing[3], const uint64_t shape[3],
+                   const float delta[3], float* out);
+
 /* ftle_estimator::estimate (ftle_estimator.cpp:7-65) with
  * regular_grid::gradient (regular_grid.hpp:87-108), restated in float64 from
  * the float32 flow map: scatter final positions to the cell of their original

@@ -87,6 +87,7 @@ def bind(path) -> C.CDLL:
         "dpo_ftle_dpa": (None, [_vp, _vp, _vp, _vp, _u64, C.c_float, _vp, _vp, _u64, _vp, _vp,
                                 _vp]),
         "dpo_ftle_aux": (None, [_vp, _u64, _vp, C.c_double, _vp]),
+        "dpo_aux_seeds": (None, [_vp, _vp, _vp, _vp, _vp]),
         "dpo_sym3_max_eigenvalue": (C.c_double, [_vp]),
     }
     for name, (res, args) in sig.items():
@@ -281,20 +282,42 @@ def ftle_aux(finals, delta, T) -> np.ndarray:
     return out
 
 
+def aux_shares(spacing, delta, shape) -> bool:
+    """Whether an auxiliary lattice's offsets are half its spacing on every
+    axis (exactly, in float32; every axis below 2^24 points), so that each
+    particle between two neighbouring points is one particle."""
+    sp, dl = _f32(spacing), _f32(delta)
+    return all(dl[i] > 0 and np.float32(2.0) * dl[i] == sp[i] and int(shape[i]) < (1 << 24)
+               for i in range(3))
+
+
+def aux_particle_steps(remaining, shape, iterations, shared) -> int:
+    """Particle-steps of an auxiliary lattice's advection from the remaining
+    counts of its six-per-point seeds (aux_seeds order): all six per point, or,
+    when shared, each particle between two points once -- the +delta particles
+    of every point and the -delta particles of the points on the low faces."""
+    shape = tuple(int(x) for x in shape)
+    steps = (int(iterations) - np.asarray(remaining, np.int64)).reshape(-1, 6)
+    if not shared:
+        return int(steps.sum())
+    idx = np.indices(shape).reshape(3, -1)
+    total = int(steps[:, 1::2].sum())
+    for d in range(3):
+        total += int(steps[idx[d] == 0, 2 * d].sum())
+    return total
+
+
 def sym3_max_eigenvalue(m) -> float:
     a = np.ascontiguousarray(m, np.float64).reshape(9)
     return float(load().dpo_sym3_max_eigenvalue(_p(a)))
 
 
 def aux_seeds(origin, spacing, shape, delta) -> np.ndarray:
-    """Seeds of the auxiliary-neighbour FTLE: for each lattice point
-    p = origin + spacing * i (float32, C order) the six points p -/+ delta_d
-    along axes 0, 1, 2, in that order; shape [points * 6, 3]."""
-    o, h, dl = _f32(origin), _f32(spacing), _f32(delta)
-    idx = np.indices(tuple(int(s) for s in shape)).reshape(3, -1).T
-    base = (o[None, :] + (h[None, :] * idx.astype(np.float32)).astype(np.float32)).astype(np.float32)
-    out = np.repeat(base[:, None, :], 6, axis=1)
-    for d in range(3):
-        out[:, 2 * d, d] = (base[:, d] - dl[d]).astype(np.float32)
-        out[:, 2 * d + 1, d] = (base[:, d] + dl[d]).astype(np.float32)
-    return out.reshape(-1, 3)
+    """Seeds of the auxiliary-neighbour FTLE (dpo_aux_seeds): for each lattice
+    point i (C order) the six points with coordinate d moved to
+    origin_d + fl(spacing_d * i_d -/+ delta_d), axes 0, 1, 2 in that order;
+    shape [points * 6, 3]."""
+    shp = np.ascontiguousarray([int(x) for x in shape], np.uint64)
+    out = np.empty((int(np.prod(shp)) * 6, 3), np.float32)
+    load().dpo_aux_seeds(_p(_f32(origin)), _p(_f32(spacing)), _p(shp), _p(_f32(delta)), _p(out))
+    return out

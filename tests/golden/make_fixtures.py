@@ -110,9 +110,14 @@ def ftle_aux_abc():
     res = po.advect_endpoints(po.Field(field, off, h), seeds, iters, dt)
     T = float(f32(dt * f32(iters)))
     ftle = po.ftle_aux(res["position"], delta, T)
+    # delta is half the spacing: the library advects each particle shared by
+    # two neighbouring points once and reports those particle-steps
+    assert po.aux_shares(h, delta, shape)
+    shared = po.aux_particle_steps(res["remaining"], shape, iters, True)
     return dict(field=field, offset=off, spacing=h, delta=delta, step_size=np.array(dt, f32),
                 iterations=np.array(iters, np.uint64), lattice_shape=np.array(shape, np.uint64),
-                ftle=ftle.astype(f32), total_steps=np.array(res["total_steps"], np.uint64))
+                ftle=ftle.astype(f32), total_steps=np.array(res["total_steps"], np.uint64),
+                shared_steps=np.array(shared, np.uint64))
 
 
 CASES = {"abc16_rounds": abc16_rounds, "vortex24_endpoints": vortex24_endpoints,

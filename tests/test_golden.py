@@ -75,3 +75,7 @@ def test_ftle_aux_abc():
     T = float(f32(g["step_size"] * f32(g["iterations"])))
     assert res["total_steps"] == int(g["total_steps"])
     assert same(po.ftle_aux(res["position"], g["delta"], T).astype(f32), g["ftle"])
+    shape = tuple(int(x) for x in g["lattice_shape"])
+    assert po.aux_shares(g["spacing"], g["delta"], shape)
+    assert po.aux_particle_steps(res["remaining"], shape, int(g["iterations"]),
+                                 True) == int(g["shared_steps"])

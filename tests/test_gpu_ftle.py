@@ -24,6 +24,10 @@ def say(msg):
     sys.stdout.flush()
 
 
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
 def assert_ftle_close(got, want, T):
     assert got.shape == want.shape
     fin = np.isfinite(want)
@@ -59,13 +63,17 @@ def test_ftle_dpa_lattice_mode(ctx, stride):
     assert_ftle_close(got, want, float(f32(dt * f32(iters))))
 
 
-def test_ftle_aux_mode(ctx):
-    say("case ftle auxiliary neighbours, ABC 48x48x24, 200 steps")
+@pytest.mark.parametrize("frac", [0.5, 0.3])
+def test_ftle_aux_mode(ctx, frac):
+    # delta = h / 2: each particle between two neighbouring points is advected
+    # once (three per point plus the low faces); 0.3 h: six per point
+    say(f"case ftle auxiliary neighbours, ABC 48x48x24, 200 steps, delta {frac} h")
     field, h = W.abc_field((48, 48, 24), spacing=(2 * math.pi / 47,) * 3)
     off = np.zeros(3, f32)
     iters, dt = 200, f32(0.02)
     shape = field.shape[:3]
-    delta = (f32(0.5) * h).astype(f32)
+    delta = (f32(frac) * h).astype(f32)
+    assert po.aux_shares(h, delta, shape) == (frac == 0.5)
     fld = ctx.field(field, off, h)
     n = int(np.prod(shape))
     d_out = ctx.alloc(4 * n)
@@ -79,8 +87,43 @@ def test_ftle_aux_mode(ctx):
     res = po.advect_endpoints(po.Field(field, off, h), seeds, iters, dt)
     T = float(f32(dt * f32(iters)))
     want = po.ftle_aux(res["position"], delta, T)
-    assert steps == res["total_steps"]
+    assert steps == po.aux_particle_steps(res["remaining"], shape, iters,
+                                          po.aux_shares(h, delta, shape))
     assert_ftle_close(got, want, T)
+
+
+@pytest.mark.parametrize("frac", [0.5, 0.3])
+def test_ftle_aux_chunks(frac):
+    # a context with a small memory budget processes the lattice in chunks
+    # (whole axis-0 slabs when particles are shared, each chunk advecting its
+    # own low face); the volume equals the one-launch volume bit for bit
+    say(f"case ftle auxiliary chunks, delta {frac} h")
+    field, h = W.abc_field((23, 17, 13), spacing=(2 * math.pi / 22,) * 3)
+    off = np.zeros(3, f32)
+    iters, dt = 80, f32(0.02)
+    shape = field.shape[:3]
+    n = int(np.prod(shape))
+    delta = (f32(frac) * h).astype(f32)
+    out = []
+    for budget in (0, 1 << 18):
+        with dpa_amd.Context(0, memory_budget=budget) as c:
+            fld = c.field(field, off, h)
+            d = c.alloc(4 * n)
+            try:
+                steps = c.ftle_aux_device(fld, off, h, shape, delta, iters, dt, d)
+                out.append((c.to_host(d, np.empty(n, f32)), steps))
+            finally:
+                c.free(d)
+                fld.release()
+    (one, s1), (chunked, s2) = out
+    assert np.array_equal(bits(one), bits(chunked))
+    if frac == 0.5:
+        assert s2 > s1   # each chunk after the first repeats a slab of particles
+    else:
+        assert s2 == s1
+    seeds = po.aux_seeds(off, h, shape, delta)
+    res = po.advect_endpoints(po.Field(field, off, h), seeds, iters, dt)
+    assert s1 == po.aux_particle_steps(res["remaining"], shape, iters, frac == 0.5)
 
 
 def test_ftle_estimate_from_given_positions(ctx):

@@ -4,7 +4,9 @@
                  in dpa's lattice mode (stride 1) and in the +-0.5-cell
                  auxiliary mode.  The whole flow map is also advected as a
                  seed array: its per-particle steps must add up to the lattice
-                 launch's total, a strided sample of its final positions must
+                 launch's total (in the auxiliary mode each particle shared by
+                 two neighbouring points counted once), a strided sample of its
+                 final positions must
                  equal the oracle's bit for bit, and FTLE at a strided sample
                  of points must be within 1e-3 |ref| + 1e-5 / T of the float64
                  reduction of the oracle on the same flow map.
@@ -121,20 +123,18 @@ def test_ftle_aux_mode_abc512(ctx, abc512):
         vol = ctx.to_host(d_out, np.empty(P, f32))
     finally:
         ctx.free(d_out)
-    # the six particles of every point as a seed array: their steps add up
+    # the six particles of every point as a seed array: the steps of the
+    # particles the launch advected (each shared by two points once) add up
     seeds = po.aux_seeds(c["lattice_origin"], c["lattice_spacing"], shape, delta)
     _, rem, _, total = endpoints(ctx, fld, seeds, iters, dt)
-    assert total == steps == int(np.sum(iters - rem.astype(np.int64)))
-    del seeds, rem
+    assert po.aux_shares(c["lattice_spacing"], delta, shape)
+    assert steps == po.aux_particle_steps(rem, shape, iters, True)
+    assert total == int(np.sum(iters - rem.astype(np.int64)))
+    del rem
     # a strided sample of points against the oracle
     pts = np.arange(0, P, 499)
-    i3 = np.stack(np.unravel_index(pts, tuple(shape)), 1).astype(f32)
-    o, sp = np.asarray(c["lattice_origin"], f32), np.asarray(c["lattice_spacing"], f32)
-    base = (o + (sp * i3).astype(f32)).astype(f32)
-    s6 = np.repeat(base[:, None, :], 6, axis=1)
-    for d in range(3):
-        s6[:, 2 * d, d] = (base[:, d] - delta[d]).astype(f32)
-        s6[:, 2 * d + 1, d] = (base[:, d] + delta[d]).astype(f32)
+    s6 = seeds.reshape(P, 6, 3)[pts].copy()
+    del seeds
     ora = po.advect_endpoints(po.Field(c["field"], off, h), s6.reshape(-1, 3), iters, dt)
     want = po.ftle_aux(ora["position"], delta, T)
     assert_ftle_close(vol[pts], want, T)

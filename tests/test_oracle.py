@@ -345,3 +345,80 @@ def test_dpa_structured_cpu_variant_equals_oracle():
     assert a["total_steps"] == b["total_steps"] > 10000
     assert np.array_equal(a["position"].view(np.uint32), b["position"].view(np.uint32))
     assert np.array_equal(a["remaining"], b["remaining"]) and np.array_equal(a["status"], b["status"])
+
+
+def _round_f32(fr):
+    """A rational rounded to float32, to nearest with ties to even."""
+    from fractions import Fraction
+    c = np.float32(float(fr))
+    best = None
+    for cand in (np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))):
+        err = abs(Fraction(float(cand)) - fr)
+        even = (int(np.float32(cand).view(np.uint32)) & 1) == 0
+        if best is None or err < best[0] or (err == best[0] and even):
+            best = (err, cand)
+    return np.float32(best[1])
+
+
+def test_aux_seeds_offset_rounded_once():
+    # the moved coordinate is origin + round(spacing * i +- delta), the offset
+    # position rounded once, computed here in exact rational arithmetic; the
+    # other coordinates are the lattice point's origin + round(spacing * i)
+    from fractions import Fraction
+    rng = np.random.default_rng(7)
+    shape = (5, 4, 3)
+    idx = np.indices(shape).reshape(3, -1).T
+    for _ in range(30):
+        sp = rng.uniform(0.01, 2.0, 3).astype(f32)
+        dl = (rng.uniform(0.1, 0.9, 3) * sp).astype(f32)
+        o = rng.uniform(-5.0, 5.0, 3).astype(f32)
+        s = po.aux_seeds(o, sp, shape, dl).reshape(-1, 6, 3)
+        for p in rng.choice(len(idx), 8, replace=False):
+            for a in range(6):
+                ax, sign = a // 2, (1 if a & 1 else -1)
+                m = _round_f32(Fraction(float(sp[ax])) * int(idx[p][ax]) +
+                               sign * Fraction(float(dl[ax])))
+                assert s[p, a, ax].view(np.uint32) == f32(o[ax] + m).view(np.uint32)
+                for e in range(3):
+                    if e != ax:
+                        want = f32(o[e] + f32(sp[e] * f32(idx[p][e])))
+                        assert s[p, a, e].view(np.uint32) == want.view(np.uint32)
+
+
+def test_aux_seeds_half_spacing_particles_coincide():
+    # with delta = spacing / 2, point k's +delta particle is point k + 1's
+    # -delta particle, bit for bit, on every axis: the library advects it once.
+    # Two roundings (lattice point, then +- delta) would not give that.
+    sp = np.array([0.1, 2 * np.pi / 511, 1 / 3], f32)
+    dl = (f32(0.5) * sp).astype(f32)
+    o = np.array([0.0, -1.25, 3.0], f32)
+    shape = (40, 33, 29)
+    assert po.aux_shares(sp, dl, shape)
+    assert not po.aux_shares(sp, (dl * f32(0.9)).astype(f32), shape)
+    s = po.aux_seeds(o, sp, shape, dl).reshape(shape + (6, 3))
+    twice = 0
+    for d in range(3):
+        plus = np.take(s[..., 2 * d + 1, :], np.arange(shape[d] - 1), axis=d)
+        minus = np.take(s[..., 2 * d, :], np.arange(1, shape[d]), axis=d)
+        assert np.array_equal(plus.view(np.uint32), minus.view(np.uint32))
+        k = np.arange(shape[d], dtype=np.float32)
+        base = (o[d] + (sp[d] * k).astype(f32)).astype(f32)
+        twice += int(np.sum((base[:-1] + dl[d]).astype(f32) != (base[1:] - dl[d]).astype(f32)))
+    assert twice > 0
+
+
+def test_aux_particle_steps_counts_shared_particles_once():
+    # a 2 x 2 x 1 lattice: 24 particles, of which the +d particles of the
+    # lower points equal the -d particles of the upper ones (4 + 4 + 0 pairs
+    # shared along axes 0 and 1; axis 2 has one point)
+    shape = (2, 2, 1)
+    rem = np.arange(24, dtype=np.uint64)           # steps = 100 - rem
+    steps = 100 - rem.astype(np.int64)
+    assert po.aux_particle_steps(rem, shape, 100, False) == int(steps.sum())
+    st = steps.reshape(4, 6)
+    idx = np.indices(shape).reshape(3, -1).T
+    want = int(st[:, 1::2].sum())
+    want += sum(int(st[p, 2 * d]) for p in range(4) for d in range(3) if idx[p][d] == 0)
+    assert po.aux_particle_steps(rem, shape, 100, True) == want
+    assert want == int(steps.sum()) - sum(int(st[p, 2 * d]) for p in range(4) for d in range(3)
+                                          if idx[p][d] > 0)
