@@ -165,7 +165,8 @@ bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch
 // point.
 uint32_t lattice_brick(const Tuning& T, int seed_kind) {
   if (T.lattice_brick) return (uint32_t)T.lattice_brick;
-  return seed_kind == kSeedLattice ? kLatticeBrick : kAuxBrick;
+  return seed_kind == kSeedLattice ? kLatticeBrick
+         : seed_kind == kSeedAuxShared ? kAuxSharedBrick : kAuxBrick;
 }
 
 // Bricks per group edge of the grouped walk; 1: no groups.

@@ -101,7 +101,9 @@ struct Tuning {
   int order_brick = 0;
   // whole-lattice launches walk their points in bricks of this many points per
   // axis (2, 4 or 8); 0: by launch kind (kLatticeBrick for lattices of points,
-  // kAuxBrick for auxiliary lattices: FTLE ±½-cell 59.4 ms at 4, 58.4 at 2)
+  // kAuxBrick for auxiliary lattices of six particles per point: FTLE ±½-cell
+  // 59.4 ms at 4, 58.4 at 2; kAuxSharedBrick for shared particles: 28.6 ms at
+  // 4, 29.6 at 2, 29.8 at 8)
   int lattice_brick = 0;
   // and in groups of this many bricks per axis above them (2, 4, 8 or 16), so
   // that what a queue part has in flight is a compact block; 1: no groups;
@@ -279,7 +281,8 @@ struct OrderScratch {
 // and few enough for 32-bit positions.
 constexpr uint64_t kOrderMinParticles = 1ull << 15;
 constexpr uint32_t kLatticeBrick = 4;   // edge of a brick-ordered lattice queue, in points
-constexpr uint32_t kAuxBrick = 2;       // the same for auxiliary (six-particle) lattices
+constexpr uint32_t kAuxBrick = 2;       // the same for auxiliary lattices of six particles
+constexpr uint32_t kAuxSharedBrick = 4; // and of shared particles (three per point)
 constexpr uint32_t kLatticeGroup = 2;   // bricks per group edge, lattices of points
 
 // Launchers (advect.hip / compact.hip).

@@ -211,10 +211,18 @@ int dpah_ftle_lattice(dpah_context* ctx, const dpah_field* field, const float st
 /* FTLE with auxiliary neighbours (no counterpart in dpa; replaces the scatter
  * and one-sided borders of ftle_estimator.cpp:39-45 by central differences
  * that hold at the border): for every lattice point p = origin + spacing * i
- * (C order over lattice_shape) six particles start at p -/+ delta_d along each
- * axis d; J[:, d] = (phi(p + delta_d) - phi(p - delta_d)) / (2 delta_d);
+ * (C order over lattice_shape, each coordinate origin + fl(spacing * i))
+ * particles start at p -/+ delta_d along each axis d, that coordinate being
+ * origin_d + fl(spacing_d * i_d -/+ delta_d) (the offset rounded once);
+ * J[:, d] = (phi(p + delta_d) - phi(p - delta_d)) / (2 delta_d);
  * ftle_out (device, one float per point) = ln(sqrt(lmax(J^T J))) /
- * |step_size * iterations|. */
+ * |step_size * iterations|.  When delta is half the spacing on every axis,
+ * point i's +delta particle and point i + e_d's -delta particle are the same
+ * float and are advected once (three particles per point plus the low faces'
+ * -delta particles); *total_steps counts the particle-steps advected, a
+ * shared particle once (a lattice processed in chunks -- above 2^25 points,
+ * or finals above a quarter of the memory budget -- repeats one slab of
+ * particles per chunk boundary and counts it again). */
 int dpah_ftle_aux_device(dpah_context* ctx, const dpah_field* field, const float origin[3],
                          const float lattice_spacing[3], const uint64_t lattice_shape[3],
                          const float delta[3], uint64_t iterations, float step_size,
