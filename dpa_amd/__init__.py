@@ -698,6 +698,8 @@ def aux_particles(spacing, delta, shape) -> int:
     sp = np.asarray(spacing, np.float32).reshape(3)
     dl = np.asarray(delta, np.float32).reshape(3)
     d0, d1, d2 = (int(x) for x in shape)
+    if d0 * d1 * d2 == 0:
+        return 0
     if all(dl[i] > 0 and np.float32(2.0) * dl[i] == sp[i] and int(shape[i]) < (1 << 24)
            for i in range(3)):
         return 3 * d0 * d1 * d2 + d1 * d2 + d0 * d2 + d0 * d1

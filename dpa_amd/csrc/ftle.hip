@@ -175,7 +175,9 @@ int dpah::aux_lattice(SeedLattice& lat, const float origin[3], const float latti
     shared = shared && delta[i] > 0.0f && 2.0f * delta[i] == lattice_spacing[i] &&
              shape[i] < (1ull << 24);
   const uint64_t points = shape[0] * shape[1] * shape[2];
-  if (particles) *particles = shared ? aux_shared_count(shape[0], shape[1], shape[2]) : 6 * points;
+  if (particles)   // (an empty lattice has no face particles either)
+    *particles = points == 0 ? 0 : shared ? aux_shared_count(shape[0], shape[1], shape[2])
+                                          : 6 * points;
   return shared ? kSeedAuxShared : kSeedAux;
 }
 
@@ -207,6 +209,10 @@ void dpah::ftle_aux_into(dpah_context* ctx, const dpah_field* field, const float
   DPAH_REQUIRE(iterations < (1ull << 32) - 2, "iterations must be below 2^32 - 2");
   const uint64_t points = lattice_shape[0] * lattice_shape[1] * lattice_shape[2];
   DPAH_REQUIRE(points == 0 || ftle_out, "ftle_out is null");
+  if (points == 0) {   // an empty lattice (some axis of no points): nothing to advect
+    if (total_steps) *total_steps = 0;
+    return;
+  }
   SeedLattice lat;
   const int kind = aux_lattice(lat, origin, lattice_spacing, lattice_shape, delta, nullptr);
   // points are processed in chunks of at most 2^25 whose final positions take

@@ -93,6 +93,25 @@ def test_ftle_aux_mode(ctx, frac):
 
 
 @pytest.mark.parametrize("frac", [0.5, 0.3])
+def test_ftle_aux_empty_lattice(ctx, frac):
+    # a lattice with no points along one axis advects nothing
+    say(f"case ftle auxiliary, empty lattice, delta {frac} h")
+    field, h = W.abc_field((12, 10, 8), spacing=(2 * math.pi / 11,) * 3)
+    off = np.zeros(3, f32)
+    fld = ctx.field(field, off, h)
+    d = ctx.alloc(64)
+    try:
+        for shape in ((0, 10, 8), (12, 0, 8), (12, 10, 0)):
+            assert dpa_amd.aux_particles(h, (f32(frac) * h).astype(f32), shape) == 0
+            steps = ctx.ftle_aux_device(fld, off, h, shape, (f32(frac) * h).astype(f32), 50,
+                                        f32(0.02), d)
+            assert steps == 0
+    finally:
+        ctx.free(d)
+        fld.release()
+
+
+@pytest.mark.parametrize("frac", [0.5, 0.3])
 def test_ftle_aux_chunks(frac):
     # a context with a small memory budget processes the lattice in chunks
     # (whole axis-0 slabs when particles are shared, each chunk advecting its

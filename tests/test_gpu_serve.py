@@ -67,8 +67,11 @@ def make_requests(fields):
             reqs.append(dict(kind="advect", field=fs, seeds=seeds, uniform_remaining=50 + 90 * k,
                              step_size=f32(0.25), start_time=f32(0.0), record=(k % 2 == 0),
                              particles_per_round=None))
+    # delta half the spacing (shared particles) and not (six per point)
     reqs.append(dict(kind="ftle", field=fs, origin=(2, 2, 2), spacing=(1, 1, 1), shape=(8, 10, 6),
                      delta=(0.5, 0.5, 0.5), iterations=150, step_size=f32(0.25)))
+    reqs.append(dict(kind="ftle", field=fs, origin=(3, 1, 2), spacing=(1, 1, 1), shape=(7, 9, 5),
+                     delta=(0.3, 0.3, 0.3), iterations=120, step_size=f32(0.25)))
     return reqs
 
 
