@@ -19,8 +19,12 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
     const int block = T.block;
     // recorded launches: 6 (each workgroup holds a 24 KB slot stage, and 6 fill
     // the 160 KB of LDS of a CU); endpoints: 4 for seed arrays, 3 for lattices
+    // of steady fields (their neighbours' node rows hit in the L2), 4 for
+    // lattices of time-variant fields, whose reloads read two slices (a
+    // 256x256x16-point FTLE slab on a 256^3 x 16-slice field: 3.32 ms at 4,
+    // 3.43 at 5, 3.55-3.70 at 3)
     const int bpc = T.blocks_per_cu ? T.blocks_per_cu
-                  : kRecord ? 6 : a.seed_kind == kSeedArray ? 4 : 3;
+                  : kRecord ? 6 : a.seed_kind == kSeedArray || kUnsteady ? 4 : 3;
     uint64_t grid = (uint64_t)ctx->cu_count * bpc;
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;

@@ -384,8 +384,9 @@ int dpah_cell_loads(dpah_context* ctx, uint64_t* record_loads, uint64_t* node_lo
 typedef struct dpah_tuning {
   int block;           /* threads per workgroup: 64, 128, 192 or 256          */
   int blocks_per_cu;   /* persistent grid = CUs x blocks_per_cu, 1..32; 0: by
-                          launch kind (4 endpoint seed arrays, 3 lattices,
-                          6 recorded launches)                                 */
+                          launch kind (4 endpoint seed arrays, 3 lattices of
+                          steady fields, 4 of time-variant fields, 6 recorded
+                          launches)                                            */
   int order;           /* 1: endpoint launches of seed arrays and lattices take
                           their queue in brick Morton order, split across the
                           XCDs (auxiliary FTLE lattices are already compact;
