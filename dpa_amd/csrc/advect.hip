@@ -259,6 +259,11 @@ uint32_t record_slot_shift(const dpah_context* ctx, const FieldView& f, int inte
   return slot_shift_for(rows, persistent, T.slot_stage);
 }
 
+bool record_grid(const dpah_context* ctx, const FieldView& f, int integrator, bool batch) {
+  const Tuning& T = tuning_for(ctx, launch_key(f, integrator, kSeedArray, true));
+  return recorded_grid(T.slot_stage, batch);
+}
+
 void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   const Tuning& T =
       tuning_for(ctx, launch_key(*L.f, L.integrator, L.seed_kind, L.slots != nullptr));

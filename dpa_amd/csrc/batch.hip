@@ -155,8 +155,12 @@ struct GroupRun {
       segs[s].stride = record ? r.stride : 0;
       q += tiled_rows(r.count);   // segments start on 64-ordinal boundaries (whole waves)
     }
-    // one slot layout for the launch
-    slot_shift = record ? record_slot_shift(ctx, j0.field->view, j0.params.integrator, q, true)
+    // one slot layout for the launch; a batch's recorded launch stays
+    // persistent unless the tuning asks for one thread per row
+    const bool persistent =
+        !record || !record_grid(ctx, j0.field->view, j0.params.integrator, true);
+    slot_shift = record ? record_slot_shift(ctx, j0.field->view, j0.params.integrator, q,
+                                            persistent)
                         : 0u;
     for (size_t s = 0; s < refs.size(); ++s) {
       segs[s].slot_base = slot_floats;
@@ -225,7 +229,7 @@ struct GroupRun {
     L.dt = j0.params.step_size;
     L.t0 = j0.params.start_time;
     L.integrator = j0.params.integrator;
-    L.persistent = true;
+    L.persistent = persistent;
   }
 
   void enqueue(dpah_context* ctx) const {
@@ -470,6 +474,12 @@ DPAH_EXPORT int dpah_autotune_batch(dpah_context* ctx, const dpah_job* jobs, uin
             t.slot_stage = j0.params.record ? stage : 0;
             cands.push_back(t);
           }
+      if (j0.params.record) {   // and one thread per row (no launch shape to vary)
+        Tuning t = ctx->tune;
+        t.block = 256;
+        t.slot_stage = 3;
+        cands.push_back(t);
+      }
       std::unique_ptr<GroupRun> g;
       auto prepare = [&](const Tuning&) {
         g.reset();

@@ -617,6 +617,10 @@ dpah_result* dpah::run_advect(dpah_context* ctx, const dpah_field* field, const 
       launch_advect(ctx, L);
       read_steps();
     } else {
+      // a single request's recorded rounds run one thread per row unless the
+      // tuning asks for the persistent kernel (record_grid)
+      if (persistent && record_grid(ctx, field->view, params->integrator, false))
+        persistent = L.persistent = false;
       const uint64_t ppr = params->particles_per_round;
       DeviceBuffer d_steps(ctx, n1 * sizeof(uint32_t));
       DeviceBuffer d_special(ctx, n1);
@@ -912,7 +916,7 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(t->lattice_brick == 0 || t->lattice_brick == 2 || t->lattice_brick == 4 ||
                      t->lattice_brick == 8,
                  "lattice_brick must be 0, 2, 4 or 8");
-    DPAH_REQUIRE(t->slot_stage >= 0 && t->slot_stage <= 2, "slot_stage must be 0, 1 or 2");
+    DPAH_REQUIRE(t->slot_stage >= 0 && t->slot_stage <= 3, "slot_stage must be 0, 1, 2 or 3");
     DPAH_REQUIRE(t->lattice_group == 0 || t->lattice_group == 1 || t->lattice_group == 2 ||
                      t->lattice_group == 4 || t->lattice_group == 8 || t->lattice_group == 16,
                  "lattice_group must be 0, 1, 2, 4, 8 or 16");

@@ -111,8 +111,9 @@ struct Tuning {
   // FTLE 11.47 ms without, 10.87 with groups of 2 -- none for auxiliary
   // lattices, which lose 2-5% with groups)
   int lattice_group = 0;
-  // record slots of persistent recorded launches: 0 by launch size (8-slot
-  // blocks staged in LDS from kStageMinRows rows), 1 one-slot runs, 2 staged
+  // recorded launches: 0 by launch kind (recorded_grid, slot_shift_for),
+  // 1 persistent with one-slot runs, 2 persistent with 8-slot blocks staged in
+  // LDS, 3 one thread per row (a grid over the rows) with one-slot runs
   int slot_stage = 0;
 };
 
@@ -385,13 +386,27 @@ dpah_result* run_ftle_aux(dpah_context* ctx, const dpah_field* field, const floa
 // shift 0, 0.356 ms with blocks of eight stored directly, 0.39 ms staged).
 constexpr uint64_t kStageMinRows = 1ull << 15;
 inline uint32_t slot_shift_for(uint64_t rows, bool persistent, int slot_stage = 0) {
-  if (!persistent || slot_stage == 1) return 0u;
+  if (!persistent || slot_stage == 1 || slot_stage == 3) return 0u;
   return slot_stage == 2 || rows >= kStageMinRows ? kStagedShift : 0u;
+}
+// Whether a recorded launch runs one thread per row instead of the persistent
+// kernel.  By default a single request's rounds do (dpah_advect: the largest
+// 64-job batch job alone, 243,573 rows x 3679 steps, advect 14.0 -> 10.7 ms
+// and compaction 4.3 -> 5.5 ms; 100k vortex rows x 1000: 2.17 -> 1.98 ms with
+// the compaction; ABC 64^3: 0.243 -> 0.226 ms; served pathlines on a 256^3 x
+// 16-slice field: 6.1 -> 5.2 ms), and a batch's launch over many jobs' rounds
+// stays persistent with staged blocks (the 64-job batch: advect 24.5 ms
+// against 29.1-30.1 one thread per row, compaction 18.6-19.1 against
+// 24.2-24.5, the one-slot layout leaving 12-byte holes of retired rows).
+inline bool recorded_grid(int slot_stage, bool batch) {
+  return slot_stage == 3 || (slot_stage == 0 && !batch);
 }
 // The same under the tuning of recorded launches on field f with this
 // integrator (an autotuned entry or the context's).
 uint32_t record_slot_shift(const dpah_context* ctx, const FieldView& f, int integrator,
                            uint64_t rows, bool persistent);
+// recorded_grid under the same tuning.
+bool record_grid(const dpah_context* ctx, const FieldView& f, int integrator, bool batch);
 
 // Prunes, indexes and colours `count` slot rows (slot shift g, slot_float);
 // allocates the chunk's outputs.  Synchronises the stream twice (the output

@@ -76,7 +76,7 @@ void load_autotune_cache(dpah_context* ctx) {
                     t.order_brick >= 0 && t.order_brick <= 5 &&
                     (t.lattice_brick == 0 || t.lattice_brick == 2 || t.lattice_brick == 4 ||
                      t.lattice_brick == 8) &&
-                    t.slot_stage >= 0 && t.slot_stage <= 2 &&
+                    t.slot_stage >= 0 && t.slot_stage <= 3 &&
                     (t.lattice_group == 0 || t.lattice_group == 1 || t.lattice_group == 2 ||
                      t.lattice_group == 4 || t.lattice_group == 8 || t.lattice_group == 16);
     if (ok) ctx->tuned[key] = t;

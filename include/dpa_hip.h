@@ -415,9 +415,12 @@ typedef struct dpah_tuning {
   int lattice_brick;   /* whole-lattice launches (FTLE) walk their points in
                           bricks of this many points per axis: 2, 4 or 8;
                           0: 4 for lattices of points, 2 for auxiliary ones   */
-  int slot_stage;      /* record slots of persistent recorded launches: 0 by
-                          launch size (8-slot blocks staged in LDS from 2^15
-                          rows), 1 one-slot runs, 2 staged blocks             */
+  int slot_stage;      /* recorded launches: 0 by launch kind (a single
+                          request's rounds one thread per row; a batch
+                          persistent, its slots in 8-slot blocks staged in LDS
+                          from 2^15 rows), 1 persistent with one-slot runs,
+                          2 persistent with staged blocks, 3 one thread per
+                          row with one-slot runs                              */
   int lattice_group;   /* whole-lattice launches walk groups of this many
                           bricks per axis one after another, bricks inside a
                           group: 2, 4, 8 or 16; 1: no groups; 0: 2 for

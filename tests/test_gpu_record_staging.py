@@ -1,11 +1,14 @@
-"""Recorded launches that stage slot blocks in LDS.
+"""The forms of a recorded launch.
 
-Launches of at least 2^15 recorded particles keep each lane's current block of
-eight slots in LDS and write whole 96-byte blocks: as one run per wave when
-the refill threshold is 64 (lanes in lockstep, lane l on row 64t + l), lane
-by lane otherwise.  Smaller launches store every slot directly.  Either way
-the rounds must equal the oracle's bit for bit, with particles retiring at
-every position within a block (uneven remaining counts, exits, NaN seeds).
+Persistent launches of at least 2^15 recorded particles (a batch's, by
+default; a single request's with slot_stage 2) keep each lane's current
+block of eight slots in LDS and write whole 96-byte blocks: as one run per
+wave when the refill threshold is 64 (lanes in lockstep, lane l on row
+64t + l), lane by lane otherwise.  A single request's rounds run one thread
+per row by default (slot_stage 0, or 3 explicitly) and store every slot
+directly, as smaller persistent launches do.  Every form must give the
+oracle's rounds bit for bit, with particles retiring at every position
+within a block (uneven remaining counts, exits, NaN seeds).
 """
 import sys
 
@@ -43,15 +46,16 @@ def uneven(n, seed):
     return seeds, rem
 
 
-@pytest.mark.parametrize("refill,bpc", [(0, 0), (16, 0), (1, 2)])
-def test_staged_round_matches_oracle(ctx, vortex, refill, bpc):
+@pytest.mark.parametrize("refill,bpc,stage", [(0, 0, 2), (16, 0, 2), (1, 2, 2), (0, 0, 0),
+                                              (0, 0, 3), (0, 0, 1)])
+def test_staged_round_matches_oracle(ctx, vortex, refill, bpc, stage):
     field, fld = vortex
-    say(f"case staged recorded round, refill={refill} blocks_per_cu={bpc}")
+    say(f"case recorded round, refill={refill} blocks_per_cu={bpc} slot_stage={stage}")
     seeds, rem = uneven(40000, 501)
     dt = f32(0.25)
     ora = po.run_rounds(po.Field(field, (0, 0, 0), (1, 1, 1)), seeds, rem, 40000, dt)
     before = ctx.tuning()
-    ctx.set_tuning(refill=refill, blocks_per_cu=bpc)
+    ctx.set_tuning(refill=refill, blocks_per_cu=bpc, slot_stage=stage)
     try:
         res = ctx.advect(fld, seeds, remaining=rem, particles_per_round=40000, step_size=dt)
         try:
@@ -63,8 +67,9 @@ def test_staged_round_matches_oracle(ctx, vortex, refill, bpc):
 
 
 def test_staged_batch_equals_direct_jobs(ctx, vortex):
-    """Three jobs of 15000 rows: alone each launch stores slots directly; in
-    one batch (45000 rows) the launch stages them.  Same bits either way."""
+    """Three jobs of 15000 rows: alone each runs one thread per row and stores
+    slots directly; in one batch (45000 rows) the persistent launch stages
+    them.  Same bits either way."""
     field, fld = vortex
     say("case staged batch vs direct jobs")
     jobs = []
@@ -116,11 +121,15 @@ def test_staged_batch_pass_replays_equal_eager(ctx, vortex):
             say(f"  replay {rep} matches")
 
 
-@pytest.mark.parametrize("integrator", [po.DPA_RK4, po.RK4])
-def test_staged_pathlines_match_oracle(ctx, integrator):
-    """Staged slot blocks on a time-variant field (pathlines; the sample
-    lerps time first), one round of 36000 rows, both integrators."""
-    say(f"case staged pathlines, integrator {integrator}")
+@pytest.mark.parametrize("integrator,stage", [(po.DPA_RK4, 2), (po.RK4, 2), (po.DPA_RK4, 0),
+                                              (po.RK4, 0)])
+def test_staged_pathlines_match_oracle(ctx, integrator, stage):
+    """Staged slot blocks (slot_stage 2) and one thread per row (0) on a
+    time-variant field (pathlines; the sample lerps time first), one round
+    of 36000 rows, both integrators."""
+    say(f"case pathlines, integrator {integrator}, slot_stage {stage}")
+    before = ctx.tuning()
+    ctx.set_tuning(slot_stage=stage)
     field = W.unsteady_vortex_field(32, time_steps=4, count=8)
     fld = ctx.field_unsteady(field, (0, 0, 0), (1, 1, 1), 0.0, 1.0)
     try:
@@ -138,18 +147,20 @@ def test_staged_pathlines_match_oracle(ctx, integrator):
             res.free()
     finally:
         fld.release()
+        ctx.set_tuning(**before)
 
 
-@pytest.mark.parametrize("fastdiv", [1, 0])
-def test_staged_round_other_division_modes(ctx, fastdiv):
-    """Staged slot blocks in the verified-reciprocal (fastdiv 1) and the
-    dividing (fastdiv 0) kernel instances: an anisotropic, offset grid whose
-    spacings are not all powers of two."""
-    say(f"case staged round, spacing (0.75, 0.1, 0.5), fastdiv={fastdiv}")
+@pytest.mark.parametrize("fastdiv,stage", [(1, 2), (0, 2), (1, 0), (0, 0)])
+def test_staged_round_other_division_modes(ctx, fastdiv, stage):
+    """Staged slot blocks (slot_stage 2) and one thread per row (0) in the
+    verified-reciprocal (fastdiv 1) and the dividing (fastdiv 0) kernel
+    instances: an anisotropic, offset grid whose spacings are not all powers
+    of two."""
+    say(f"case round, spacing (0.75, 0.1, 0.5), fastdiv={fastdiv}, slot_stage={stage}")
     offset, spacing = (0.25, -0.5, 1.0), (0.75, float(f32(0.1)), 0.5)
     data = W.vortex_field(36, count=10)
     before = ctx.tuning()
-    ctx.set_tuning(fastdiv=fastdiv)
+    ctx.set_tuning(fastdiv=fastdiv, slot_stage=stage)
     try:
         fld = ctx.field(data, offset, spacing)
         try:
