@@ -284,9 +284,10 @@ int main() {
   r.r0 = 1.0f / r.h0; r.r1 = 1.0f / r.h1; r.r2 = 1.0f / r.h2;
   diffs += probes<kDivFast>(r, samples);
   diffs += walk<false, kDivFast>(r, rng, samples, shifts);
-  // buffer reloads (24-bit offset arithmetic, byte offsets, rows in soffset)
-  // against the pointer path: the node array of steady and time-variant
-  // fields, and cell layouts in C order and in bricks of 2 and 8 cells
+  // 32-bit reloads (24-bit offset arithmetic into a buffer resource or from a
+  // uniform base, and 32-bit float / record offsets) against the 64-bit
+  // pointer path: the node array of steady and time-variant fields, and cell
+  // layouts in C order and in bricks of 2 and 8 cells
   {
     FieldView b = f;
     b.node_bytes32 = (uint32_t)(data.size() * sizeof(float));
@@ -303,7 +304,18 @@ int main() {
       c.cell_bytes32 = (uint32_t)(cells.size() * sizeof(float4));
       diffs += walk<false, kDivFast>(c, rng, samples, shifts);
       diffs += probes<kDivFast>(c, samples);
+      // the same layout through 32-bit record indices (fields past 2^32 bytes)
+      c.cell_bytes32 = 0;
+      c.idx32 = 1;
+      diffs += walk<false, kDivFast>(c, rng, samples, shifts);
     }
+    // nodes through 32-bit float offsets, steady and time-variant
+    FieldView i32 = f;
+    i32.idx32 = 1;
+    diffs += walk<false, kDivFast>(i32, rng, samples, shifts);
+    FieldView i32t = g;
+    i32t.idx32 = 1;
+    diffs += walk<true, kDivFast>(i32t, rng, samples, shifts);
   }
   std::printf("samples %ld shifts %ld roundups %ld diffs %ld\n", samples, shifts, roundups, diffs);
   if (roundups < 100) {
