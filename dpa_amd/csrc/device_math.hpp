@@ -3,8 +3,10 @@
 // Every operation is one IEEE binary32 rounding, in the order dpa's CPU build
 // performs it; the library is compiled with -ffp-contract=off and correctly
 // rounded division/sqrt, so no multiply-add is fused behind our back.  The
-// only fused operation is the explicit fmaf in exact_fmod(), whose result is
-// exact (see there).
+// explicit fmaf calls compute exact quantities -- fmod's remainder
+// (exact_fmod, fast_axis) and the residual of a reciprocal quotient before
+// its one correction (div_by_fast) -- so each result is still the correctly
+// rounded one the reference computes (see there).
 #pragma once
 
 #include <hip/hip_runtime.h>
