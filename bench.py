@@ -682,9 +682,14 @@ def run_ftle(args, dpa_mode=False):
                                               delta=c["delta"])
         log(f"autotuned in {time.perf_counter() - t:.1f} s: {tuned} ({tms:.3f} ms)")
 
+    # dpa's stage returns the volume to host memory: one array, reused by
+    # every pass, as a caller writing a volume per round would keep it
+    volume = np.empty(tuple(shape), np.float32) if dpa_mode else None
+
     def one():
         if dpa_mode:
-            v, s = ctx.ftle_lattice(fld, (1.0, 1.0, 1.0), c["iterations"], c["step_size"])
+            v, s = ctx.ftle_lattice(fld, (1.0, 1.0, 1.0), c["iterations"], c["step_size"],
+                                    out=volume)
             return s
         return ctx.ftle_aux_device(fld, c["lattice_origin"], c["lattice_spacing"], shape,
                                    c["delta"], c["iterations"], c["step_size"], d_out)

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "runtime.hpp"
@@ -68,6 +69,62 @@ void* dpah_context::host_staging(uint64_t bytes) {
     pinned_bytes = want;
   }
   return pinned;
+}
+
+void dpah_context::download(void* dst, const void* src, uint64_t bytes, hipStream_t s) {
+  if (!bytes) return;
+  hipPointerAttribute_t at{};
+  bool pinned_dst = false;
+  if (hipPointerGetAttributes(&at, dst) == hipSuccess) pinned_dst = at.type == hipMemoryTypeHost;
+  else (void)hipGetLastError();   // pageable memory is not known to the runtime
+  uint8_t* stage = nullptr;
+  if (!pinned_dst && bytes >= kStagedMin)
+    stage = static_cast<uint8_t*>(host_staging(2 * kStagedChunk));
+  if (!stage) {
+    DPAH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    DPAH_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  hipEvent_t ev[2] = {take_event(), take_event()};
+  const uint64_t n = (bytes + kStagedChunk - 1) / kStagedChunk;
+  auto issue = [&](uint64_t i) {
+    const uint64_t off = i * kStagedChunk, len = std::min(kStagedChunk, bytes - off);
+    DPAH_HIP(hipMemcpyAsync(stage + (i & 1) * kStagedChunk, static_cast<const uint8_t*>(src) + off,
+                            len, hipMemcpyDeviceToHost, s));
+    DPAH_HIP(hipEventRecord(ev[i & 1], s));
+  };
+  try {
+    issue(0);
+    for (uint64_t i = 0; i < n; ++i) {
+      // chunk i + 1 goes into the buffer chunk i - 1 left, already copied out
+      if (i + 1 < n) issue(i + 1);
+      DPAH_HIP(hipEventSynchronize(ev[i & 1]));
+      const uint64_t off = i * kStagedChunk, len = std::min(kStagedChunk, bytes - off);
+      uint8_t* to = static_cast<uint8_t*>(dst) + off;
+      const uint8_t* from = stage + (i & 1) * kStagedChunk;
+      const uint64_t part = ((len + kStagedThreads - 1) / kStagedThreads + 63) & ~uint64_t(63);
+      std::thread w[kStagedThreads - 1];
+      for (int k = 1; k < kStagedThreads; ++k) {
+        const uint64_t a = std::min(len, k * part), b = std::min(len, a + part);
+        if (a >= b) continue;
+        try {
+          w[k - 1] = std::thread([=] { std::memcpy(to + a, from + a, b - a); });
+        } catch (...) {   // no thread to be had: this part on the calling thread
+          std::memcpy(to + a, from + a, b - a);
+        }
+      }
+      std::memcpy(to, from, std::min(len, part));
+      for (auto& t : w)
+        if (t.joinable()) t.join();
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    event_pool.push_back(ev[0]);
+    event_pool.push_back(ev[1]);
+    throw;
+  }
+  event_pool.push_back(ev[0]);
+  event_pool.push_back(ev[1]);
 }
 
 void* dpah_context::alloc(uint64_t bytes) {
@@ -307,7 +364,7 @@ DPAH_EXPORT int dpah_copy_to_host(dpah_context* ctx, void* dst, const void* src,
     DPAH_REQUIRE(ctx && (bytes == 0 || (dst && src)), "null argument");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     DPAH_HIP(hipSetDevice(ctx->device));
-    if (bytes) DPAH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->download(dst, src, bytes, ctx->stream);
     ctx->sync();
   });
 }
@@ -745,14 +802,10 @@ DPAH_EXPORT int dpah_result_copy_round(const dpah_result* r, uint64_t round, flo
         continue;
       }
       if (vertices && c.V)
-        DPAH_HIP(hipMemcpyAsync(vertices + 3 * v0, c.vertices, 3 * c.V * sizeof(float),
-                                hipMemcpyDeviceToHost, ctx->stream));
-      if (colors && c.V)
-        DPAH_HIP(hipMemcpyAsync(colors + v0, c.colors, c.V * sizeof(float), hipMemcpyDeviceToHost,
-                                ctx->stream));
+        ctx->download(vertices + 3 * v0, c.vertices, 3 * c.V * sizeof(float), ctx->stream);
+      if (colors && c.V) ctx->download(colors + v0, c.colors, c.V * sizeof(float), ctx->stream);
       if (indices && c.I)
-        DPAH_HIP(hipMemcpyAsync(static_cast<char*>(indices) + i0 * ib, c.indices, c.I * ib,
-                                hipMemcpyDeviceToHost, ctx->stream));
+        ctx->download(static_cast<char*>(indices) + i0 * ib, c.indices, c.I * ib, ctx->stream);
       v0 += c.V;
       i0 += c.I;
     }

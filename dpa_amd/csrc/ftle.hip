@@ -156,9 +156,7 @@ DPAH_EXPORT int dpah_ftle_lattice(dpah_context* ctx, const dpah_field* field, co
     uint64_t m[3];
     estimate(ctx, field->shape, offset, spacing, stride, iterations, step_size, nullptr, &lat,
              finals.as<float>(), count, out.as<float>(), m, nullptr);
-    if (cells)
-      DPAH_HIP(hipMemcpyAsync(ftle_out, out.ptr, cells * sizeof(float), hipMemcpyDeviceToHost,
-                              ctx->stream));
+    ctx->download(ftle_out, out.ptr, cells * sizeof(float), ctx->stream);
     ctx->sync();
   });
 }

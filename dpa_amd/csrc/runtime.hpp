@@ -178,6 +178,17 @@ struct dpah_context {
   void* pinned = nullptr;
   uint64_t pinned_bytes = 0;
   void* host_staging(uint64_t bytes);   // nullptr above kPinnedStagingMax
+  // Device-to-host copy on stream s that returns once the bytes are in dst
+  // (call with mu held).  hipMemcpyAsync into pageable memory ran at 17 GB/s
+  // (7.9 ms for 128 MiB into pages already touched); a pinned destination
+  // takes one DMA (2.4 ms).  Pageable destinations of kStagedMin bytes and
+  // more go through the pinned staging in kStagedChunk chunks, two in flight,
+  // each chunk copied out by kStagedThreads host threads while the next one
+  // transfers (2.7 ms).
+  static constexpr uint64_t kStagedMin = 4ull << 20;
+  static constexpr uint64_t kStagedChunk = 16ull << 20;
+  static constexpr int kStagedThreads = 4;
+  void download(void* dst, const void* src, uint64_t bytes, hipStream_t s);
   void* alloc(uint64_t bytes);
   void release(void* p, uint64_t bytes);
   hipEvent_t take_event();

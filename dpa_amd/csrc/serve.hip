@@ -1136,8 +1136,7 @@ DPAH_EXPORT int dpah_result_ftle(const dpah_result* r, float* out, uint64_t* cou
     } else if (out && r->ftle_count) {
       std::lock_guard<std::recursive_mutex> lk(r->ctx->mu);
       DPAH_HIP(hipSetDevice(r->ctx->device));
-      DPAH_HIP(hipMemcpyAsync(out, r->d_ftle, r->ftle_count * sizeof(float), hipMemcpyDeviceToHost,
-                              r->ctx->stream));
+      r->ctx->download(out, r->d_ftle, r->ftle_count * sizeof(float), r->ctx->stream);
       r->ctx->sync();
     }
   });
