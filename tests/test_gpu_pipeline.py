@@ -85,6 +85,69 @@ def test_pipeline_curves_and_ftle(tmp_path):
     assert sum(rec["time"] for rec in rounds) <= total * (1 + 1e-9)
 
 
+def test_pipeline_random_seeds(tmp_path):
+    """seed_generation_count: uniform_seed_generator::generate_random
+    (uniform_seed_generator.cpp:46-80) draws the seeds in the boundary box
+    clipped to the domain.  dpa seeds its generator from std::random_device,
+    so no fixed vector pins the draws; what is checked: one fixed
+    seed_generation_random_seed repeats the output bit for bit and another
+    changes it, every curve starts inside the clipped box, the rounds hold
+    500 curves in rounds of at most 200, and each curve equals the oracle's
+    recorded particle started at its first vertex."""
+    say("case pipeline random seeds: ABC 24^3, 500 random seeds in a box, 3 rounds, recorded")
+    field_path = tmp_path / "abc.h5"
+    assert subprocess.run([str(CLI), "--make-abc-field", str(field_path), "24", "24", "24"],
+                          capture_output=True).returncode == 0
+    box_lo, box_hi = [0.5, 0.25, 0.75], [4.0, 3.0, 100.0]
+
+    def run(random_seed, name):
+        cfg = {
+            "input_dataset_filepath": str(field_path), "input_dataset_name": "Data3",
+            "input_dataset_spacing_name": "spacing", "seed_generation_count": "500",
+            "seed_generation_random_seed": str(random_seed),
+            "seed_generation_iterations": "60",
+            "seed_generation_boundaries": {"minimum": box_lo, "maximum": box_hi},
+            "particle_advector_particles_per_round": "200",
+            "particle_advector_load_balancer": "none",
+            "particle_advector_integrator": "runge_kutta_4",
+            "particle_advector_step_size": 0.01, "particle_advector_gather_particles": False,
+            "particle_advector_record": True, "estimate_ftle": False,
+            "output_dataset_filepath": str(tmp_path / f"{name}.h5")}
+        (tmp_path / f"{name}.json").write_text(json.dumps(cfg))
+        r = subprocess.run([str(CLI), str(tmp_path / f"{name}.json")], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        out = tmp_path / f"{name}.rank_0.h5"
+        names = h5util.datasets(out)
+        rounds = []
+        while f"vertices_{len(rounds)}" in names:
+            rounds.append(h5util.read(out, f"/vertices_{len(rounds)}", np.float32).reshape(-1, 3))
+        return rounds
+
+    a, b, c = run(7, "a"), run(7, "b"), run(8, "c")
+    assert len(a) == len(b) and all(np.array_equal(x.view(np.uint32), y.view(np.uint32))
+                                    for x, y in zip(a, b))
+    assert not np.array_equal(a[0], c[0])
+    field, h = W.abc_field((24, 24, 24))
+    size = (h * f32(24)).astype(f32)
+    lo = np.maximum(np.array(box_lo, f32), np.zeros(3, f32))
+    hi = np.minimum(np.array(box_hi, f32), size)
+    fld = po.Field(field, (0, 0, 0), h)
+    per_round = []
+    for v in a:
+        ends = np.nonzero(np.all(v == -2.0, axis=1))[0]   # each curve ends in a terminal vertex
+        assert len(ends) and ends[-1] == len(v) - 1
+        starts = np.concatenate([[0], ends[:-1] + 1])
+        per_round.append(len(starts))
+        for s0, e in zip(starts, ends):
+            curve = v[s0:e + 1]
+            seed = curve[0]
+            assert np.all(seed >= lo) and np.all(seed <= hi), seed
+            want = po.advect_particle(fld, seed, 60, f32(0.01))["vertices"]
+            assert np.array_equal(curve.view(np.uint32), want.view(np.uint32))
+    assert sum(per_round) == 500 and max(per_round) == 200 and len(per_round) == 3
+
+
 def parse_benchmark(path):
     """The reading of utility/python/benchmark_parser.py:29-56: rows keyed by
     rank and name, total_time per rank, round.<index>.<kind> per round."""
