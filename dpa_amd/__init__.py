@@ -69,7 +69,7 @@ class _Tuning(C.Structure):
                 ("parts", C.c_int), ("refill", C.c_int), ("cells", C.c_int),
                 ("fastdiv", C.c_int), ("cell_brick", C.c_int), ("order_brick", C.c_int),
                 ("lattice_brick", C.c_int), ("slot_stage", C.c_int),
-                ("lattice_group", C.c_int)]
+                ("lattice_group", C.c_int), ("node_brick", C.c_int)]
 
 
 def _tuning_dict(t) -> dict:

@@ -506,6 +506,50 @@ static void build_cells(dpah_context* ctx, dpah_field* f) {
   v.cell_bytes32 = bytes < (1ull << 32) && plane < (1ull << 24) ? (uint32_t)bytes : 0u;
 }
 
+// One thread per node of every slice: the node's 12 bytes to its place in the
+// node bricks (FieldView::nbrick).
+static __global__ __launch_bounds__(256) void k_build_nbrick(FieldView v, float* out,
+                                                             uint64_t total) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const uint64_t t = i / (uint64_t)v.slice, r = i - t * (uint64_t)v.slice;
+  const uint32_t n12 = (uint32_t)v.n1 * (uint32_t)v.n2;
+  const uint32_t i0 = (uint32_t)(r / n12), q = (uint32_t)(r - (uint64_t)i0 * n12);
+  const uint32_t i1 = q / (uint32_t)v.n2, i2 = q - i1 * (uint32_t)v.n2;
+  const uint64_t brick = ((uint64_t)(i0 >> 2) * v.nb1 + (i1 >> 2)) * v.nb2 + (i2 >> 2);
+  const uint64_t local = ((i0 & 3u) << 4) | ((i1 & 3u) << 2) | (i2 & 3u);
+  float* o = (float*)((char*)out + t * v.nbrick_slice + brick * 768 + local * 12);
+  const float* p = v.data + 3 * i;
+  o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+}
+
+// The node bricks of a field, when the tuning asks for them and they fit:
+// every slice's bricks below 2^32 bytes in all and a slice's bricks numbering
+// below 2^24 (the 24-bit offsets of nbrick_byte24), a tenth of the memory
+// budget at most.
+static void build_nbrick(dpah_context* ctx, dpah_field* f) {
+  FieldView& v = f->view;
+  if (!ctx->tune.node_brick) return;
+  const uint64_t nb0 = ((uint64_t)v.n0 + 3) / 4, nb1 = ((uint64_t)v.n1 + 3) / 4,
+                 nb2 = ((uint64_t)v.n2 + 3) / 4;
+  const uint64_t per_slice = nb0 * nb1 * nb2 * 768;
+  const uint64_t bytes = per_slice * (uint64_t)v.nt;
+  if (nb0 * nb1 * nb2 >= (1ull << 24) || bytes >= (1ull << 32) ||
+      bytes > ctx->memory_budget / 10)
+    return;
+  v.nb1 = (uint32_t)nb1;
+  v.nb2 = (uint32_t)nb2;
+  v.nbrick_slice = (uint32_t)per_slice;
+  f->d_nbrick = static_cast<float*>(ctx->alloc(bytes));
+  f->nbrick_bytes = bytes;
+  const uint64_t total = (uint64_t)v.slice * (uint64_t)v.nt;
+  hipLaunchKernelGGL(k_build_nbrick, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     ctx->stream, v, f->d_nbrick, total);
+  DPAH_HIP(hipGetLastError());
+  ctx->sync();
+  v.nbrick = (const char*)f->d_nbrick;
+}
+
 static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shape[3], uint64_t nt,
                        const float offset[3], const float spacing[3], float t_off, float t_sp,
                        dpah_field** out) {
@@ -552,10 +596,14 @@ static void make_field(dpah_context* ctx, const float* xyzv, const uint64_t shap
   v.node_bytes32 =
       bytes < (1ull << 32) && 12 * shape[1] * shape[2] < (1ull << 24) ? (uint32_t)bytes : 0u;
   v.cell_bytes32 = 0;
+  v.nbrick = nullptr;
+  v.nb1 = v.nb2 = v.nbrick_slice = 0;
   try {
     set_division(ctx, v);
     build_cells(ctx, f);
+    build_nbrick(ctx, f);
   } catch (...) {
+    if (f->d_nbrick) ctx->release(f->d_nbrick, f->nbrick_bytes);
     if (f->d_cells) ctx->release(f->d_cells, f->cell_bytes);
     ctx->release(f->d_data, f->bytes);
     delete f;
@@ -589,6 +637,7 @@ DPAH_EXPORT int dpah_field_release(dpah_field* field) {
     DPAH_HIP(hipStreamSynchronize(ctx->stream));
     ctx->release(field->d_data, field->bytes);
     if (field->d_cells) ctx->release(field->d_cells, field->cell_bytes);
+    if (field->d_nbrick) ctx->release(field->d_nbrick, field->nbrick_bytes);
     delete field;
   });
 }
@@ -948,6 +997,7 @@ DPAH_EXPORT int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out) {
     out->lattice_brick = ctx->tune.lattice_brick;
     out->slot_stage = ctx->tune.slot_stage;
     out->lattice_group = ctx->tune.lattice_group;
+    out->node_brick = ctx->tune.node_brick;
   });
 }
 
@@ -973,6 +1023,7 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(t->lattice_group == 0 || t->lattice_group == 1 || t->lattice_group == 2 ||
                      t->lattice_group == 4 || t->lattice_group == 8 || t->lattice_group == 16,
                  "lattice_group must be 0, 1, 2, 4, 8 or 16");
+    DPAH_REQUIRE(t->node_brick == 0 || t->node_brick == 1, "node_brick must be 0 or 1");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     ctx->tune.block = t->block;
     ctx->tune.blocks_per_cu = t->blocks_per_cu;
@@ -986,6 +1037,7 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     ctx->tune.lattice_brick = t->lattice_brick;
     ctx->tune.slot_stage = t->slot_stage;
     ctx->tune.lattice_group = t->lattice_group;
+    ctx->tune.node_brick = t->node_brick;
     ctx->tuned.clear();   // explicit settings replace autotuned ones
   });
 }

@@ -56,6 +56,17 @@ struct FieldView {
   // 2^32 and every factor of the offset's products below 2^24 (node_byte24,
   // cell_byte24); 0 otherwise.
   uint32_t node_bytes32, cell_bytes32;
+  // Optional node bricks: the nodes again, in bricks of 4^3 nodes (768 B)
+  // stored brick after brick in C order over the bricks, nodes in C order
+  // inside a brick (the grid padded to whole bricks; padding nodes are never
+  // read).  A cell's eight corners then lie in two or three lines instead of
+  // the four to eight of four C-order rows, and a wave's neighbouring
+  // particles share them: read by launches that sample nodes (lattice
+  // seeds).  nb1, nb2: bricks along axes 1 and 2; nbrick_slice: bytes per
+  // time slice.  Present only when all slices fit 2^32 bytes and the bricks
+  // of a slice number below 2^24 (nbrick_byte24); null otherwise.
+  const char* nbrick;
+  uint32_t nb1, nb2, nbrick_slice;
 };
 
 // Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format,
@@ -332,6 +343,40 @@ __device__ __forceinline__ float4 buf_load4(BufferRes r, uint32_t voff, uint32_t
 }
 #endif
 
+// Byte offset of node (c0, c1, c2) in a slice of the node bricks.
+__device__ __forceinline__ uint32_t nbrick_byte24(const FieldView& f, uint32_t c0,
+                                                           uint32_t c1, uint32_t c2) {
+  const uint32_t brick = mul24(c0 >> 2, f.nb1 * f.nb2) + mul24(c1 >> 2, f.nb2) + (c2 >> 2);
+  const uint32_t local = ((c0 & 3u) << 4) | ((c1 & 3u) << 2) | (c2 & 3u);
+  return mul24(brick, 768u) + local * 12u;
+}
+
+// One node (12 bytes) at a 32-bit byte offset from a uniform base.
+__device__ __forceinline__ V3 load_node(const char* base, uint32_t off) {
+  const float* p = (const float*)(base + off);
+  return V3{p[0], p[1], p[2]};
+}
+
+// The eight corners of cell (c0, c1, c2) from the node bricks at byte offset
+// `at` of the slice holding them: per axis the step to the next node stays
+// inside the brick, or goes to the next brick's first node on that axis.
+__device__ __forceinline__ void load_nbrick_cell(const FieldView& f, const char* b, uint32_t at,
+                                                 uint32_t c0, uint32_t c1, uint32_t c2,
+                                                 V3& L0, V3& L1, V3& L2, V3& L3, V3& L4,
+                                                 V3& L5, V3& L6, V3& L7) {
+  const uint32_t d2 = (c2 & 3u) != 3u ? 12u : 768u - 36u;
+  const uint32_t d1 = (c1 & 3u) != 3u ? 48u : f.nb2 * 768u - 144u;
+  const uint32_t d0 = (c0 & 3u) != 3u ? 192u : f.nb1 * f.nb2 * 768u - 576u;
+  L0 = load_node(b, at);
+  L1 = load_node(b, at + d2);
+  L2 = load_node(b, at + d1);
+  L3 = load_node(b, at + (d1 + d2));
+  L4 = load_node(b, at + d0);
+  L5 = load_node(b, at + (d0 + d2));
+  L6 = load_node(b, at + (d0 + d1));
+  L7 = load_node(b, at + (d0 + d1 + d2));
+}
+
 // One node row of a cell (two nodes along axis 2, 24 bytes) at a 32-bit byte
 // offset from a uniform base: the loads take the base in scalar registers and
 // the offset in one vector register.
@@ -384,6 +429,10 @@ __device__ __forceinline__ V3 trilinear(const FieldView& f, const float* slice,
       const float4* rec = f.idx32 ? f.cells + 8u * cell_record32(f, 0u, c0, c1, c2)
                                   : f.cells + 8 * cell_record(f, 0, c0, c1, c2);
       load_cell(rec, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
+    } else if (f.nbrick) {   // slice 0, like every caller
+      const uint32_t u0 = (uint32_t)c0, u1 = (uint32_t)c1, u2 = (uint32_t)c2;
+      load_nbrick_cell(f, f.nbrick, nbrick_byte24(f, u0, u1, u2), u0, u1, u2, cc.L0, cc.L1,
+                       cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
     } else if (f.node_bytes32) {   // (slice is f.data: every caller samples slice 0)
       // one 32-bit byte offset per reload from the array's uniform base (a
       // buffer resource here would add scalar registers that lattice
@@ -451,6 +500,13 @@ __device__ __forceinline__ V3 quadlinear(const FieldView& f, const Axis& a0, con
       const float4* r1 = r0 + 8 * f.cell_slice;
       load_cell(r0, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5, cc.L6, cc.L7);
       load_cell(r1, cc.M0, cc.M1, cc.M2, cc.M3, cc.M4, cc.M5, cc.M6, cc.M7);
+    } else if (f.nbrick) {
+      const uint32_t u0 = (uint32_t)c0, u1 = (uint32_t)c1, u2 = (uint32_t)c2;
+      const uint32_t v = (uint32_t)ct * f.nbrick_slice + nbrick_byte24(f, u0, u1, u2);
+      load_nbrick_cell(f, f.nbrick, v, u0, u1, u2, cc.L0, cc.L1, cc.L2, cc.L3, cc.L4, cc.L5,
+                       cc.L6, cc.L7);
+      load_nbrick_cell(f, f.nbrick, v + f.nbrick_slice, u0, u1, u2, cc.M0, cc.M1, cc.M2, cc.M3,
+                       cc.M4, cc.M5, cc.M6, cc.M7);
     } else if (f.node_bytes32) {
       const char* b = (const char*)f.data;
       const uint32_t sb = 12u * (uint32_t)f.slice;   // bytes per slice

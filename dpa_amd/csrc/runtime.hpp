@@ -115,6 +115,11 @@ struct Tuning {
   // 1 persistent with one-slot runs, 2 persistent with 8-slot blocks staged in
   // LDS, 3 one thread per row (a grid over the rows) with one-slot runs
   int slot_stage = 0;
+  // 1: fields created afterwards also keep their nodes in 4^3-node bricks
+  // (the node bytes again) when they fit, read by launches that sample nodes
+  // (FTLE +-1/2-cell lattice 28.8 -> 24.6 ms, dpa lattice 10.4 -> 9.4 ms);
+  // 0: C-order nodes only
+  int node_brick = 1;
 };
 
 // Device counters of a context: [1] particle-steps, [2] FTLE misses,
@@ -216,6 +221,8 @@ struct dpah_field {
   uint64_t bytes = 0;
   float4* d_cells = nullptr;      // cell layout (FieldView::cells), or null
   uint64_t cell_bytes = 0;
+  float* d_nbrick = nullptr;      // node bricks (FieldView::nbrick), or null
+  uint64_t nbrick_bytes = 0;
   dpah::FieldView view{};
   uint64_t shape[3] = {0, 0, 0};
   uint64_t time_steps = 1;

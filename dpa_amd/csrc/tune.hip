@@ -42,7 +42,8 @@ void save_cache(const dpah_context* ctx, const std::string& key, const Tuning& t
   out << ctx->arch << " " << ctx->cu_count << " " << key << " " << t.block << " "
       << t.blocks_per_cu << " " << t.order << " " << t.parts << " " << t.refill << " " << t.cells
       << " " << t.fastdiv << " " << t.cell_brick << " " << t.order_brick << " " << ms << " "
-      << t.lattice_brick << " " << t.slot_stage << " " << t.lattice_group << "\n";
+      << t.lattice_brick << " " << t.slot_stage << " " << t.lattice_group << " " << t.node_brick
+      << "\n";
   if (!out) throw Error(DPAH_ERR_INVALID, "cannot write the autotune cache " + ctx->autotune_path);
 }
 
@@ -67,6 +68,8 @@ void load_autotune_cache(dpah_context* ctx) {
     int lb = 0, st = 0, lg = 0;
     if (ss >> lb >> st) { t.lattice_brick = lb; t.slot_stage = st; }
     if (ss >> lg) t.lattice_group = lg;
+    int nbk = 1;
+    if (ss >> nbk) t.node_brick = nbk;
     if (arch != ctx->arch || cus != ctx->cu_count) continue;
     const bool ok = t.block >= 64 && t.block <= 256 && t.block % 64 == 0 &&
                     t.blocks_per_cu >= 0 && t.blocks_per_cu <= 32 && (t.order == 0 || t.order == 1) &&
@@ -78,7 +81,8 @@ void load_autotune_cache(dpah_context* ctx) {
                      t.lattice_brick == 8) &&
                     t.slot_stage >= 0 && t.slot_stage <= 3 &&
                     (t.lattice_group == 0 || t.lattice_group == 1 || t.lattice_group == 2 ||
-                     t.lattice_group == 4 || t.lattice_group == 8 || t.lattice_group == 16);
+                     t.lattice_group == 4 || t.lattice_group == 8 || t.lattice_group == 16) &&
+                    (t.node_brick == 0 || t.node_brick == 1);
     if (ok) ctx->tuned[key] = t;
   }
 }
@@ -96,6 +100,7 @@ void tuning_to_abi(const Tuning& t, dpah_tuning* out) {
   out->lattice_brick = t.lattice_brick;
   out->slot_stage = t.slot_stage;
   out->lattice_group = t.lattice_group;
+  out->node_brick = t.node_brick;
 }
 
 // Times `launch` (one launch on the context's stream) under every candidate

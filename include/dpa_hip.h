@@ -425,6 +425,12 @@ typedef struct dpah_tuning {
                           bricks per axis one after another, bricks inside a
                           group: 2, 4, 8 or 16; 1: no groups; 0: 2 for
                           lattices of points, none for auxiliary ones         */
+  int node_brick;      /* 1: fields created afterwards also keep their nodes
+                          in bricks of 4^3 nodes (the node bytes again) when
+                          all slices fit 2^32 bytes and a tenth of the memory
+                          budget; launches that sample nodes (lattice seeds,
+                          fields without a cell layout) read them.  0: C-order
+                          nodes only                                          */
 } dpah_tuning;
 int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
 /* Sets the context's tuning and drops autotuned entries. */

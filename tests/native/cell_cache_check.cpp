@@ -1,7 +1,9 @@
 // Host build of the device sampling code: a sample through a warm cell cache
 // (kept across steps) must equal, bit for bit, a sample through an empty cache
-// (full eight-corner load) that divides by the spacings, for steady and
-// time-variant fields, over random walks that cross cells in every direction.
+// (full eight-corner load from the C-order nodes) that divides by the
+// spacings, for steady and time-variant fields, over random walks that cross
+// cells in every direction, with every layout a reload can read: C-order
+// nodes, cell records (C order and bricks), node bricks.
 // The warm sample runs each division mode a launch can take: plain division,
 // the reciprocal with one exact-residual correction (kDivFast, on power-of-two
 // and on a non-power-of-two grid), and on grids whose spacings are all powers
@@ -31,6 +33,7 @@ static long walk(const FieldView& f, std::mt19937& rng, long& samples, long& shi
   // 64-bit pointer offsets (no cell layout, no buffer offsets)
   FieldView plain = f;
   plain.cells = nullptr;
+  plain.nbrick = nullptr;
   plain.idx32 = 0;
   plain.node_bytes32 = plain.cell_bytes32 = 0;
   std::uniform_real_distribution<float> u(-1.0f, 1.0f);
@@ -71,6 +74,7 @@ template <int kDiv>
 static long probes(const FieldView& e, long& samples) {
   FieldView plain = e;   // dividing, pointer reads of the nodes
   plain.cells = nullptr;
+  plain.nbrick = nullptr;
   plain.idx32 = 0;
   plain.node_bytes32 = plain.cell_bytes32 = 0;
   const float tiny[] = {0.0f, -0.0f, 1e-45f, -1e-45f, 1e-38f, -1e-38f, 1e-30f, -1e-30f,
@@ -211,6 +215,29 @@ static std::vector<float4> build_cells(FieldView& f, const std::vector<float>& n
   return cells;
 }
 
+// The node bricks of a field as the library's k_build_nbrick writes them:
+// bricks of 4^3 nodes (768 bytes) in C order over the bricks, nodes in C order
+// inside a brick, one slice after another; padding nodes left at zero.
+static std::vector<float> build_nbrick(FieldView& f, const std::vector<float>& nodes) {
+  const uint32_t nb0 = ((uint32_t)f.n0 + 3) / 4, nb1 = ((uint32_t)f.n1 + 3) / 4,
+                 nb2 = ((uint32_t)f.n2 + 3) / 4;
+  f.nb1 = nb1;
+  f.nb2 = nb2;
+  f.nbrick_slice = nb0 * nb1 * nb2 * 768u;
+  std::vector<float> out((size_t)f.nbrick_slice / 4 * (size_t)f.nt, 0.0f);
+  for (int32_t t = 0; t < f.nt; ++t)
+    for (int32_t i0 = 0; i0 < f.n0; ++i0)
+      for (int32_t i1 = 0; i1 < f.n1; ++i1)
+        for (int32_t i2 = 0; i2 < f.n2; ++i2) {
+          const size_t brick = ((size_t)(i0 / 4) * nb1 + i1 / 4) * nb2 + i2 / 4;
+          const size_t at = (size_t)t * f.nbrick_slice / 4 +
+                            3 * (brick * 64 + (size_t)(i0 % 4) * 16 + (i1 % 4) * 4 + i2 % 4);
+          const size_t node = (((size_t)t * f.n0 + i0) * f.n1 + i1) * f.n2 + i2;
+          for (int c = 0; c < 3; ++c) out[at + c] = nodes[3 * node + c];
+        }
+  return out;
+}
+
 int main() {
   const int n0 = 19, n1 = 23, n2 = 29, nt = 5;
   std::vector<float> data(3ull * n0 * n1 * n2 * nt);
@@ -308,6 +335,22 @@ int main() {
       c.cell_bytes32 = 0;
       c.idx32 = 1;
       diffs += walk<false, kDivFast>(c, rng, samples, shifts);
+    }
+    // node bricks (4^3 nodes per brick; the grid's sides are not multiples of
+    // 4, so cells at the brick faces, the padded bricks and the last cell are
+    // all walked), steady and time-variant
+    {
+      FieldView nb = f;
+      const std::vector<float> bricks = build_nbrick(nb, data);
+      nb.nbrick = (const char*)bricks.data();
+      diffs += walk<false, kDivFast>(nb, rng, samples, shifts);
+      diffs += walk<false, kDivide>(nb, rng, samples, shifts);
+      diffs += probes<kDivFast>(nb, samples);
+      FieldView nbt = g;
+      const std::vector<float> tbricks = build_nbrick(nbt, data);
+      nbt.nbrick = (const char*)tbricks.data();
+      diffs += walk<true, kDivide>(nbt, rng, samples, shifts);
+      diffs += walk<true, kDivFast>(nbt, rng, samples, shifts);
     }
     // nodes through 32-bit float offsets, steady and time-variant
     FieldView i32 = f;

@@ -44,11 +44,16 @@ def make_field(ctx, data, unsteady):
     return ctx.field(data, OFFSET, SPACING)
 
 
+@pytest.mark.parametrize("node_brick", [0, 1])
 @pytest.mark.parametrize("cells", [0, 1, 2])
 @pytest.mark.parametrize("unsteady", [False, True])
 @pytest.mark.parametrize("integrator", [po.DPA_RK4, po.RK4])
-def test_layout_matches_oracle(ctx, cells, unsteady, integrator):
-    say(f"case layout cells={cells} unsteady={unsteady} integrator={integrator}")
+def test_layout_matches_oracle(ctx, cells, node_brick, unsteady, integrator):
+    # cells 0: the launches read the nodes, in C order (node_brick 0) or in
+    # 4^3-node bricks (node_brick 1; the 40^3 and 24^3 grids pad their last
+    # bricks)
+    say(f"case layout cells={cells} node_brick={node_brick} unsteady={unsteady} "
+        f"integrator={integrator}")
     if unsteady:
         data = W.unsteady_vortex_field(24, time_steps=5, count=6)
         ofld = po.Field(data, OFFSET, SPACING, 0.5, 0.5)
@@ -61,7 +66,7 @@ def test_layout_matches_oracle(ctx, cells, unsteady, integrator):
     seeds = W.uniform_seeds(3000, lo, hi, seed_offset=900 + cells)
     rem = np.full(len(seeds), 120, np.uint64)
     dt, t0 = (f32(0.01) if unsteady else f32(0.2)), f32(0.75)
-    with tuned(ctx, cells=cells):
+    with tuned(ctx, cells=cells, node_brick=node_brick):
         fld = make_field(ctx, data, unsteady)
         try:
             res = ctx.advect(fld, seeds, remaining=rem, particles_per_round=1000, step_size=dt,
@@ -83,8 +88,8 @@ def test_ftle_aux_same_on_both_layouts(ctx, unsteady):
             else W.vortex_field(40, count=12))
     shape = (6, 9, 11)
     out = []
-    for cells in (0, 2):
-        with tuned(ctx, cells=cells):
+    for cells, node_brick in ((0, 0), (0, 1), (2, 0)):   # C-order nodes, node bricks, cells
+        with tuned(ctx, cells=cells, node_brick=node_brick):
             fld = make_field(ctx, data, unsteady)
             d = ctx.alloc(4 * int(np.prod(shape)))
             try:
@@ -95,8 +100,27 @@ def test_ftle_aux_same_on_both_layouts(ctx, unsteady):
             finally:
                 ctx.free(d)
                 fld.release()
-    assert out[0][1] == out[1][1] and out[0][1] > 0
+    assert out[0][1] == out[1][1] == out[2][1] and out[0][1] > 0
     assert np.array_equal(bits(out[0][0]), bits(out[1][0]))
+    assert np.array_equal(bits(out[0][0]), bits(out[2][0]))
+
+
+def test_ftle_lattice_same_on_node_layouts(ctx):
+    # dpa's lattice FTLE (every node a seed) through C-order nodes and node
+    # bricks: the same volume bit for bit on a 37 x 42 x 29 grid, whose sides
+    # leave partial bricks on every axis
+    say("case ftle lattice node layouts")
+    data = W.vortex_field(42, count=9)[:37, :, :29].copy()
+    vols = []
+    for node_brick in (0, 1):
+        with tuned(ctx, cells=1, node_brick=node_brick):
+            fld = ctx.field(data, OFFSET, SPACING)
+            try:
+                vols.append(ctx.ftle_lattice(fld, (1, 1, 1), 80, f32(0.2)))
+            finally:
+                fld.release()
+    assert vols[0][1] == vols[1][1] and vols[0][1] > 0
+    assert np.array_equal(bits(vols[0][0]), bits(vols[1][0]))
 
 
 @pytest.mark.parametrize("cell_brick", [2, 3, 5])
