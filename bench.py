@@ -917,9 +917,10 @@ def main() -> int:
                     help="whole-lattice launches: points per brick edge")
     ap.add_argument("--lattice-group", type=int, choices=[0, 1, 2, 4, 8, 16], default=None,
                     help="whole-lattice launches: bricks per group edge")
-    ap.add_argument("--node-brick", type=int, choices=[0, 1], default=None,
-                    help="1: fields also keep their nodes in 4^3-node bricks (read by "
-                         "launches that sample nodes); 0: C-order nodes only")
+    ap.add_argument("--node-brick", type=int, choices=[0, 1, 2], default=None,
+                    help="1: steady fields also keep their nodes in 4^3-node bricks (read by "
+                         "launches that sample nodes); 2: time-variant fields too; 0: C-order "
+                         "nodes only")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="skip the hipGraph replay leg")
     ap.add_argument("--autotune", dest="autotune", action="store_true", default=False,

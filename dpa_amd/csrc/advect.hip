@@ -18,13 +18,20 @@ void launch_t(dpah_context* ctx, const Tuning& T, const AdvectArgs& a, bool pers
   if (persistent) {
     const int block = T.block;
     // recorded launches: 6 (each workgroup holds a 24 KB slot stage, and 6 fill
-    // the 160 KB of LDS of a CU); endpoints: 4 for seed arrays, 3 for lattices
-    // of steady fields (their neighbours' node rows hit in the L2), 4 for
+    // the 160 KB of LDS of a CU); endpoints: 4 for seed arrays; for lattices
+    // of steady fields 3 on C-order nodes (their neighbours' node rows hit in
+    // the L2), and on node bricks, whose lines hold more of what neighbouring
+    // particles read, 5 for auxiliary lattices and 4 for lattices of points
+    // (FTLE on ABC 512x512x128: +-1/2-cell 24.75 / 22.08 / 21.66 / 21.97 ms at
+    // 3 / 4 / 5 / 6, dpa lattice 9.52 / 9.14 / 9.41 at 3 / 4 / 5); 4 for
     // lattices of time-variant fields, whose reloads read two slices (a
     // 256x256x16-point FTLE slab on a 256^3 x 16-slice field: 3.32 ms at 4,
     // 3.43 at 5, 3.55-3.70 at 3)
     const int bpc = T.blocks_per_cu ? T.blocks_per_cu
-                  : kRecord ? 6 : a.seed_kind == kSeedArray || kUnsteady ? 4 : 3;
+                  : kRecord ? 6
+                  : a.seed_kind == kSeedArray || kUnsteady ? 4
+                  : !a.f.nbrick ? 3
+                  : a.seed_kind == kSeedLattice ? 4 : 5;
     uint64_t grid = (uint64_t)ctx->cu_count * bpc;
     const uint64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
@@ -167,16 +174,20 @@ bool ordered_launch(const dpah_context* ctx, const Tuning& T, const AdvectLaunch
 // advection, 39.7 ms in C order).  Whole lattices in one launch only, since
 // the walk is over the whole lattice and the outputs are indexed by lattice
 // point.
-uint32_t lattice_brick(const Tuning& T, int seed_kind) {
+// nbrick: the field's nodes are read from node bricks (lattices of points walk
+// smaller bricks in larger groups there: dpa lattice FTLE at 4 workgroups per
+// CU 9.14 ms with 4-point bricks in groups of 2, 8.91 ms with 2-point bricks
+// in groups of 4).
+uint32_t lattice_brick(const Tuning& T, int seed_kind, bool nbrick) {
   if (T.lattice_brick) return (uint32_t)T.lattice_brick;
-  return seed_kind == kSeedLattice ? kLatticeBrick
+  return seed_kind == kSeedLattice ? (nbrick ? kLatticeBrickNb : kLatticeBrick)
          : seed_kind == kSeedAuxShared ? kAuxSharedBrick : kAuxBrick;
 }
 
 // Bricks per group edge of the grouped walk; 1: no groups.
-uint32_t lattice_group(const Tuning& T, int seed_kind) {
+uint32_t lattice_group(const Tuning& T, int seed_kind, bool nbrick) {
   if (T.lattice_group) return (uint32_t)T.lattice_group;
-  return seed_kind == kSeedLattice ? kLatticeGroup : 1u;
+  return seed_kind == kSeedLattice ? (nbrick ? kLatticeGroupNb : kLatticeGroup) : 1u;
 }
 
 bool brick_launch(const Tuning& T, const AdvectLaunch& L) {
@@ -184,7 +195,8 @@ bool brick_launch(const Tuning& T, const AdvectLaunch& L) {
     return false;
   const uint64_t points = L.lat.d0 * L.lat.d1 * L.lat.d2;
   // the walk's 32-bit arithmetic: positions and a whole slab of bricks / groups
-  if ((uint64_t)lattice_brick(T, L.seed_kind) * lattice_group(T, L.seed_kind) * L.lat.d1 *
+  const bool nb = L.f->nbrick != nullptr;
+  if ((uint64_t)lattice_brick(T, L.seed_kind, nb) * lattice_group(T, L.seed_kind, nb) * L.lat.d1 *
           L.lat.d2 >= (1ull << 32))
     return false;
   if (L.seed_kind == kSeedLattice) return L.n == points && L.n < (1ull << 32);
@@ -312,8 +324,8 @@ void launch_advect(dpah_context* ctx, const AdvectLaunch& L) {
   if (ordered_launch(ctx, T, L)) {
     order_queue(ctx, T, a, q, L.order_scratch, s);
   } else if (brick_launch(T, L)) {
-    a.lat.brick = lattice_brick(T, L.seed_kind);
-    const uint32_t g = lattice_group(T, L.seed_kind);
+    a.lat.brick = lattice_brick(T, L.seed_kind, a.f.nbrick != nullptr);
+    const uint32_t g = lattice_group(T, L.seed_kind, a.f.nbrick != nullptr);
     a.lat.group = g > 1 ? a.lat.brick * g : 0u;
     set_parts(a, (uint32_t)std::max(1, std::min(T.parts, kMaxParts)));
   }

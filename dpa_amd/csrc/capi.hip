@@ -523,13 +523,13 @@ static __global__ __launch_bounds__(256) void k_build_nbrick(FieldView v, float*
   o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
 }
 
-// The node bricks of a field, when the tuning asks for them and they fit:
-// every slice's bricks below 2^32 bytes in all and a slice's bricks numbering
-// below 2^24 (the 24-bit offsets of nbrick_byte24), a tenth of the memory
-// budget at most.
+// The node bricks of a field, when the tuning asks for them (node_brick 1:
+// steady fields, 2: every field) and they fit: every slice's bricks below
+// 2^32 bytes in all and a slice's bricks numbering below 2^24 (the 24-bit
+// offsets of nbrick_byte24), a tenth of the memory budget at most.
 static void build_nbrick(dpah_context* ctx, dpah_field* f) {
   FieldView& v = f->view;
-  if (!ctx->tune.node_brick) return;
+  if (ctx->tune.node_brick == 0 || (v.nt > 1 && ctx->tune.node_brick < 2)) return;
   const uint64_t nb0 = ((uint64_t)v.n0 + 3) / 4, nb1 = ((uint64_t)v.n1 + 3) / 4,
                  nb2 = ((uint64_t)v.n2 + 3) / 4;
   const uint64_t per_slice = nb0 * nb1 * nb2 * 768;
@@ -1023,7 +1023,7 @@ DPAH_EXPORT int dpah_set_tuning(dpah_context* ctx, const dpah_tuning* t) {
     DPAH_REQUIRE(t->lattice_group == 0 || t->lattice_group == 1 || t->lattice_group == 2 ||
                      t->lattice_group == 4 || t->lattice_group == 8 || t->lattice_group == 16,
                  "lattice_group must be 0, 1, 2, 4, 8 or 16");
-    DPAH_REQUIRE(t->node_brick == 0 || t->node_brick == 1, "node_brick must be 0 or 1");
+    DPAH_REQUIRE(t->node_brick >= 0 && t->node_brick <= 2, "node_brick must be 0, 1 or 2");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     ctx->tune.block = t->block;
     ctx->tune.blocks_per_cu = t->blocks_per_cu;

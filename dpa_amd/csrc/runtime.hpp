@@ -101,24 +101,27 @@ struct Tuning {
   int order_brick = 0;
   // whole-lattice launches walk their points in bricks of this many points per
   // axis (2, 4 or 8); 0: by launch kind (kLatticeBrick for lattices of points,
-  // kAuxBrick for auxiliary lattices of six particles per point: FTLE ±½-cell
-  // 59.4 ms at 4, 58.4 at 2; kAuxSharedBrick for shared particles: 28.6 ms at
-  // 4, 29.6 at 2, 29.8 at 8)
+  // kLatticeBrickNb on fields read from node bricks; kAuxBrick for auxiliary
+  // lattices of six particles per point: FTLE ±½-cell 59.4 ms at 4, 58.4 at
+  // 2; kAuxSharedBrick for shared particles: 28.6 ms at 4, 29.6 at 2, 29.8 at
+  // 8 on C-order nodes, 21.85 / 22.35 / 22.74 on node bricks)
   int lattice_brick = 0;
   // and in groups of this many bricks per axis above them (2, 4, 8 or 16), so
   // that what a queue part has in flight is a compact block; 1: no groups;
   // 0: by launch kind (kLatticeGroup for lattices of points -- dpa-lattice
-  // FTLE 11.47 ms without, 10.87 with groups of 2 -- none for auxiliary
-  // lattices, which lose 2-5% with groups)
+  // FTLE 11.47 ms without, 10.87 with groups of 2 -- and kLatticeGroupNb on
+  // node bricks; none for auxiliary lattices, which lose 2-5% with groups)
   int lattice_group = 0;
   // recorded launches: 0 by launch kind (recorded_grid, slot_shift_for),
   // 1 persistent with one-slot runs, 2 persistent with 8-slot blocks staged in
   // LDS, 3 one thread per row (a grid over the rows) with one-slot runs
   int slot_stage = 0;
-  // 1: fields created afterwards also keep their nodes in 4^3-node bricks
-  // (the node bytes again) when they fit, read by launches that sample nodes
-  // (FTLE +-1/2-cell lattice 28.8 -> 24.6 ms, dpa lattice 10.4 -> 9.4 ms);
-  // 0: C-order nodes only
+  // 1: steady fields created afterwards also keep their nodes in 4^3-node
+  // bricks (the node bytes again) when they fit, read by launches that sample
+  // nodes (FTLE +-1/2-cell lattice 28.8 -> 24.6 ms, dpa lattice 10.4 -> 9.4
+  // ms); 2: time-variant fields too (the served 16-slice fields: no change at
+  // saturation, 1.10-1.11e11 steps/s either way, for 12.9 GB more); 0: C-order
+  // nodes only
   int node_brick = 1;
 };
 
@@ -303,6 +306,8 @@ constexpr uint32_t kLatticeBrick = 4;   // edge of a brick-ordered lattice queue
 constexpr uint32_t kAuxBrick = 2;       // the same for auxiliary lattices of six particles
 constexpr uint32_t kAuxSharedBrick = 4; // and of shared particles (three per point)
 constexpr uint32_t kLatticeGroup = 2;   // bricks per group edge, lattices of points
+constexpr uint32_t kLatticeBrickNb = 2; // the same two on fields read from node bricks
+constexpr uint32_t kLatticeGroupNb = 4;
 
 // Launchers (advect.hip / compact.hip).
 struct AdvectLaunch {

@@ -82,7 +82,7 @@ void load_autotune_cache(dpah_context* ctx) {
                     t.slot_stage >= 0 && t.slot_stage <= 3 &&
                     (t.lattice_group == 0 || t.lattice_group == 1 || t.lattice_group == 2 ||
                      t.lattice_group == 4 || t.lattice_group == 8 || t.lattice_group == 16) &&
-                    (t.node_brick == 0 || t.node_brick == 1);
+                    t.node_brick >= 0 && t.node_brick <= 2;
     if (ok) ctx->tuned[key] = t;
   }
 }

@@ -384,9 +384,10 @@ int dpah_cell_loads(dpah_context* ctx, uint64_t* record_loads, uint64_t* node_lo
 typedef struct dpah_tuning {
   int block;           /* threads per workgroup: 64, 128, 192 or 256          */
   int blocks_per_cu;   /* persistent grid = CUs x blocks_per_cu, 1..32; 0: by
-                          launch kind (4 endpoint seed arrays, 3 lattices of
-                          steady fields, 4 of time-variant fields, 6 recorded
-                          launches)                                            */
+                          launch kind (4 endpoint seed arrays; lattices of
+                          steady fields 3 on C-order nodes, 5 auxiliary and 4
+                          of points on node bricks; 4 lattices of time-variant
+                          fields; 6 recorded launches)                         */
   int order;           /* 1: endpoint launches of seed arrays and lattices take
                           their queue in brick Morton order, split across the
                           XCDs (auxiliary FTLE lattices are already compact;
@@ -414,7 +415,8 @@ typedef struct dpah_tuning {
                           2^order_brick cells per axis, 1..5; 0: 4 cells      */
   int lattice_brick;   /* whole-lattice launches (FTLE) walk their points in
                           bricks of this many points per axis: 2, 4 or 8;
-                          0: 4 for lattices of points, 2 for auxiliary ones   */
+                          0: lattices of points 4 (2 on node bricks),
+                          auxiliary lattices 2 (4 with shared particles)      */
   int slot_stage;      /* recorded launches: 0 by launch kind (a single
                           request's rounds one thread per row; a batch
                           persistent, its slots in 8-slot blocks staged in LDS
@@ -423,14 +425,15 @@ typedef struct dpah_tuning {
                           row with one-slot runs                              */
   int lattice_group;   /* whole-lattice launches walk groups of this many
                           bricks per axis one after another, bricks inside a
-                          group: 2, 4, 8 or 16; 1: no groups; 0: 2 for
-                          lattices of points, none for auxiliary ones         */
-  int node_brick;      /* 1: fields created afterwards also keep their nodes
-                          in bricks of 4^3 nodes (the node bytes again) when
-                          all slices fit 2^32 bytes and a tenth of the memory
-                          budget; launches that sample nodes (lattice seeds,
-                          fields without a cell layout) read them.  0: C-order
-                          nodes only                                          */
+                          group: 2, 4, 8 or 16; 1: no groups; 0: lattices of
+                          points 2 (4 on node bricks), none for auxiliary
+                          ones                                                 */
+  int node_brick;      /* 1: steady fields created afterwards also keep their
+                          nodes in bricks of 4^3 nodes (the node bytes again)
+                          when all slices fit 2^32 bytes and a tenth of the
+                          memory budget; launches that sample nodes (lattice
+                          seeds, fields without a cell layout) read them.
+                          2: time-variant fields too.  0: C-order nodes only */
 } dpah_tuning;
 int dpah_get_tuning(dpah_context* ctx, dpah_tuning* out);
 /* Sets the context's tuning and drops autotuned entries. */

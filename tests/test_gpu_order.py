@@ -106,7 +106,7 @@ def test_set_tuning_rejects_bad_shapes(ctx):
                 dict(order=2),
                 dict(parts=9), dict(parts=0), dict(cell_brick=1), dict(cell_brick=6),
                 dict(order_brick=-1), dict(order_brick=6), dict(lattice_group=3),
-                dict(lattice_group=32), dict(node_brick=2)):
+                dict(lattice_group=32), dict(node_brick=3), dict(node_brick=-1)):
         with pytest.raises(dpa_amd.DpahError):
             ctx.set_tuning(**bad)
 
